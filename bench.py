@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: direct J/K build of a 40-atom, 220-function cluster on one MI355X.
+
+One step = one direct J/K build (all ERIs screened by Schwarz bounds,
+evaluated and digested into J and K) from a converged SCF density; inputs
+are resident in HBM.  The mixed-precision build (quartets with Schwarz bound
+< 1e-4 in FP32) is the headline; the plain FP64 build is timed in the same
+run, and the SCF energies of both modes are compared (|dE| < 1e-6 Eh).
+
+Prints one JSON line.  ``--gpus`` above 1 is not supported (one GPU per run).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
+PEAK_HBM_GBS = 8000.0
+CUTOFF = 1e-11
+FP32_BOUND = 1e-4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--cpu-seconds', type=float, default=12.0,
+                    help='wall seconds of the sampled CPU baseline (0 to skip)')
+    ap.add_argument('--seed', type=int, default=2)
+    return ap.parse_args()
+
+
+def timed_builds(eng, P, steps, warmup, **kw):
+    import torch
+    for _ in range(warmup):
+        eng.jk(P, **kw)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record()
+        eng.jk(P, **kw)
+        ends[i].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
+    return {'median_ms': ms[len(ms) // 2], 'mean_ms': sum(ms) / len(ms), 'min_ms': ms[0], 'wall_s': wall}
+
+
+def class_rows(eng, mixed):
+    import ctypes
+    import numpy as np
+    from gaussium_amd import _lib
+    out = np.zeros((64, 10), dtype=np.int64)
+    n = eng.lib.gx_jk_class_counts(eng.ctx, CUTOFF, FP32_BOUND, _lib.GX_JK_MIXED if mixed else 0,
+                                   out.ctypes.data_as(ctypes.c_void_p), 64)
+    if n < 0:
+        _lib.check(n, 'gx_jk_class_counts')
+    return [tuple(int(x) for x in r) for r in out[:n]]
+
+
+def cpu_baseline(basis, seconds):
+    from oracle.ref_faithful import baseline
+    b = baseline(basis, seconds=seconds)
+    return {'value': b['quartets_per_s'], 'unit': 'bf-quartets/s', 'cores': b['procs'], 'kind': 'port',
+            'sample': ('pure-Python recursive Obara-Saika (reference algorithm, no memo) on %d random '
+                       'canonical quartets of the 220-function cluster over a %d-process Pool for %.0f s; '
+                       'ERI build %.3g s + dense fill %.3g s + one Fock loop %.3g s, all extrapolated '
+                       'to the %d canonical quartets'
+                       % (b['sampled_quartets'], b['procs'], b['sample_wall_s'], b['eri_s'], b['fill_s'],
+                          b['fock_s'], b['canonical_quartets'])),
+            'extrapolated_total_s': b['total_s'], 'eri_quartets_per_s': b['eri_quartets_per_s']}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py measures one GPU; --gpus %d is not supported (run with --gpus 1)' % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from gaussium_amd import workloads
+    from gaussium_amd.basis import nuclear_repulsion
+    from gaussium_amd.engine import Engine
+    from gaussium_amd.flops import jk_model_flops
+    from gaussium_amd.hartree_fock import RestrictedHF
+
+    nuc, basis, ne, info = workloads.cluster_220bf(seed=args.seed)
+    eng = Engine(basis, nuc)
+    e_nuc = nuclear_repulsion(nuc)
+
+    # Converged SCF energies in both modes (untimed set-up).
+    scf = {}
+    for mode, kw in (('fp64', dict(precision='fp64')), ('mixed', dict(precision='mixed', fp32_bound=FP32_BOUND))):
+        t0 = time.perf_counter()
+        hf = RestrictedHF(nuc, basis, ne, engine=eng, schwarz_cutoff=CUTOFF, max_iter=100, **kw)
+        e = hf.energies()[0] + e_nuc
+        scf[mode] = {'energy': e, 'iterations': hf.scf_method.iterations, 'wall_s': time.perf_counter() - t0,
+                     'density': hf.scf_method.density}
+    dE = abs(scf['mixed']['energy'] - scf['fp64']['energy'])
+    P = scf['fp64']['density']
+
+    fp64 = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF)
+    mixed = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
+    st64 = eng.jk_stats(cutoff=CUTOFF)
+    stmx = eng.jk_stats(cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
+    f64_a, _ = jk_model_flops(class_rows(eng, False))
+    fm64, fm32 = jk_model_flops(class_rows(eng, True))
+
+    ok = dE < 1e-6
+    head = mixed if ok else fp64
+    head_stats = stmx if ok else st64
+    t = head['median_ms'] * 1e-3
+    if ok:
+        flops = fm64 + fm32
+        floor_s = fm64 / (PEAK_FP64_TFLOPS * 1e12) + fm32 / (PEAK_FP32_TFLOPS * 1e12)
+        peak = flops / floor_s / 1e12
+    else:
+        flops = f64_a
+        peak = PEAK_FP64_TFLOPS
+    achieved = flops / t / 1e12
+    line = {
+        'metric': 'ERI quartets/s + J/K build ms (220 bf), mixed vs FP64 at dE<1e-6 Eh',
+        'value': head_stats['bf_quartets'] / t,
+        'unit': 'bf-quartets/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': head['median_ms'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'mixed-f32/f64' if ok else 'f64',
+        'data': 'synthetic (seeded random cluster and basis)',
+        'config': {'workload': info['workload'], 'nbf': info['nbf'], 'atoms': info['n_atoms'],
+                   'electrons': ne, 'schwarz_cutoff': CUTOFF, 'fp32_bound': FP32_BOUND, 'seed': args.seed},
+        'jk_ms_fp64': fp64['median_ms'],
+        'jk_ms_mixed': mixed['median_ms'],
+        'mixed_speedup': fp64['median_ms'] / mixed['median_ms'],
+        'scf_energy_fp64': scf['fp64']['energy'],
+        'scf_energy_mixed': scf['mixed']['energy'],
+        'dE_mixed_Eh': dE,
+        'scf_iterations': [scf['fp64']['iterations'], scf['mixed']['iterations']],
+        'scf_wall_s': [scf['fp64']['wall_s'], scf['mixed']['wall_s']],
+        'quartets': {'canonical_bf': st64['bf_quartets'], 'canonical_shell': st64['shell_quartets'],
+                     'evaluated_bf_fp64_build': st64['bf_quartets_fp64'],
+                     'evaluated_bf_mixed_build': [stmx['bf_quartets_fp64'], stmx['bf_quartets_fp32']],
+                     'evaluated_bf_per_s': (stmx['bf_quartets_fp64'] + stmx['bf_quartets_fp32']) / t
+                     if ok else st64['bf_quartets_fp64'] / t},
+        'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
+                     'frac': achieved / peak, 'traffic': None,
+                     'model_flops': flops, 'fp64_build': {'flops': f64_a,
+                                                          'achieved': f64_a / (fp64['median_ms'] * 1e-3) / 1e12,
+                                                          'frac': f64_a / (fp64['median_ms'] * 1e-3) / 1e12
+                                                          / PEAK_FP64_TFLOPS}},
+    }
+    traffic_file = os.path.join(ROOT, 'profiles', 'jk_220bf_traffic.json')
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        line['roofline']['traffic'] = tr.get('bytes_per_build_mixed' if ok else 'bytes_per_build_fp64')
+    if args.cpu_seconds > 0:
+        cb = cpu_baseline(basis, args.cpu_seconds)
+        line['cpu_baseline'] = cb
+        line['gpu_vs_cpu'] = line['value'] / cb['value']
+    else:
+        line['cpu_baseline'] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

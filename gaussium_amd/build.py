@@ -1,0 +1,45 @@
+"""Build the gx HIP library (libgx.so) for gfx950, in-tree.
+
+``python -m gaussium_amd.build`` or ``gaussium_amd.build.build()``.
+The code generator writes csrc/_gen/eri_gen.h; hipcc compiles the single
+translation unit csrc/gx_api.hip into gaussium_amd/libgx.so.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, 'csrc')
+GEN = os.path.join(CSRC, '_gen')
+LIB = os.path.join(HERE, 'libgx.so')
+ROOT = os.path.dirname(HERE)
+
+# Angular momentum handled by the generated class kernels.
+ERI_LMAX = int(os.environ.get('GX_ERI_LMAX', '1'))
+
+HIPCC_FLAGS = [
+    '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
+    '-munsafe-fp-atomics',
+    # packed FP32 (v_pk_*_f32) is kept out of the generated code
+    '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    '-Wno-unused-parameter',
+]
+
+
+def build(verbose=False):
+    os.makedirs(GEN, exist_ok=True)
+    subprocess.run([sys.executable, os.path.join(CSRC, 'gen_eri.py'),
+                    os.path.join(GEN, 'eri_gen.h'), str(ERI_LMAX)], check=True)
+    hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    tmp = LIB + '.tmp'
+    cmd = [hipcc] + HIPCC_FLAGS + ['-I', os.path.join(ROOT, 'include'), '-I', CSRC,
+                                   os.path.join(CSRC, 'gx_api.hip'), '-o', tmp]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == '__main__':
+    print(build(verbose=True))

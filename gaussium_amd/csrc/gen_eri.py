@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""Generate straight-line HIP device code for each ERI angular-momentum class.
+
+For a shell quartet class (la lb | lc ld) (la >= lb, lc >= ld) this emits
+
+* ``prim<R>``: one primitive quartet of the Obara-Saika vertical recursion
+  (VRR) on centre A (bra) and C (ket), accumulating the contracted
+  [e0|f0]^(0) intermediates, la <= |e| <= la+lb, lc <= |f| <= lc+ld;
+* ``hrr<R>``: the Head-Gordon-Pople horizontal recursion (HRR) moving
+  angular momentum from A to B and from C to D on the contracted
+  intermediates, producing (ab|cd) in the reference's component order.
+
+Everything is unrolled into SSA temporaries so that the compiler keeps the
+recursion in VGPRs; ``R`` is double (FP64 class) or float (FP32 class).
+
+Recursions (``a``/``c`` Cartesian exponent triples, ``i`` an axis, ``m`` the
+Boys order; PA=P-A, WP=W-P, QC=Q-C, WQ=W-Q, AB=A-B, CD=C-D):
+
+  [a+1i|0]^m  = PA_i [a|0]^m + WP_i [a|0]^{m+1}
+               + a_i/(2p) ([a-1i|0]^m - rho/p [a-1i|0]^{m+1})
+  [a|c+1i]^m  = QC_i [a|c]^m + WQ_i [a|c]^{m+1}
+               + c_i/(2q) ([a|c-1i]^m - rho/q [a|c-1i]^{m+1})
+               + a_i/(2(p+q)) [a-1i|c-1i]^{m+1}
+  (a,b+1i|    = (a+1i,b| + AB_i (a,b|            (likewise on the ket)
+
+These are the recursions of obara_saika_scheme.py:131-146 (reference) in the
+VRR+HRR arrangement of head_gordon_pople.py:111-181 (reference).
+
+Usage:  python gen_eri.py OUT.h [LMAX]
+"""
+import sys
+
+AXES = 'xyz'
+
+
+def cart(l):
+    """Cartesian components of angular momentum l, reference order for l <= 2."""
+    if l == 0:
+        return [(0, 0, 0)]
+    if l == 1:
+        return [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    if l == 2:
+        return [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)]
+    out = []
+    for lx in range(l, -1, -1):
+        for ly in range(l - lx, -1, -1):
+            out.append((lx, ly, l - lx - ly))
+    return out
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def sub(t, i, k=1):
+    t = list(t)
+    t[i] -= k
+    return tuple(t)
+
+
+def add(t, i, k=1):
+    t = list(t)
+    t[i] += k
+    return tuple(t)
+
+
+def pick_axis(t):
+    """Axis to lower: the non-zero component with the smallest value (fewest terms)."""
+    best = None
+    for i in range(3):
+        if t[i] > 0 and (best is None or t[i] < t[best]):
+            best = i
+    return best
+
+
+class Emitter:
+    def __init__(self, R):
+        self.lines = []
+        self.n = 0
+        self.R = R
+
+    def tmp(self, expr):
+        name = 't%d' % self.n
+        self.n += 1
+        self.lines.append('    const %s %s = %s;' % (self.R, name, expr))
+        return name
+
+
+def gen_vrr(la, lb, lc, ld):
+    """Body of prim<R>: returns (lines, accumulator layout)."""
+    L = la + lb + lc + ld
+    em = Emitter('R')
+    memo = {}
+    zero = (0, 0, 0)
+
+    def node(e, f, m):
+        key = (e, f, m)
+        if key in memo:
+            return memo[key]
+        if min(e) < 0 or min(f) < 0:
+            raise AssertionError
+        if e == zero and f == zero:
+            memo[key] = 'F[%d]' % m
+            return memo[key]
+        terms = []
+        if f == zero:
+            i = pick_axis(e)
+            em1 = sub(e, i)
+            ax = AXES[i]
+            terms.append('PA%s * %s' % (ax, node(em1, f, m)))
+            terms.append('WP%s * %s' % (ax, node(em1, f, m + 1)))
+            if e[i] >= 2:
+                em2 = sub(e, i, 2)
+                k = e[i] - 1
+                a0 = node(em2, f, m)
+                a1 = node(em2, f, m + 1)
+                terms.append('R(%d) * o2p * (%s - rho_p * %s)' % (k, a0, a1))
+        else:
+            i = pick_axis(f)
+            fm1 = sub(f, i)
+            ax = AXES[i]
+            terms.append('QC%s * %s' % (ax, node(e, fm1, m)))
+            terms.append('WQ%s * %s' % (ax, node(e, fm1, m + 1)))
+            if f[i] >= 2:
+                fm2 = sub(f, i, 2)
+                k = f[i] - 1
+                c0 = node(e, fm2, m)
+                c1 = node(e, fm2, m + 1)
+                terms.append('R(%d) * o2q * (%s - rho_q * %s)' % (k, c0, c1))
+            if e[i] >= 1:
+                terms.append('R(%d) * o2pq * %s' % (e[i], node(sub(e, i), fm1, m + 1)))
+        name = em.tmp(' + '.join(terms))
+        memo[key] = name
+        return name
+
+    layout = []
+    for le in range(la, la + lb + 1):
+        for e in cart(le):
+            for lf in range(lc, lc + ld + 1):
+                for f in cart(lf):
+                    layout.append((e, f))
+    body = []
+    for idx, (e, f) in enumerate(layout):
+        v = node(e, f, 0)
+        body.append('    acc[%d] += %s;' % (idx, v))
+    assert max(m for (_, _, m) in memo) <= L
+    return em.lines + body, layout
+
+
+def gen_hrr(la, lb, lc, ld, layout):
+    """Body of hrr<R>: contracted [e0|f0] -> (ab|cd) in reference component order."""
+    em = Emitter('R')
+    index = {ef: k for k, ef in enumerate(layout)}
+    zero = (0, 0, 0)
+    bra_memo = {}
+
+    def bra(a, b, f):
+        key = (a, b, f)
+        if key in bra_memo:
+            return bra_memo[key]
+        if b == zero:
+            v = 'acc[%d]' % index[(a, f)]
+        else:
+            i = [k for k in range(3) if b[k] > 0][0]
+            v = em.tmp('%s + AB%s * %s' % (bra(add(a, i), sub(b, i), f), AXES[i], bra(a, sub(b, i), f)))
+        bra_memo[key] = v
+        return v
+
+    ket_memo = {}
+
+    def ket(a, b, c, d):
+        key = (a, b, c, d)
+        if key in ket_memo:
+            return ket_memo[key]
+        if d == zero:
+            v = bra(a, b, c)
+        else:
+            i = [k for k in range(3) if d[k] > 0][0]
+            v = em.tmp('%s + CD%s * %s' % (ket(a, b, add(c, i), sub(d, i)), AXES[i], ket(a, b, c, sub(d, i))))
+        ket_memo[key] = v
+        return v
+
+    out = []
+    k = 0
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    out.append('    out[%d] = %s;' % (k, ket(a, b, c, d)))
+                    k += 1
+    return em.lines + out
+
+
+def classes(lmax):
+    pairs = [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+    out = []
+    for i, (la, lb) in enumerate(pairs):
+        for (lc, ld) in pairs[:i + 1]:
+            out.append((la, lb, lc, ld))
+    return out
+
+
+def emit_class(la, lb, lc, ld):
+    vrr, layout = gen_vrr(la, lb, lc, ld)
+    hrr = gen_hrr(la, lb, lc, ld, layout)
+    L = la + lb + lc + ld
+    na, nb, nc, nd = ncart(la), ncart(lb), ncart(lc), ncart(ld)
+    s = []
+    s.append('template <> struct EriClass<%d, %d, %d, %d> {' % (la, lb, lc, ld))
+    s.append('  static constexpr int L = %d, NACC = %d, NOUT = %d;' % (L, len(layout), na * nb * nc * nd))
+    s.append('  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;' % (na, nb, nc, nd))
+    s.append('  template <typename R>')
+    s.append('  static __device__ __forceinline__ void prim(R (&acc)[NACC], const R (&F)[L + 1],')
+    s.append('      R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,')
+    s.append('      R WQx, R WQy, R WQz, R o2p, R o2q, R o2pq, R rho_p, R rho_q) {')
+    s.append('    (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz;')
+    s.append('    (void)QCx; (void)QCy; (void)QCz; (void)WQx; (void)WQy; (void)WQz;')
+    s.append('    (void)o2p; (void)o2q; (void)o2pq; (void)rho_p; (void)rho_q;')
+    s += vrr
+    s.append('  }')
+    s.append('  template <typename R>')
+    s.append('  static __device__ __forceinline__ void hrr(const R (&acc)[NACC], R ABx, R ABy, R ABz,')
+    s.append('      R CDx, R CDy, R CDz, R (&out)[NOUT]) {')
+    s.append('    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;')
+    s += hrr
+    s.append('  }')
+    s.append('};')
+    return '\n'.join(s)
+
+
+def main():
+    out_path = sys.argv[1]
+    lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    parts = ['// Generated by gen_eri.py -- do not edit.',
+             '#pragma once',
+             'template <int LA, int LB, int LC, int LD> struct EriClass;']
+    for c in classes(lmax):
+        parts.append(emit_class(*c))
+    parts.append('#define GX_ERI_LMAX %d' % lmax)
+    parts.append('#define GX_FOR_EACH_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in classes(lmax)))
+    parts.append('#define GX_FOR_EACH_PAIRCLASS(X) ' + ' '.join(
+        'X(%d, %d)' % (a, b) for a in range(lmax + 1) for b in range(a + 1)))
+    text = '\n\n'.join(parts) + '\n'
+    with open(out_path, 'w') as f:
+        f.write(text)
+
+
+if __name__ == '__main__':
+    main()

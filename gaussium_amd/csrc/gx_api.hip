@@ -1,0 +1,875 @@
+// C ABI of the gx engine (declared in include/gx.h): context, basis set-up,
+// pair lists, launches.  Host bookkeeping is O(nshell^2); all floating-point
+// integral work runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/gx.h"
+#include "gx_common.h"
+#include "_gen/eri_gen.h"
+// Single translation unit: the kernels are templates instantiated below.
+#include "gx_eri.hip"
+#include "gx_one.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define GX_HIP(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return fail(GX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define GX_TRY(x)            \
+  do {                       \
+    int r_ = (x);            \
+    if (r_ != GX_OK) return r_; \
+  } while (0)
+
+constexpr int NCLS = (GX_ERI_LMAX + 1) * (GX_ERI_LMAX + 2) / 2;
+
+inline int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+inline int class_la(int c) {
+  int la = 0;
+  while ((la + 1) * (la + 2) / 2 <= c) ++la;
+  return la;
+}
+inline int class_lb(int c) { return c - class_la(c) * (class_la(c) + 1) / 2; }
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(GX_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  return GX_OK;
+}
+
+template <class T>
+void dfree(T** p) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+}
+
+template <class T>
+int upload(T** p, const std::vector<T>& v, hipStream_t s) {
+  GX_TRY(dalloc(p, v.size()));
+  if (!v.empty()) GX_HIP(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return GX_OK;
+}
+
+// Boys master table: F_m(T_n) by the everywhere-convergent series
+// F_M(T) = e^-T sum_k (2T)^k / ((2M+1)(2M+3)...(2M+2k+1)) in long double,
+// then downward recursion; last column exp(-T_n).
+void boys_master_table(std::vector<double>& tab) {
+  const int M = BOYS_MCOL - 1;
+  tab.assign((size_t)BOYS_NT * (BOYS_MCOL + 1), 0.0);
+  for (int n = 0; n < BOYS_NT; ++n) {
+    const long double T = (long double)n * (long double)BOYS_DT;
+    long double term = 1.0L / (2 * M + 1), sum = term;
+    for (int k = 1; k < 4000; ++k) {
+      term *= 2.0L * T / (2 * M + 2 * k + 1);
+      sum += term;
+      if (term < sum * 1e-22L) break;
+    }
+    const long double e = expl(-T);
+    long double f = e * sum;
+    double* row = &tab[(size_t)n * (BOYS_MCOL + 1)];
+    row[M] = (double)f;
+    for (int m = M; m >= 1; --m) {
+      f = (2.0L * T * f + e) / (2 * m - 1);
+      row[m - 1] = (double)f;
+    }
+    row[BOYS_MCOL] = (double)e;
+  }
+}
+
+}  // namespace
+
+struct gx_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  int max_lds = 65536;
+  double* d_boys = nullptr;
+  // basis
+  int nshell = 0, nbf = 0, nprim = 0;
+  std::vector<int> l, np, poff, bf, atom;
+  std::vector<double> ctr, ctr1, exps, coefs, scale;
+  int *d_l = nullptr, *d_np = nullptr, *d_poff = nullptr, *d_bf = nullptr;
+  double *d_ctr = nullptr, *d_ctr1 = nullptr, *d_exps = nullptr, *d_coefs = nullptr,
+         *d_scale = nullptr;
+  // pairs
+  int npair = 0, npp = 0;
+  std::vector<int> pA, pB, pcls, ppoff, pnp;
+  std::vector<double> Q;
+  std::vector<int> cls_start;  // NCLS + 1
+  std::vector<int> pairmap;    // nshell*nshell -> pair id
+  int *d_pA = nullptr, *d_pB = nullptr, *d_ppoff = nullptr, *d_pnp = nullptr;
+  double* d_Q = nullptr;
+  gx::PrimPair64* d_pp64 = nullptr;
+  gx::PrimPair32* d_pp32 = nullptr;
+  std::vector<int*> d_order;  // per class combo
+  // ket segments: per class, runs of pairs with the same primitive-count key
+  std::vector<int> seg_lo, seg_hi, cls_seg;  // cls_seg: NCLS + 1 offsets into seg_*
+  int *d_seg_lo = nullptr, *d_seg_hi = nullptr;
+  std::vector<int64_t> pre_pnp, pre_nab;     // prefix sums over pair ids
+  std::vector<int> nab;                       // basis-function pairs per shell pair
+  // cached work orders for (cutoff, fp32_bound, mixed)
+  double ord_cutoff = -1.0, ord_fp32 = -1.0;
+  int ord_mixed = -1;
+  int num_cu = 256;
+  // scratch
+  double *d_Ps = nullptr, *d_Jt = nullptr, *d_Kt = nullptr;
+  // nuclei
+  int natom = 0;
+  std::vector<double> nxyz, nZ;
+  double *d_nxyz = nullptr, *d_nZ = nullptr;
+
+  gx::ShellView shells() const { return gx::ShellView{d_l, d_bf, d_ctr}; }
+  gx::PairView pairs() const { return gx::PairView{d_pA, d_pB, d_ppoff, d_pnp, d_Q, d_pp64, d_pp32}; }
+
+  void free_pairs() {
+    dfree(&d_pA);
+    dfree(&d_pB);
+    dfree(&d_ppoff);
+    dfree(&d_pnp);
+    dfree(&d_Q);
+    dfree(&d_pp64);
+    dfree(&d_pp32);
+    for (auto& p : d_order) dfree(&p);
+    d_order.clear();
+    dfree(&d_seg_lo);
+    dfree(&d_seg_hi);
+    ord_cutoff = ord_fp32 = -1.0;
+    ord_mixed = -1;
+  }
+  void free_all() {
+    free_pairs();
+    dfree(&d_l);
+    dfree(&d_np);
+    dfree(&d_poff);
+    dfree(&d_bf);
+    dfree(&d_ctr);
+    dfree(&d_ctr1);
+    dfree(&d_exps);
+    dfree(&d_coefs);
+    dfree(&d_scale);
+    dfree(&d_Ps);
+    dfree(&d_Jt);
+    dfree(&d_Kt);
+    dfree(&d_nxyz);
+    dfree(&d_nZ);
+    dfree(&d_boys);
+  }
+};
+
+namespace {
+
+int combo_count() { return NCLS * (NCLS + 1) / 2; }
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch by class.
+hipError_t launch_schwarz(int la, int lb, dim3 g, hipStream_t s, gx::ShellView sh, gx::PairView pr,
+                          int p0, int np, const double* boys, const double* scale, double* Q) {
+#define GX_CASE(A, B)                                                                   \
+  if (la == A && lb == B) {                                                             \
+    hipLaunchKernelGGL((gx::k_schwarz<A, B>), g, dim3(64), 0, s, sh, pr, p0, np, boys, scale, Q); \
+    return hipGetLastError();                                                           \
+  }
+  GX_FOR_EACH_PAIRCLASS(GX_CASE)
+#undef GX_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_dense(int la, int lb, int lc, int ld, dim3 g, hipStream_t s, gx::ShellView sh,
+                        gx::PairView pr, int b0, int nb, int k0, int nk, int same, const double* boys,
+                        const double* scale, double* out, long N) {
+#define GX_CASE(A, B, C, D)                                                                   \
+  if (la == A && lb == B && lc == C && ld == D) {                                             \
+    hipLaunchKernelGGL((gx::k_dense<A, B, C, D>), g, dim3(64), 0, s, sh, pr, b0, nb, k0, nk, same, \
+                       boys, scale, out, N);                                                  \
+    return hipGetLastError();                                                                 \
+  }
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_quartets(int la, int lb, int lc, int ld, dim3 g, hipStream_t s, gx::ShellView sh,
+                           gx::PairView pr, int n, const int* bra, const int* ket, const int* comp,
+                           const double* sc, const long* pos, const double* boys, double* out) {
+#define GX_CASE(A, B, C, D)                                                                       \
+  if (la == A && lb == B && lc == C && ld == D) {                                                 \
+    hipLaunchKernelGGL((gx::k_quartets<A, B, C, D>), g, dim3(64), 0, s, sh, pr, n, bra, ket, comp, \
+                       sc, pos, boys, out);                                                       \
+    return hipGetLastError();                                                                     \
+  }
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hipStream_t s,
+                     const gx::JKArgs& a) {
+#define GX_CASE(A, B, C, D)                                                                   \
+  if (la == A && lb == B && lc == C && ld == D) {                                             \
+    const void* fn = (const void*)gx::k_jk<A, B, C, D>;                                      \
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                            \
+    int per_cu = 0;                                                                           \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, JK_THREADS, lds);           \
+    if (e != hipSuccess) return e;                                                            \
+    if (per_cu < 1) per_cu = 1;                                                               \
+    const int grid = std::min(a.nbra, per_cu * num_cu);                                      \
+    hipLaunchKernelGGL((gx::k_jk<A, B, C, D>), dim3(grid), dim3(JK_THREADS), lds, s, a);      \
+    return hipGetLastError();                                                                 \
+  }
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return hipErrorInvalidValue;
+}
+
+size_t jk_lds_bytes(int la, int lb, int N, int mixed) {
+  const int na = ncart(la), nb = ncart(lb);
+  size_t b = (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
+  if (mixed) b += (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float);
+  b += 2 * (size_t)(na + nb) * N * sizeof(double);
+  return b;
+}
+
+// Build the primitive-pair records of the current pair list on the device.
+int build_primpairs(gx_ctx* c) {
+  std::vector<int> pair_of, pia, pib;
+  c->ppoff.assign(c->npair, 0);
+  c->pnp.assign(c->npair, 0);
+  int off = 0;
+  for (int p = 0; p < c->npair; ++p) {
+    const int A = c->pA[p], B = c->pB[p];
+    c->ppoff[p] = off;
+    c->pnp[p] = c->np[A] * c->np[B];
+    for (int i = 0; i < c->np[A]; ++i)
+      for (int j = 0; j < c->np[B]; ++j) {
+        pair_of.push_back(p);
+        pia.push_back(i);
+        pib.push_back(j);
+      }
+    off += c->pnp[p];
+  }
+  c->npp = off;
+  int *d_of = nullptr, *d_ia = nullptr, *d_ib = nullptr;
+  int rc = GX_OK;
+  if ((rc = upload(&d_of, pair_of, c->stream)) != GX_OK || (rc = upload(&d_ia, pia, c->stream)) != GX_OK ||
+      (rc = upload(&d_ib, pib, c->stream)) != GX_OK) {
+    dfree(&d_of);
+    dfree(&d_ia);
+    dfree(&d_ib);
+    return rc;
+  }
+  GX_TRY(upload(&c->d_pA, c->pA, c->stream));
+  GX_TRY(upload(&c->d_pB, c->pB, c->stream));
+  GX_TRY(upload(&c->d_ppoff, c->ppoff, c->stream));
+  GX_TRY(upload(&c->d_pnp, c->pnp, c->stream));
+  GX_TRY(dalloc(&c->d_pp64, (size_t)c->npp));
+  GX_TRY(dalloc(&c->d_pp32, (size_t)c->npp));
+  if (c->npp > 0) {
+    hipLaunchKernelGGL(gx::k_primpairs, dim3((c->npp + 255) / 256), dim3(256), 0, c->stream, c->npp,
+                       d_of, d_ia, d_ib, c->d_pA, c->d_pB, c->d_poff, c->d_ctr, c->d_exps,
+                       c->d_coefs, c->d_pp64, c->d_pp32);
+  }
+  hipError_t e = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(c->stream);
+  dfree(&d_of);
+  dfree(&d_ia);
+  dfree(&d_ib);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(GX_ERR_HIP, std::string("k_primpairs: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  return GX_OK;
+}
+
+int compute_schwarz(gx_ctx* c) {
+  GX_TRY(dalloc(&c->d_Q, (size_t)c->npair));
+  for (int k = 0; k < NCLS; ++k) {
+    const int p0 = c->cls_start[k], n = c->cls_start[k + 1] - p0;
+    if (n == 0) continue;
+    GX_HIP(launch_schwarz(class_la(k), class_lb(k), dim3((n + 63) / 64), c->stream, c->shells(),
+                          c->pairs(), p0, n, c->d_boys, c->d_scale, c->d_Q));
+  }
+  c->Q.assign(c->npair, 0.0);
+  GX_HIP(hipMemcpyAsync(c->Q.data(), c->d_Q, c->npair * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  GX_HIP(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+// Surviving kets of bra pair ip in ket segment sg: count with Q_ip Q_j >= thr
+// (segments are sorted by decreasing Q).
+int host_count(const gx_ctx* c, int lo, int hi, double QA, double thr) {
+  if (thr <= 0.0) return hi - lo;
+  int a = lo, b = hi;
+  while (a < b) {
+    const int m = (a + b) / 2;
+    if (QA * c->Q[m] >= thr) a = m + 1;
+    else b = m;
+  }
+  return a - lo;
+}
+
+// For every class combo and bra: [lo, hi, n64, n32] per ket segment.
+template <class F>
+void for_each_bra_segment(const gx_ctx* c, int X, int Y, double cutoff, double fp32, int mixed, F&& f) {
+  const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
+  for (int i = 0; i < nb; ++i) {
+    const int ip = b0 + i;
+    const double QA = c->Q[ip];
+    for (int sg = c->cls_seg[Y]; sg < c->cls_seg[Y + 1]; ++sg) {
+      const int lo = c->seg_lo[sg];
+      int hi = c->seg_hi[sg];
+      if (X == Y && hi > ip + 1) hi = ip + 1;
+      if (hi <= lo) continue;
+      const int ns = host_count(c, lo, hi, QA, cutoff);
+      const int n64 = mixed ? std::min(ns, host_count(c, lo, hi, QA, fp32)) : ns;
+      f(i, ip, lo, hi, n64, ns - n64);
+    }
+  }
+}
+
+// Orders for the J/K workgroups: bra positions by decreasing estimated work
+// (primitive quartets that survive the cutoff, FP32 ones weighted 1/2).
+int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
+  if (cutoff == c->ord_cutoff && fp32 == c->ord_fp32 && mixed == c->ord_mixed && !c->d_order.empty())
+    return GX_OK;
+  for (auto& p : c->d_order) dfree(&p);
+  c->d_order.assign(combo_count(), nullptr);
+  int combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      const int nb = c->cls_start[X + 1] - c->cls_start[X];
+      const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
+      if (nb == 0 || nk == 0) continue;
+      std::vector<double> cost(nb, 64.0);
+      for_each_bra_segment(c, X, Y, cutoff, fp32, mixed, [&](int i, int ip, int lo, int, int n64, int n32) {
+        const double k64 = (double)(c->pre_pnp[lo + n64] - c->pre_pnp[lo]);
+        const double k32 = (double)(c->pre_pnp[lo + n64 + n32] - c->pre_pnp[lo + n64]);
+        cost[i] += c->pnp[ip] * (k64 + 0.5 * k32) + 8.0 * (n64 + n32);
+      });
+      std::vector<int> order(nb);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+      GX_TRY(upload(&c->d_order[combo], order, c->stream));
+    }
+  c->ord_cutoff = cutoff;
+  c->ord_fp32 = fp32;
+  c->ord_mixed = mixed;
+  return GX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gx_last_error(void) { return g_err.c_str(); }
+
+int gx_create(gx_ctx** out, int device) {
+  if (!out) return fail(GX_ERR_ARG, "gx_create: null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  GX_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(GX_ERR_ARG, "gx_create: no such device");
+  GX_HIP(hipSetDevice(device));
+  gx_ctx* c = new gx_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreate(&c->own);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(GX_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  c->stream = c->own;
+  int lds = 0, ncu = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lds > 0)
+    c->max_lds = lds;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    c->num_cu = ncu;
+  std::vector<double> tab;
+  boys_master_table(tab);
+  int rc = upload(&c->d_boys, tab, c->stream);
+  if (rc == GX_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GX_ERR_HIP, "boys upload");
+  if (rc != GX_OK) {
+    c->free_all();
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return GX_OK;
+}
+
+int gx_destroy(gx_ctx* c) {
+  if (!c) return fail(GX_ERR_ARG, "gx_destroy: null context");
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->free_all();
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+  return GX_OK;
+}
+
+int gx_set_stream(gx_ctx* c, void* stream) {
+  if (!c) return fail(GX_ERR_ARG, "gx_set_stream: null context");
+  c->stream = stream ? (hipStream_t)stream : c->own;
+  return GX_OK;
+}
+
+int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const int* prim_off,
+                 const int* bf_off, const double* ctr, const double* ctr1, const int* atom,
+                 int nprim_total, const double* exps, const double* coefs, int nbf,
+                 const double* bf_scale) {
+  if (!c) return fail(GX_ERR_ARG, "gx_set_basis: null context");
+  if (nshell <= 0 || nbf <= 0 || nprim_total <= 0 || !l || !nprim || !prim_off || !bf_off || !ctr ||
+      !ctr1 || !atom || !exps || !coefs || !bf_scale)
+    return fail(GX_ERR_ARG, "gx_set_basis: null array or non-positive size");
+  GX_HIP(hipSetDevice(c->device));
+  int covered = 0;
+  for (int s = 0; s < nshell; ++s) {
+    if (l[s] < 0 || l[s] > GX_ERI_LMAX)
+      return fail(GX_ERR_ARG, "gx_set_basis: shell angular momentum " + std::to_string(l[s]) +
+                                  " not supported (max " + std::to_string(GX_ERI_LMAX) + ")");
+    if (nprim[s] <= 0 || prim_off[s] < 0 || prim_off[s] + nprim[s] > nprim_total)
+      return fail(GX_ERR_ARG, "gx_set_basis: primitive range of shell " + std::to_string(s));
+    if (bf_off[s] < 0 || bf_off[s] + ncart(l[s]) > nbf)
+      return fail(GX_ERR_ARG, "gx_set_basis: basis-function range of shell " + std::to_string(s));
+    covered += ncart(l[s]);
+  }
+  if (covered != nbf) return fail(GX_ERR_ARG, "gx_set_basis: shells do not cover nbf functions");
+  for (int p = 0; p < nprim_total; ++p)
+    if (!(exps[p] > 0.0)) return fail(GX_ERR_ARG, "gx_set_basis: non-positive exponent");
+  GX_HIP(hipStreamSynchronize(c->stream));
+  c->free_pairs();
+  c->nshell = nshell;
+  c->nbf = nbf;
+  c->nprim = nprim_total;
+  c->l.assign(l, l + nshell);
+  c->np.assign(nprim, nprim + nshell);
+  c->poff.assign(prim_off, prim_off + nshell);
+  c->bf.assign(bf_off, bf_off + nshell);
+  c->atom.assign(atom, atom + nshell);
+  c->ctr.assign(ctr, ctr + 3 * nshell);
+  c->ctr1.assign(ctr1, ctr1 + 3 * nshell);
+  c->exps.assign(exps, exps + nprim_total);
+  c->coefs.assign(coefs, coefs + nprim_total);
+  c->scale.assign(bf_scale, bf_scale + nbf);
+  GX_TRY(upload(&c->d_l, c->l, c->stream));
+  GX_TRY(upload(&c->d_np, c->np, c->stream));
+  GX_TRY(upload(&c->d_poff, c->poff, c->stream));
+  GX_TRY(upload(&c->d_bf, c->bf, c->stream));
+  GX_TRY(upload(&c->d_ctr, c->ctr, c->stream));
+  GX_TRY(upload(&c->d_ctr1, c->ctr1, c->stream));
+  GX_TRY(upload(&c->d_exps, c->exps, c->stream));
+  GX_TRY(upload(&c->d_coefs, c->coefs, c->stream));
+  GX_TRY(upload(&c->d_scale, c->scale, c->stream));
+  const size_t nn = (size_t)nbf * nbf;
+  GX_TRY(dalloc(&c->d_Ps, nn));
+  GX_TRY(dalloc(&c->d_Jt, nn));
+  GX_TRY(dalloc(&c->d_Kt, nn));
+
+  // Shell pairs, oriented l_A >= l_B, sorted by class then primitive-pair count.
+  struct P {
+    int A, B, cls, kp;
+  };
+  std::vector<P> pl;
+  pl.reserve((size_t)nshell * (nshell + 1) / 2);
+  for (int i = 0; i < nshell; ++i)
+    for (int j = 0; j <= i; ++j) {
+      int A = i, B = j;
+      if (c->l[j] > c->l[i]) std::swap(A, B);
+      pl.push_back(P{A, B, pair_class(c->l[A], c->l[B]), c->np[A] * c->np[B]});
+    }
+  std::stable_sort(pl.begin(), pl.end(), [](const P& a, const P& b) {
+    if (a.cls != b.cls) return a.cls < b.cls;
+    return a.kp > b.kp;
+  });
+  c->npair = (int)pl.size();
+  c->pA.resize(c->npair);
+  c->pB.resize(c->npair);
+  c->pcls.resize(c->npair);
+  c->cls_start.assign(NCLS + 1, 0);
+  for (int p = 0; p < c->npair; ++p) {
+    c->pA[p] = pl[p].A;
+    c->pB[p] = pl[p].B;
+    c->pcls[p] = pl[p].cls;
+  }
+  for (int k = 0, p = 0; k <= NCLS; ++k) {
+    while (p < c->npair && c->pcls[p] < k) ++p;
+    c->cls_start[k] = p;
+  }
+  GX_TRY(build_primpairs(c));
+  GX_TRY(compute_schwarz(c));
+
+  // Segment key per pair: its primitive-pair count, or floor(log2) of it when
+  // a class has more than JK_MAXSEG distinct counts.  Within (class, key)
+  // pairs are ordered by decreasing Schwarz bound.
+  std::vector<int> key(c->npair);
+  for (int k = 0; k < NCLS; ++k) {
+    std::vector<int> distinct;
+    for (int p = c->cls_start[k]; p < c->cls_start[k + 1]; ++p) distinct.push_back(c->pnp[p]);
+    std::sort(distinct.begin(), distinct.end());
+    distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+    const bool coarse = (int)distinct.size() > JK_MAXSEG;
+    for (int p = c->cls_start[k]; p < c->cls_start[k + 1]; ++p) {
+      int kk = c->pnp[p];
+      if (coarse) {
+        int lg = 0;
+        while ((2 << lg) <= kk) ++lg;
+        kk = lg;
+      }
+      key[p] = kk;
+    }
+  }
+  std::vector<int> perm(c->npair);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+    if (c->pcls[a] != c->pcls[b]) return c->pcls[a] < c->pcls[b];
+    if (key[a] != key[b]) return key[a] > key[b];
+    return c->Q[a] > c->Q[b];
+  });
+  {
+    std::vector<int> A2(c->npair), B2(c->npair), C2(c->npair), K2(c->npair);
+    std::vector<double> Q2(c->npair);
+    for (int p = 0; p < c->npair; ++p) {
+      A2[p] = c->pA[perm[p]];
+      B2[p] = c->pB[perm[p]];
+      C2[p] = c->pcls[perm[p]];
+      Q2[p] = c->Q[perm[p]];
+      K2[p] = key[perm[p]];
+    }
+    c->pA.swap(A2);
+    c->pB.swap(B2);
+    c->pcls.swap(C2);
+    c->Q.swap(Q2);
+    key.swap(K2);
+  }
+  GX_TRY(build_primpairs(c));
+  GX_TRY(upload(&c->d_Q, c->Q, c->stream));
+  c->seg_lo.clear();
+  c->seg_hi.clear();
+  c->cls_seg.assign(NCLS + 1, 0);
+  for (int k = 0; k < NCLS; ++k) {
+    c->cls_seg[k] = (int)c->seg_lo.size();
+    for (int p = c->cls_start[k]; p < c->cls_start[k + 1];) {
+      int q = p;
+      while (q < c->cls_start[k + 1] && key[q] == key[p]) ++q;
+      c->seg_lo.push_back(p);
+      c->seg_hi.push_back(q);
+      p = q;
+    }
+    if ((int)c->seg_lo.size() - c->cls_seg[k] > JK_MAXSEG)
+      return fail(GX_ERR_STATE, "gx_set_basis: too many primitive-count segments");
+  }
+  c->cls_seg[NCLS] = (int)c->seg_lo.size();
+  GX_TRY(upload(&c->d_seg_lo, c->seg_lo, c->stream));
+  GX_TRY(upload(&c->d_seg_hi, c->seg_hi, c->stream));
+  c->nab.assign(c->npair, 0);
+  c->pre_pnp.assign(c->npair + 1, 0);
+  c->pre_nab.assign(c->npair + 1, 0);
+  for (int p = 0; p < c->npair; ++p) {
+    const int na = ncart(c->l[c->pA[p]]), nb = ncart(c->l[c->pB[p]]);
+    c->nab[p] = (c->pA[p] == c->pB[p]) ? na * (na + 1) / 2 : na * nb;
+    c->pre_pnp[p + 1] = c->pre_pnp[p] + c->pnp[p];
+    c->pre_nab[p + 1] = c->pre_nab[p] + c->nab[p];
+  }
+  c->pairmap.assign((size_t)nshell * nshell, -1);
+  for (int p = 0; p < c->npair; ++p) {
+    c->pairmap[(size_t)c->pA[p] * nshell + c->pB[p]] = p;
+    c->pairmap[(size_t)c->pB[p] * nshell + c->pA[p]] = p;
+  }
+  c->ord_cutoff = c->ord_fp32 = -1.0;
+  c->ord_mixed = -1;
+  GX_HIP(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_set_nuclei(gx_ctx* c, int natom, const double* xyz, const double* charge) {
+  if (!c) return fail(GX_ERR_ARG, "gx_set_nuclei: null context");
+  if (natom <= 0 || !xyz || !charge) return fail(GX_ERR_ARG, "gx_set_nuclei: bad arguments");
+  GX_HIP(hipSetDevice(c->device));
+  GX_HIP(hipStreamSynchronize(c->stream));
+  c->natom = natom;
+  c->nxyz.assign(xyz, xyz + 3 * natom);
+  c->nZ.assign(charge, charge + natom);
+  GX_TRY(upload(&c->d_nxyz, c->nxyz, c->stream));
+  GX_TRY(upload(&c->d_nZ, c->nZ, c->stream));
+  GX_HIP(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_nbf(gx_ctx* c) { return c ? c->nbf : fail(GX_ERR_ARG, "gx_nbf: null context"); }
+int gx_npair(gx_ctx* c) { return c ? c->npair : fail(GX_ERR_ARG, "gx_npair: null context"); }
+
+int gx_schwarz(gx_ctx* c, double* Q, int* shellA, int* shellB) {
+  if (!c) return fail(GX_ERR_ARG, "gx_schwarz: null context");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_schwarz: no basis set");
+  for (int p = 0; p < c->npair; ++p) {
+    if (Q) Q[p] = c->Q[p];
+    if (shellA) shellA[p] = c->pA[p];
+    if (shellB) shellB[p] = c->pB[p];
+  }
+  return GX_OK;
+}
+
+int gx_eri_dense(gx_ctx* c, double* out) {
+  if (!c || !out) return fail(GX_ERR_ARG, "gx_eri_dense: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_eri_dense: no basis set");
+  GX_HIP(hipSetDevice(c->device));
+  const long N = c->nbf;
+  const long n4 = N * N * N * N;
+  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, c->stream, n4, out,
+                     0.0);
+  GX_HIP(hipGetLastError());
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
+      const int k0 = c->cls_start[Y], nk = c->cls_start[Y + 1] - k0;
+      if (nb == 0 || nk == 0) continue;
+      GX_HIP(launch_dense(class_la(X), class_lb(X), class_la(Y), class_lb(Y), dim3((nk + 63) / 64, nb),
+                          c->stream, c->shells(), c->pairs(), b0, nb, k0, nk, X == Y, c->d_boys,
+                          c->d_scale, out, N));
+    }
+  GX_HIP(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_eri_quartets(gx_ctx* c, int64_t n, const int32_t* ijkl, double* out) {
+  if (!c || (n > 0 && (!ijkl || !out))) return fail(GX_ERR_ARG, "gx_eri_quartets: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_eri_quartets: no basis set");
+  if (n == 0) return GX_OK;
+  GX_HIP(hipSetDevice(c->device));
+  std::vector<int> shell_of(c->nbf);
+  for (int s = 0; s < c->nshell; ++s)
+    for (int k = 0; k < ncart(c->l[s]); ++k) shell_of[c->bf[s] + k] = s;
+  const int ncombo = combo_count();
+  std::vector<std::vector<int>> tb(ncombo), tk(ncombo), tc(ncombo);
+  std::vector<std::vector<double>> ts(ncombo);
+  std::vector<std::vector<long>> tp(ncombo);
+  auto combo_id = [](int X, int Y) { return X * (X + 1) / 2 + Y; };
+  for (int64_t t = 0; t < n; ++t) {
+    int f[4];
+    for (int k = 0; k < 4; ++k) {
+      f[k] = ijkl[4 * t + k];
+      if (f[k] < 0 || f[k] >= c->nbf) return fail(GX_ERR_ARG, "gx_eri_quartets: index out of range");
+    }
+    int s[4], cc[4];
+    for (int k = 0; k < 4; ++k) {
+      s[k] = shell_of[f[k]];
+      cc[k] = f[k] - c->bf[s[k]];
+    }
+    int bra = c->pairmap[(size_t)s[0] * c->nshell + s[1]];
+    int ket = c->pairmap[(size_t)s[2] * c->nshell + s[3]];
+    int ca = cc[0], cb = cc[1], ccc = cc[2], cd = cc[3];
+    if (c->pA[bra] != s[0]) std::swap(ca, cb);
+    if (c->pA[ket] != s[2]) std::swap(ccc, cd);
+    if (c->pcls[bra] < c->pcls[ket]) {
+      std::swap(bra, ket);
+      std::swap(ca, ccc);
+      std::swap(cb, cd);
+    }
+    const int X = c->pcls[bra], Y = c->pcls[ket];
+    const int NB = ncart(class_lb(X)), NC = ncart(class_la(Y)), ND = ncart(class_lb(Y));
+    const int id = combo_id(X, Y);
+    tb[id].push_back(bra);
+    tk[id].push_back(ket);
+    tc[id].push_back(((ca * NB + cb) * NC + ccc) * ND + cd);
+    ts[id].push_back(c->scale[f[0]] * c->scale[f[1]] * c->scale[f[2]] * c->scale[f[3]]);
+    tp[id].push_back((long)t);
+  }
+  double* d_out = nullptr;
+  GX_TRY(dalloc(&d_out, (size_t)n));
+  int rc = GX_OK;
+  std::vector<int*> keep_i;
+  std::vector<double*> keep_d;
+  std::vector<long*> keep_l;
+  for (int X = 0; X < NCLS && rc == GX_OK; ++X)
+    for (int Y = 0; Y <= X && rc == GX_OK; ++Y) {
+      const int id = combo_id(X, Y);
+      const int m = (int)tb[id].size();
+      if (m == 0) continue;
+      int *db = nullptr, *dk = nullptr, *dc = nullptr;
+      double* ds = nullptr;
+      long* dp = nullptr;
+      rc = upload(&db, tb[id], c->stream);
+      if (rc == GX_OK) rc = upload(&dk, tk[id], c->stream);
+      if (rc == GX_OK) rc = upload(&dc, tc[id], c->stream);
+      if (rc == GX_OK) rc = upload(&ds, ts[id], c->stream);
+      if (rc == GX_OK) rc = upload(&dp, tp[id], c->stream);
+      keep_i.push_back(db);
+      keep_i.push_back(dk);
+      keep_i.push_back(dc);
+      keep_d.push_back(ds);
+      keep_l.push_back(dp);
+      if (rc != GX_OK) break;
+      hipError_t e = launch_quartets(class_la(X), class_lb(X), class_la(Y), class_lb(Y),
+                                     dim3((m + 63) / 64), c->stream, c->shells(), c->pairs(), m, db,
+                                     dk, dc, ds, dp, c->d_boys, d_out);
+      if (e != hipSuccess) rc = fail(GX_ERR_HIP, std::string("k_quartets: ") + hipGetErrorString(e));
+    }
+  if (rc == GX_OK) {
+    hipError_t e = hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(GX_ERR_HIP, std::string("gx_eri_quartets: ") + hipGetErrorString(e));
+  } else {
+    (void)hipStreamSynchronize(c->stream);
+  }
+  for (auto p : keep_i) dfree(&p);
+  for (auto p : keep_d) dfree(&p);
+  for (auto p : keep_l) dfree(&p);
+  dfree(&d_out);
+  return rc;
+}
+
+int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, double fp32_bound,
+          int flags) {
+  if (!c || !P || !J || !K) return fail(GX_ERR_ARG, "gx_jk: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_jk: no basis set");
+  if (!(cutoff >= 0.0) || !(fp32_bound >= 0.0)) return fail(GX_ERR_ARG, "gx_jk: negative threshold");
+  GX_HIP(hipSetDevice(c->device));
+  const int N = c->nbf;
+  const long nn = (long)N * N;
+  const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
+  for (int X = 0; X < NCLS; ++X) {
+    const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), N, mixed);
+    if (c->cls_start[X + 1] > c->cls_start[X] && lds > (size_t)c->max_lds)
+      return fail(GX_ERR_ARG, "gx_jk: basis too large for the LDS row cache (nbf=" + std::to_string(N) +
+                                  ", needs " + std::to_string(lds) + " bytes)");
+  }
+  const unsigned g = (unsigned)((nn + 255) / 256);
+  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, P, c->d_Ps);
+  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Jt, 0.0);
+  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
+  GX_HIP(hipGetLastError());
+  GX_TRY(build_orders(c, cutoff, fp32_bound, mixed));
+  int combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
+      const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
+      if (nb == 0 || nk == 0) continue;
+      gx::JKArgs a;
+      a.sh = c->shells();
+      a.pr = c->pairs();
+      a.bra0 = b0;
+      a.nbra = nb;
+      a.ket0 = c->cls_start[Y];
+      a.same = (X == Y);
+      a.seg_lo = c->d_seg_lo + c->cls_seg[Y];
+      a.seg_hi = c->d_seg_hi + c->cls_seg[Y];
+      a.nseg = c->cls_seg[Y + 1] - c->cls_seg[Y];
+      a.order = c->d_order[combo];
+      a.master = c->d_boys;
+      a.P = c->d_Ps;
+      a.Jt = c->d_Jt;
+      a.Kt = c->d_Kt;
+      a.N = N;
+      a.cutoff = cutoff;
+      a.fp32_bound = fp32_bound;
+      a.mixed = mixed;
+      GX_HIP(launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
+                       jk_lds_bytes(class_la(X), class_lb(X), N, mixed), c->stream, a));
+    }
+  hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, c->d_Jt,
+                     c->d_Kt, J, K);
+  GX_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int gx_jk_stats(gx_ctx* c, double cutoff, double fp32_bound, int flags, gx_stats* out) {
+  if (!c || !out) return fail(GX_ERR_ARG, "gx_jk_stats: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_jk_stats: no basis set");
+  std::memset(out, 0, sizeof(*out));
+  const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
+  auto bfq = [&](int ip, int lo, int hi) -> int64_t {
+    // basis-function quartets of bra ip with kets [lo, hi); the diagonal
+    // quartet (ip|ip) counts nab (nab + 1) / 2 instead of nab^2
+    int64_t n = (int64_t)c->nab[ip] * (c->pre_nab[hi] - c->pre_nab[lo]);
+    if (ip >= lo && ip < hi) n -= (int64_t)c->nab[ip] * c->nab[ip] - (int64_t)c->nab[ip] * (c->nab[ip] + 1) / 2;
+    return n;
+  };
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      for_each_bra_segment(c, X, Y, 0.0, 0.0, 0, [&](int, int ip, int lo, int hi, int, int) {
+        out->shell_quartets += hi - lo;
+        out->bf_quartets += bfq(ip, lo, hi);
+      });
+      for_each_bra_segment(c, X, Y, cutoff, fp32_bound, mixed, [&](int, int ip, int lo, int, int n64, int n32) {
+        out->shell_quartets_fp64 += n64;
+        out->shell_quartets_fp32 += n32;
+        out->bf_quartets_fp64 += bfq(ip, lo, lo + n64);
+        out->bf_quartets_fp32 += bfq(ip, lo + n64, lo + n64 + n32);
+      });
+    }
+  return GX_OK;
+}
+
+int gx_jk_class_counts(gx_ctx* c, double cutoff, double fp32_bound, int flags, int64_t* out, int cap) {
+  if (!c || !out) return fail(GX_ERR_ARG, "gx_jk_class_counts: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_jk_class_counts: no basis set");
+  const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
+  int row = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      if (row >= cap) return fail(GX_ERR_ARG, "gx_jk_class_counts: output too small");
+      int64_t* r = out + 10 * row;
+      r[0] = class_la(X);
+      r[1] = class_lb(X);
+      r[2] = class_la(Y);
+      r[3] = class_lb(Y);
+      for (int k = 4; k < 10; ++k) r[k] = 0;
+      for_each_bra_segment(c, X, Y, cutoff, fp32_bound, mixed, [&](int, int ip, int lo, int, int n64, int n32) {
+        r[4] += n64;
+        r[5] += n32;
+        r[6] += (int64_t)c->pnp[ip] * (c->pre_pnp[lo + n64] - c->pre_pnp[lo]);
+        r[7] += (int64_t)c->pnp[ip] * (c->pre_pnp[lo + n64 + n32] - c->pre_pnp[lo + n64]);
+        int64_t b64 = (int64_t)c->nab[ip] * (c->pre_nab[lo + n64] - c->pre_nab[lo]);
+        if (ip >= lo && ip < lo + n64) b64 -= (int64_t)c->nab[ip] * (c->nab[ip] - 1) / 2;
+        int64_t b32 = (int64_t)c->nab[ip] * (c->pre_nab[lo + n64 + n32] - c->pre_nab[lo + n64]);
+        if (ip >= lo + n64 && ip < lo + n64 + n32) b32 -= (int64_t)c->nab[ip] * (c->nab[ip] - 1) / 2;
+        r[8] += b64;
+        r[9] += b32;
+      });
+      ++row;
+    }
+  return row;
+}
+
+int gx_one_electron(gx_ctx* c, double* S, double* T, double* V) {
+  if (!c || !S || !T) return fail(GX_ERR_ARG, "gx_one_electron: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_one_electron: no basis set");
+  if (V && c->natom == 0) return fail(GX_ERR_STATE, "gx_one_electron: no nuclei set");
+  GX_HIP(hipSetDevice(c->device));
+  const int npair = c->nshell * (c->nshell + 1) / 2;
+  hipLaunchKernelGGL(gx::k_one_electron, dim3((npair + 63) / 64), dim3(64), 0, c->stream, c->nshell,
+                     c->d_l, c->d_np, c->d_poff, c->d_bf, c->d_ctr1, c->d_exps, c->d_coefs,
+                     c->d_scale, c->natom, c->d_nxyz, c->d_nZ, c->d_boys, c->nbf, S, T, V);
+  GX_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// Shared device definitions for the gx ERI / J-K engine (gfx950, wave64).
+//
+// Layout in HBM (all arrays owned by a gx context):
+//   shells   : per shell  l, nprim, prim offset, first basis-function index,
+//              centre (ERI) and centre (one-electron)
+//   pairs    : per shell pair (A,B) with l_A >= l_B: A, B, primitive-pair
+//              offset/count, Schwarz bound Q_AB; pairs are grouped by class
+//              (l_A,l_B) and, inside a class, sorted by primitive-pair count
+//              then by Q (descending)
+//   primpairs: per primitive pair one 64-byte FP64 record and one 32-byte
+//              FP32 record (PrimPair64 / PrimPair32)
+//   boys     : master table F_m(T_n), T_n = n * BOYS_DT, n < BOYS_NT,
+//              m <= BOYS_MCOL - 1, plus exp(-T_n); kernels stage the columns
+//              they need into LDS as Taylor coefficients F_{L+k}(T_n)/k!
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GX_WAVE 64
+
+// Boys-function interpolation grid.
+#define BOYS_DT 0.1
+#define BOYS_INV_DT 10.0
+#define BOYS_NT 481          // rows T_n = 0, 0.1, .., 48.0 (nearest row of any T < 48)
+#define BOYS_TMAX 48.0
+#define BOYS_MCOL 20         // F_0 .. F_19 stored (L + 7 <= 19)
+#define BOYS_TAYLOR64 8      // FP64: 8 Taylor terms, |dT| <= 0.05 -> < 1e-15 rel.
+#define BOYS_TAYLOR32 5      // FP32: 5 Taylor terms -> < 3e-9 rel.
+#define BOYS_ROW64 10        // LDS row: 8 coefficients, exp(-T_n), pad
+#define BOYS_ROW32 8         // LDS row: 5 coefficients, pad
+
+namespace gx {
+
+struct __align__(16) PrimPair64 {
+  double p, ip;       // p = a + b, 1/p
+  double Px, Py, Pz;  // Gaussian product centre
+  double K;           // 2 pi^(5/2)-scaled c_a c_b exp(-a b / p |AB|^2)
+  double pad0, pad1;
+};
+
+struct __align__(16) PrimPair32 {
+  float p, ip, Px, Py, Pz, K, pad0, pad1;
+};
+
+// Device-side views passed to kernels by value.
+struct ShellView {
+  const int* l;
+  const int* bf;        // first basis function
+  const double* ctr;    // ERI centre, 3 per shell
+};
+
+struct PairView {
+  const int* A;
+  const int* B;
+  const int* poff;      // primitive-pair offset
+  const int* np;        // primitive-pair count
+  const double* Q;      // Schwarz bound (scaled)
+  const PrimPair64* pp64;
+  const PrimPair32* pp32;
+};
+
+// Reciprocal and reciprocal square root to full FP64 precision from the
+// hardware estimates plus Newton steps (quadratic convergence).
+__device__ __forceinline__ double gx_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ double gx_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  // y <- y (3 - x y^2) / 2, twice
+  double h = 0.5 * x;
+  double t = fma(-h * y, y, 0.5);
+  y = fma(y, t, y);
+  t = fma(-h * y, y, 0.5);
+  return fma(y, t, y);
+}
+
+__device__ __forceinline__ float gx_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float gx_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// ---------------------------------------------------------------------------
+// Boys function F_0..F_L at T from an LDS Taylor table.
+//   T < BOYS_TMAX : F_L by Taylor about the nearest grid point, exp(-T) from
+//                   exp(-T_n) times a Taylor series, then the downward
+//                   recursion F_{m-1} = (2T F_m + e^-T) / (2m - 1)
+//   T >= BOYS_TMAX: F_0 = sqrt(pi / T) / 2 (erf(sqrt T) = 1 to < 1e-21) and
+//                   the upward recursion F_{m+1} = (2m + 1) F_m / (2T)
+//                   (the dropped e^-T term is < 2e-21)
+// Every F_m is multiplied by `scale` (the quartet prefactor).
+template <int L>
+__device__ __forceinline__ void boys64(double T, double scale, const double* __restrict__ tab,
+                                       double (&F)[L + 1]) {
+  if (T < BOYS_TMAX) {
+    const int n = (int)(T * BOYS_INV_DT + 0.5);
+    const double x = n * BOYS_DT - T;  // -(T - T_n)
+    const double* row = tab + n * BOYS_ROW64;
+    double f = row[BOYS_TAYLOR64 - 1];
+#pragma unroll
+    for (int k = BOYS_TAYLOR64 - 2; k >= 0; --k) f = fma(f, x, row[k]);
+    if (L == 0) {
+      F[0] = f * scale;
+      return;
+    }
+    // exp(-T) = exp(-T_n) * exp(x), Taylor with 1/k! coefficients
+    double e = 1.0 / 5040.0;
+    e = fma(e, x, 1.0 / 720.0);
+    e = fma(e, x, 1.0 / 120.0);
+    e = fma(e, x, 1.0 / 24.0);
+    e = fma(e, x, 1.0 / 6.0);
+    e = fma(e, x, 0.5);
+    e = fma(e, x, 1.0);
+    e = fma(e, x, 1.0);
+    e *= row[BOYS_TAYLOR64];
+    F[L] = f * scale;
+    const double t2 = 2.0 * T;
+#pragma unroll
+    for (int m = L; m >= 1; --m) {
+      f = fma(t2, f, e) * (1.0 / (2 * m - 1));
+      F[m - 1] = f * scale;
+    }
+  } else {
+    const double it = gx_rcp(T);
+    double f = 0.88622692545275801365 * sqrt(it);  // sqrt(pi)/2
+    F[0] = f * scale;
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      f = f * ((m + 0.5) * it);
+      F[m + 1] = f * scale;
+    }
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void boys32(float T, float scale, const float* __restrict__ tab,
+                                       float (&F)[L + 1]) {
+  if (T < (float)BOYS_TMAX) {
+    const int n = (int)(T * (float)BOYS_INV_DT + 0.5f);
+    const float x = n * (float)BOYS_DT - T;
+    const float* row = tab + n * BOYS_ROW32;
+    float f = row[BOYS_TAYLOR32 - 1];
+#pragma unroll
+    for (int k = BOYS_TAYLOR32 - 2; k >= 0; --k) f = fmaf(f, x, row[k]);
+    F[L] = f * scale;
+    if (L > 0) {
+      const float e = __expf(-T);
+      const float t2 = 2.0f * T;
+#pragma unroll
+      for (int m = L; m >= 1; --m) {
+        f = fmaf(t2, f, e) * (1.0f / (2 * m - 1));
+        F[m - 1] = f * scale;
+      }
+    }
+  } else {
+    const float it = gx_rcp(T);
+    float f = 0.886226925f * sqrtf(it);
+    F[0] = f * scale;
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      f = f * ((m + 0.5f) * it);
+      F[m + 1] = f * scale;
+    }
+  }
+}
+
+// Stage the class Taylor tables for total angular momentum L into LDS.
+// Master layout: master[n * (BOYS_MCOL + 1) + m] = F_m(T_n), column
+// BOYS_MCOL = exp(-T_n).
+__device__ __forceinline__ void boys_stage64(int L, const double* __restrict__ master,
+                                             double* __restrict__ lds) {
+  const double inv_fact[BOYS_TAYLOR64] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                                          1.0 / 5040};
+  for (int i = threadIdx.x; i < BOYS_NT * BOYS_ROW64; i += blockDim.x) {
+    const int n = i / BOYS_ROW64, k = i - n * BOYS_ROW64;
+    const double* src = master + n * (BOYS_MCOL + 1);
+    double v = 0.0;
+    if (k < BOYS_TAYLOR64) v = src[L + k] * inv_fact[k];
+    else if (k == BOYS_TAYLOR64) v = src[BOYS_MCOL];
+    lds[i] = v;
+  }
+}
+
+__device__ __forceinline__ void boys_stage32(int L, const double* __restrict__ master,
+                                             float* __restrict__ lds) {
+  const double inv_fact[BOYS_TAYLOR32] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24};
+  for (int i = threadIdx.x; i < BOYS_NT * BOYS_ROW32; i += blockDim.x) {
+    const int n = i / BOYS_ROW32, k = i - n * BOYS_ROW32;
+    const double* src = master + n * (BOYS_MCOL + 1);
+    lds[i] = k < BOYS_TAYLOR32 ? (float)(src[L + k] * inv_fact[k]) : 0.0f;
+  }
+}
+
+}  // namespace gx

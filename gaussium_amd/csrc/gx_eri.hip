@@ -1,0 +1,491 @@
+// ERI kernels for gfx950: primitive-pair build, Schwarz bounds, dense and
+// quartet-list ERIs, and the fused direct J/K build.
+//
+// Reference operators replaced (src/gaussium/ of the reference):
+//   ObaraSaika.integrate          integrals/twoelectronrepulsion/obara_saika_scheme.py:17-82
+//   TwoElectronRepulsion.*        matrixelements/electron_repulsion_matrix.py:18-47
+//   FockMatrixRestricted.create   hartreefock/fock_matrix.py:12-22
+//
+// One angular-momentum class (la lb|lc ld) per template instance; the
+// recursion code comes from gen_eri.py (EriClass<...>).
+#include "gx_common.h"
+#include "_gen/eri_gen.h"
+
+namespace gx {
+
+// ---------------------------------------------------------------------------
+// Primitive-pair records.  One thread per primitive pair; `pair_of` maps the
+// primitive pair to its shell pair, (ia, ib) are the primitive indices.
+__global__ void k_primpairs(int npp, const int* __restrict__ pair_of, const int* __restrict__ pia,
+                            const int* __restrict__ pib, const int* __restrict__ pairA,
+                            const int* __restrict__ pairB, const int* __restrict__ sh_poff,
+                            const double* __restrict__ ctr, const double* __restrict__ exps,
+                            const double* __restrict__ coefs, PrimPair64* __restrict__ out64,
+                            PrimPair32* __restrict__ out32) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npp) return;
+  const int ip = pair_of[t];
+  const int A = pairA[ip], B = pairB[ip];
+  const int ga = sh_poff[A] + pia[t], gb = sh_poff[B] + pib[t];
+  const double a = exps[ga], b = exps[gb];
+  const double p = a + b, ip_ = 1.0 / p;
+  const double Ax = ctr[3 * A], Ay = ctr[3 * A + 1], Az = ctr[3 * A + 2];
+  const double Bx = ctr[3 * B], By = ctr[3 * B + 1], Bz = ctr[3 * B + 2];
+  const double dx = Ax - Bx, dy = Ay - By, dz = Az - Bz;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  // 2 pi^(5/2) = 34.98683665524972 split evenly over bra and ket: sqrt = 5.914967172795612
+  const double K = 5.914967172795612 * coefs[ga] * coefs[gb] * exp(-a * b * ip_ * r2);
+  PrimPair64 r;
+  r.p = p;
+  r.ip = ip_;
+  r.Px = (a * Ax + b * Bx) * ip_;
+  r.Py = (a * Ay + b * By) * ip_;
+  r.Pz = (a * Az + b * Bz) * ip_;
+  r.K = K;
+  r.pad0 = 0.0;
+  r.pad1 = 0.0;
+  out64[t] = r;
+  PrimPair32 s;
+  s.p = (float)p;
+  s.ip = (float)ip_;
+  s.Px = (float)r.Px;
+  s.Py = (float)r.Py;
+  s.Pz = (float)r.Pz;
+  s.K = (float)K;
+  s.pad0 = 0.0f;
+  s.pad1 = 0.0f;
+  out32[t] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Contracted shell quartet (AB|CD), all Cartesian components, unscaled.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void quartet64(const ShellView& sh, const PairView& pr, int ip, int jp,
+                                          const double* __restrict__ tab,
+                                          double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
+  using C = EriClass<LA, LB, LC, LD>;
+  const int A = pr.A[ip], B = pr.B[ip], Cs = pr.A[jp], Ds = pr.B[jp];
+  const double Ax = sh.ctr[3 * A], Ay = sh.ctr[3 * A + 1], Az = sh.ctr[3 * A + 2];
+  const double Cx = sh.ctr[3 * Cs], Cy = sh.ctr[3 * Cs + 1], Cz = sh.ctr[3 * Cs + 2];
+  const double ABx = Ax - sh.ctr[3 * B], ABy = Ay - sh.ctr[3 * B + 1], ABz = Az - sh.ctr[3 * B + 2];
+  const double CDx = Cx - sh.ctr[3 * Ds], CDy = Cy - sh.ctr[3 * Ds + 1], CDz = Cz - sh.ctr[3 * Ds + 2];
+  const PrimPair64* __restrict__ bp = pr.pp64 + pr.poff[ip];
+  const PrimPair64* __restrict__ kp = pr.pp64 + pr.poff[jp];
+  const int nb = pr.np[ip], nk = pr.np[jp];
+  double acc[C::NACC];
+#pragma unroll
+  for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
+  for (int ib = 0; ib < nb; ++ib) {
+    const PrimPair64 b = bp[ib];
+    const double PAx = b.Px - Ax, PAy = b.Py - Ay, PAz = b.Pz - Az;
+    for (int ik = 0; ik < nk; ++ik) {
+      const PrimPair64 k = kp[ik];
+      const double rs = gx_rsqrt(b.p + k.p);
+      const double is = rs * rs;
+      const double rho = b.p * k.p * is;
+      const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = b.K * k.K * b.ip * k.ip * rs;
+      double F[C::L + 1];
+      boys64<C::L>(T, pref, tab, F);
+      const double rho_p = rho * b.ip, rho_q = rho * k.ip;
+      C::template prim<double>(acc, F, PAx, PAy, PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
+                               k.Px - Cx, k.Py - Cy, k.Pz - Cz, rho_q * PQx, rho_q * PQy,
+                               rho_q * PQz, 0.5 * b.ip, 0.5 * k.ip, 0.5 * is, rho_p, rho_q);
+    }
+  }
+  C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& pr, int ip, int jp,
+                                          const float* __restrict__ tab,
+                                          float (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
+  using C = EriClass<LA, LB, LC, LD>;
+  const int A = pr.A[ip], B = pr.B[ip], Cs = pr.A[jp], Ds = pr.B[jp];
+  const float Ax = (float)sh.ctr[3 * A], Ay = (float)sh.ctr[3 * A + 1], Az = (float)sh.ctr[3 * A + 2];
+  const float Cx = (float)sh.ctr[3 * Cs], Cy = (float)sh.ctr[3 * Cs + 1], Cz = (float)sh.ctr[3 * Cs + 2];
+  const float ABx = (float)(sh.ctr[3 * A] - sh.ctr[3 * B]);
+  const float ABy = (float)(sh.ctr[3 * A + 1] - sh.ctr[3 * B + 1]);
+  const float ABz = (float)(sh.ctr[3 * A + 2] - sh.ctr[3 * B + 2]);
+  const float CDx = (float)(sh.ctr[3 * Cs] - sh.ctr[3 * Ds]);
+  const float CDy = (float)(sh.ctr[3 * Cs + 1] - sh.ctr[3 * Ds + 1]);
+  const float CDz = (float)(sh.ctr[3 * Cs + 2] - sh.ctr[3 * Ds + 2]);
+  const PrimPair32* __restrict__ bp = pr.pp32 + pr.poff[ip];
+  const PrimPair32* __restrict__ kp = pr.pp32 + pr.poff[jp];
+  const int nb = pr.np[ip], nk = pr.np[jp];
+  float acc[C::NACC];
+#pragma unroll
+  for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0f;
+  for (int ib = 0; ib < nb; ++ib) {
+    const PrimPair32 b = bp[ib];
+    const float PAx = b.Px - Ax, PAy = b.Py - Ay, PAz = b.Pz - Az;
+    for (int ik = 0; ik < nk; ++ik) {
+      const PrimPair32 k = kp[ik];
+      const float rs = gx_rsqrt(b.p + k.p);
+      const float is = rs * rs;
+      const float rho = b.p * k.p * is;
+      const float PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+      const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const float pref = b.K * k.K * b.ip * k.ip * rs;
+      float F[C::L + 1];
+      boys32<C::L>(T, pref, tab, F);
+      const float rho_p = rho * b.ip, rho_q = rho * k.ip;
+      C::template prim<float>(acc, F, PAx, PAy, PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
+                              k.Px - Cx, k.Py - Cy, k.Pz - Cz, rho_q * PQx, rho_q * PQy,
+                              rho_q * PQz, 0.5f * b.ip, 0.5f * k.ip, 0.5f * is, rho_p, rho_q);
+    }
+  }
+  C::template hrr<float>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+// ---------------------------------------------------------------------------
+// Schwarz bounds: Q_AB = sqrt(max_ab |(ab|ab)| s_a^2 s_b^2), one thread per pair.
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_schwarz(ShellView sh, PairView pr, int p0, int np,
+                                                const double* __restrict__ master,
+                                                const double* __restrict__ scale,
+                                                double* __restrict__ Q) {
+  using C = EriClass<LA, LB, LA, LB>;
+  __shared__ double tab[BOYS_NT * BOYS_ROW64];
+  boys_stage64(C::L, master, tab);
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const int ip = p0 + t;
+  double v[C::NOUT];
+  quartet64<LA, LB, LA, LB>(sh, pr, ip, ip, tab, v);
+  const int ao = sh.bf[pr.A[ip]], bo = sh.bf[pr.B[ip]];
+  double m = 0.0;
+#pragma unroll
+  for (int ia = 0; ia < C::NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < C::NB; ++ib) {
+      const double s = scale[ao + ia] * scale[bo + ib];
+      const double d = fabs(v[((ia * C::NB + ib) * C::NA + ia) * C::NB + ib]) * s * s;
+      m = d > m ? d : m;
+    }
+  Q[ip] = sqrt(m);
+}
+
+// ---------------------------------------------------------------------------
+// Dense ERI tensor for small N: one thread per (bra, ket) shell quartet,
+// written to all eight permutational images.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_dense(ShellView sh, PairView pr, int bra0, int nbra,
+                                              int ket0, int nket, int same,
+                                              const double* __restrict__ master,
+                                              const double* __restrict__ scale,
+                                              double* __restrict__ out, long N) {
+  using C = EriClass<LA, LB, LC, LD>;
+  __shared__ double tab[BOYS_NT * BOYS_ROW64];
+  boys_stage64(C::L, master, tab);
+  __syncthreads();
+  const int bi = blockIdx.y;
+  const int kj = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bi >= nbra || kj >= nket || (same && kj > bi)) return;
+  const int ip = bra0 + bi, jp = ket0 + kj;
+  double v[C::NOUT];
+  quartet64<LA, LB, LC, LD>(sh, pr, ip, jp, tab, v);
+  const long ao = sh.bf[pr.A[ip]], bo = sh.bf[pr.B[ip]];
+  const long co = sh.bf[pr.A[jp]], dof = sh.bf[pr.B[jp]];
+  int k = 0;
+  for (int ia = 0; ia < C::NA; ++ia)
+    for (int ib = 0; ib < C::NB; ++ib)
+      for (int ic = 0; ic < C::NC; ++ic)
+        for (int id = 0; id < C::ND; ++id, ++k) {
+          const long i = ao + ia, j = bo + ib, kk = co + ic, l = dof + id;
+          const double x = v[k] * scale[i] * scale[j] * scale[kk] * scale[l];
+          out[((i * N + j) * N + kk) * N + l] = x;
+          out[((j * N + i) * N + kk) * N + l] = x;
+          out[((i * N + j) * N + l) * N + kk] = x;
+          out[((j * N + i) * N + l) * N + kk] = x;
+          out[((kk * N + l) * N + i) * N + j] = x;
+          out[((l * N + kk) * N + i) * N + j] = x;
+          out[((kk * N + l) * N + j) * N + i] = x;
+          out[((l * N + kk) * N + j) * N + i] = x;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Quartet list: task t evaluates shell quartet (bra[t]|ket[t]) and stores
+// component comp[t] (scaled by sc[t]) at out[pos[t]].
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int ntask,
+                                                 const int* __restrict__ bra,
+                                                 const int* __restrict__ ket,
+                                                 const int* __restrict__ comp,
+                                                 const double* __restrict__ sc,
+                                                 const long* __restrict__ pos,
+                                                 const double* __restrict__ master,
+                                                 double* __restrict__ out) {
+  using C = EriClass<LA, LB, LC, LD>;
+  __shared__ double tab[BOYS_NT * BOYS_ROW64];
+  boys_stage64(C::L, master, tab);
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntask) return;
+  double v[C::NOUT];
+  quartet64<LA, LB, LC, LD>(sh, pr, bra[t], ket[t], tab, v);
+  const int c = comp[t];
+  double x = 0.0;
+#pragma unroll
+  for (int k = 0; k < C::NOUT; ++k) x = (k == c) ? v[k] : x;
+  out[pos[t]] = x * sc[t];
+}
+
+// ---------------------------------------------------------------------------
+// Fused direct J/K.
+//
+// Persistent workgroups stride over the bra shell pairs of one class (in
+// decreasing-work order).  For each bra the ket class is split into
+// segments of equal primitive-pair count, each sorted by decreasing Schwarz
+// bound, so the kets that survive the cutoff -- and, in mixed mode, the ones
+// that stay in FP64 -- are prefixes found by binary search.  The surviving
+// kets form a compact virtual index space that the waves sweep 64 at a time
+// (one ket pair per lane); a chunk only mixes precisions or primitive counts
+// at range boundaries.  Integrals are digested at once:
+//   Jt_ab += 2 f v P_cd      (register sums, wave-reduced, one atomic per element)
+//   Jt_cd += 2 f v P_ab      (global FP64 atomics)
+//   Kt_ac += f v P_bd, Kt_ad += f v P_bc, Kt_bc += f v P_ad, Kt_bd += f v P_ac
+//                            (LDS rows of A and B, flushed once per bra)
+// with f the 8-fold degeneracy weight; J = Jt + Jt^T, K = Kt + Kt^T.
+#define JK_MAXSEG 16
+#define JK_THREADS 256
+
+struct JKArgs {
+  ShellView sh;
+  PairView pr;
+  int bra0, nbra, ket0, same;
+  const int* seg_lo;     // ket segments of the ket class: [seg_lo, seg_hi) global pair ids
+  const int* seg_hi;
+  int nseg;
+  const int* order;      // bra positions, largest estimated work first
+  const double* master;  // Boys master table
+  const double* P;       // density, pre-scaled by s_k s_l
+  double* Jt;
+  double* Kt;
+  int N;
+  double cutoff;
+  double fp32_bound;
+  int mixed;
+};
+
+// Number of pairs in [lo, hi) (Q descending) with QA * Q >= thr.
+__device__ __forceinline__ int count_at_least(const double* __restrict__ Q, int lo, int hi, double QA,
+                                              double thr) {
+  int a = lo, b = hi;  // first index in [lo, hi) failing the test
+  while (a < b) {
+    const int m = (a + b) >> 1;
+    if (QA * Q[m] >= thr) a = m + 1;
+    else b = m;
+  }
+  return a - lo;
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
+  using C = EriClass<LA, LB, LC, LD>;
+  constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+  extern __shared__ double smem[];
+  __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1];
+  __shared__ double jab_s[NA * NB];
+  const int N = a.N;
+  double* tab64 = smem;
+  float* tab32 = (float*)(tab64 + BOYS_NT * BOYS_ROW64);
+  double* prow = (double*)(tab32 + (a.mixed ? BOYS_NT * BOYS_ROW32 : 0));
+  double* krow = prow + (NA + NB) * N;
+  const int nr = 2 * a.nseg;  // range r: segment r/2, FP64 (even) or FP32 (odd)
+
+  boys_stage64(C::L, a.master, tab64);
+  if (a.mixed) boys_stage32(C::L, a.master, tab32);
+
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  const int nwave = blockDim.x / GX_WAVE;
+
+  for (int w = blockIdx.x; w < a.nbra; w += gridDim.x) {
+    const int bpos = a.order[w];
+    const int ip = a.bra0 + bpos;
+    const int A = a.pr.A[ip], B = a.pr.B[ip];
+    const int ao = a.sh.bf[A], bo = a.sh.bf[B];
+    const int rowB = (A == B) ? 0 : NA;
+    const double QA = a.pr.Q[ip];
+    __syncthreads();  // the previous bra's flush is complete
+    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
+      const int r = i / N, c = i - r * N;
+      const int g = r < NA ? ao + r : bo + (r - NA);
+      prow[i] = a.P[(long)g * N + c];
+      krow[i] = 0.0;
+    }
+    if (threadIdx.x < NA * NB) jab_s[threadIdx.x] = 0.0;
+    if (threadIdx.x < a.nseg) {
+      const int sg = threadIdx.x;
+      const int lo = a.seg_lo[sg];
+      int hi = a.seg_hi[sg];
+      if (a.same && hi > ip + 1) hi = ip + 1;
+      if (hi < lo) hi = lo;
+      const int nsurv = a.cutoff > 0.0 ? count_at_least(a.pr.Q, lo, hi, QA, a.cutoff) : hi - lo;
+      const int n64 = a.mixed ? min(nsurv, count_at_least(a.pr.Q, lo, hi, QA, a.fp32_bound)) : nsurv;
+      r_start[2 * sg] = lo;
+      r_start[2 * sg + 1] = lo + n64;
+      r_pre[2 * sg + 1] = n64;            // lengths for now
+      r_pre[2 * sg + 2] = nsurv - n64;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      r_pre[0] = 0;
+      for (int r = 0; r < nr; ++r) {
+        acc += r_pre[r + 1];
+        r_pre[r + 1] = acc;
+      }
+    }
+    __syncthreads();
+    const int total = r_pre[nr];
+    const double fA = (A == B) ? 0.5 : 1.0;
+    double jab[NA * NB];
+#pragma unroll
+    for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
+
+    for (int v0 = wave * GX_WAVE; v0 < total; v0 += nwave * GX_WAVE) {
+      const int v = v0 + lane;
+      const bool act = v < total;
+      int r = 0;
+      if (act)
+        while (v >= r_pre[r + 1]) ++r;
+      const int jp = act ? r_start[r] + (v - r_pre[r]) : 0;
+      const bool lo = act && (r & 1);
+      const bool hi = act && !(r & 1);
+      double val[C::NOUT];
+      if (hi) quartet64<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab64, val);
+      if (lo) {
+        float vf[C::NOUT];
+        quartet32<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab32, vf);
+#pragma unroll
+        for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
+      }
+      if (act) {
+        const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
+        const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
+        double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
+        if (jp == ip) f *= 0.5;
+#pragma unroll
+        for (int k = 0; k < C::NOUT; ++k) val[k] *= f;
+        double Pcd[NC * ND];
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = a.P[(long)(co + ic) * N + dof + id];
+        double jcd[NC * ND], kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
+#pragma unroll
+        for (int i = 0; i < NC * ND; ++i) jcd[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NA * NC; ++i) kac[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NA * ND; ++i) kad[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB * NC; ++i) kbc[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB * ND; ++i) kbd[i] = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+            const double pab = prow[ia * N + bo + ib];
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) {
+              const double pac = prow[ia * N + co + ic];
+              const double pbc = prow[(rowB + ib) * N + co + ic];
+#pragma unroll
+              for (int id = 0; id < ND; ++id) {
+                const double x = val[((ia * NB + ib) * NC + ic) * ND + id];
+                const double pad = prow[ia * N + dof + id];
+                const double pbd = prow[(rowB + ib) * N + dof + id];
+                jab[ia * NB + ib] = fma(x, Pcd[ic * ND + id], jab[ia * NB + ib]);
+                jcd[ic * ND + id] = fma(x, pab, jcd[ic * ND + id]);
+                kac[ia * NC + ic] = fma(x, pbd, kac[ia * NC + ic]);
+                kad[ia * ND + id] = fma(x, pbc, kad[ia * ND + id]);
+                kbc[ib * NC + ic] = fma(x, pad, kbc[ib * NC + ic]);
+                kbd[ib * ND + id] = fma(x, pac, kbd[ib * ND + id]);
+              }
+            }
+          }
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < ND; ++id)
+            atomicAdd(&a.Jt[(long)(co + ic) * N + dof + id], 2.0 * jcd[ic * ND + id]);
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+          for (int ic = 0; ic < NC; ++ic) atomicAdd(&krow[ia * N + co + ic], kac[ia * NC + ic]);
+#pragma unroll
+          for (int id = 0; id < ND; ++id) atomicAdd(&krow[ia * N + dof + id], kad[ia * ND + id]);
+        }
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+          for (int ic = 0; ic < NC; ++ic)
+            atomicAdd(&krow[(rowB + ib) * N + co + ic], kbc[ib * NC + ic]);
+#pragma unroll
+          for (int id = 0; id < ND; ++id)
+            atomicAdd(&krow[(rowB + ib) * N + dof + id], kbd[ib * ND + id]);
+        }
+      }
+    }
+
+    // J_ab: wave reduction, then one LDS atomic per wave and element.
+#pragma unroll
+    for (int i = 0; i < NA * NB; ++i) {
+      double x = jab[i];
+#pragma unroll
+      for (int off = GX_WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, GX_WAVE);
+      if (lane == 0 && x != 0.0) atomicAdd(&jab_s[i], x);
+    }
+    __syncthreads();
+    if (threadIdx.x < NA * NB) {
+      const double x = jab_s[threadIdx.x];
+      if (x != 0.0) {
+        const int ia = threadIdx.x / NB, ib = threadIdx.x - ia * NB;
+        atomicAdd(&a.Jt[(long)(ao + ia) * N + bo + ib], 2.0 * x);
+      }
+    }
+    const int nrow = (A == B) ? NA : NA + NB;
+    for (int i = threadIdx.x; i < nrow * N; i += blockDim.x) {
+      const double x = krow[i];
+      if (x != 0.0) {
+        const int r = i / N, c = i - r * N;
+        const int g = r < NA ? ao + r : bo + (r - NA);
+        atomicAdd(&a.Kt[(long)g * N + c], x);
+      }
+    }
+  }
+}
+
+// J = s (Jt + Jt^T) s, K = s (Kt + Kt^T) s; optional P' = s P s.
+__global__ void k_symmetrize(int N, const double* __restrict__ scale, const double* __restrict__ Jt,
+                             const double* __restrict__ Kt, double* __restrict__ J,
+                             double* __restrict__ K) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  const int i = t / N, j = t - i * N;
+  const double s = scale[i] * scale[j];
+  J[t] = s * (Jt[t] + Jt[j * N + i]);
+  K[t] = s * (Kt[t] + Kt[j * N + i]);
+}
+
+__global__ void k_scale_density(int N, const double* __restrict__ scale, const double* __restrict__ P,
+                                double* __restrict__ Ps) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  const int i = t / N, j = t - i * N;
+  Ps[t] = P[t] * scale[i] * scale[j];
+}
+
+__global__ void k_fill(long n, double* __restrict__ x, double v) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+
+}  // namespace gx

@@ -1,0 +1,226 @@
+// One-electron integrals S, T, V on gfx950 (McMurchie-Davidson).
+//
+// Replaces (src/gaussium/ of the reference):
+//   orbital_overlap          integrals/orbital_overlap_integral.py:16-36
+//   kinetic (del operator)   matrixelements/kinetic_energy_matrix.py:16-33,
+//                            factory/primitive_basis_factory.py:89-122
+//   nuclear_attraction       integrals/nuclear_attraction_integral.py:19-57
+// The reference evaluates the Taketa-Huzinaga-O-ohata explicit sums; this
+// kernel uses Hermite expansion coefficients E^{ij}_t and Hermite Coulomb
+// integrals R_tuv, which give the same integrals.  Kinetic energy uses the
+// reference's expansion  -1/2 nabla^2 g_b = b(2l_b+3) g_b - 2b^2 sum_i g_{b+2i}
+//                        - 1/2 sum_i l_i(l_i-1) g_{b-2i}.
+// One thread per shell pair (i >= j); all component blocks are written to
+// both triangles.
+#include "gx_common.h"
+
+namespace gx {
+
+#define ONE_LMAX 2            // shell l
+#define ONE_EMAX (ONE_LMAX + 2)  // l + 2 for the kinetic expansion
+#define ONE_RMAX (2 * ONE_LMAX)  // Hermite order for V
+
+// Boys F_0..F_n(T) from the master table in global memory (Taylor + downward).
+__device__ inline void boys_master(int n, double T, const double* __restrict__ master, double* F) {
+  if (T < BOYS_TMAX) {
+    const int k0 = (int)(T * BOYS_INV_DT + 0.5);
+    const double x = k0 * BOYS_DT - T;
+    const double* row = master + k0 * (BOYS_MCOL + 1);
+    const double invf[8] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+    double f = row[n + 7] * invf[7];
+    for (int k = 6; k >= 0; --k) f = fma(f, x, row[n + k] * invf[k]);
+    double e = 1.0 / 5040.0;
+    e = fma(e, x, 1.0 / 720.0);
+    e = fma(e, x, 1.0 / 120.0);
+    e = fma(e, x, 1.0 / 24.0);
+    e = fma(e, x, 1.0 / 6.0);
+    e = fma(e, x, 0.5);
+    e = fma(e, x, 1.0);
+    e = fma(e, x, 1.0);
+    e *= row[BOYS_MCOL];
+    F[n] = f;
+    for (int m = n; m >= 1; --m) {
+      f = fma(2.0 * T, f, e) / (2 * m - 1);
+      F[m - 1] = f;
+    }
+  } else {
+    double f = 0.88622692545275801365 * sqrt(1.0 / T);
+    F[0] = f;
+    for (int m = 0; m < n; ++m) {
+      f *= (m + 0.5) / T;
+      F[m + 1] = f;
+    }
+  }
+}
+
+// Hermite coefficients E[i][j][t] (without the exp(-mu X^2) factor) for
+// i <= imax, j <= jmax along one axis.
+__device__ inline void hermite_e(int imax, int jmax, double PA, double PB, double o2p,
+                                 double (*E)[ONE_EMAX + 1][2 * ONE_EMAX + 1]) {
+  for (int i = 0; i <= imax; ++i)
+    for (int j = 0; j <= jmax; ++j)
+      for (int t = 0; t <= 2 * ONE_EMAX; ++t) E[i][j][t] = 0.0;
+  E[0][0][0] = 1.0;
+  for (int i = 0; i <= imax; ++i) {
+    if (i > 0) {
+      for (int t = 0; t <= i; ++t) {
+        double v = PA * E[i - 1][0][t];
+        if (t > 0) v += o2p * E[i - 1][0][t - 1];
+        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
+        E[i][0][t] = v;
+      }
+    }
+    for (int j = 1; j <= jmax; ++j) {
+      for (int t = 0; t <= i + j; ++t) {
+        double v = (t <= i + j - 1) ? PB * E[i][j - 1][t] : 0.0;
+        if (t > 0) v += o2p * E[i][j - 1][t - 1];
+        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+        E[i][j][t] = v;
+      }
+    }
+  }
+}
+
+__device__ inline int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
+
+__device__ inline void cart_d(int l, int k, int* e) {
+  // reference component order for l <= 2
+  const int c1[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const int c2[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+  if (l == 0) {
+    e[0] = e[1] = e[2] = 0;
+  } else if (l == 1) {
+    e[0] = c1[k][0]; e[1] = c1[k][1]; e[2] = c1[k][2];
+  } else {
+    e[0] = c2[k][0]; e[1] = c2[k][1]; e[2] = c2[k][2];
+  }
+}
+
+__global__ __launch_bounds__(64) void k_one_electron(
+    int nshell, const int* __restrict__ sl, const int* __restrict__ snp,
+    const int* __restrict__ spoff, const int* __restrict__ sbf, const double* __restrict__ ctr1,
+    const double* __restrict__ exps, const double* __restrict__ coefs,
+    const double* __restrict__ scale, int natom, const double* __restrict__ nxyz,
+    const double* __restrict__ nZ, const double* __restrict__ master, int N,
+    double* __restrict__ S, double* __restrict__ T, double* __restrict__ V) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npair = nshell * (nshell + 1) / 2;
+  if (t >= npair) return;
+  // unrank t -> (si >= sj)
+  int si = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (si * (si + 1) / 2 > t) --si;
+  while ((si + 1) * (si + 2) / 2 <= t) ++si;
+  const int sj = t - si * (si + 1) / 2;
+  const int la = sl[si], lb = sl[sj];
+  const int na = ncart_d(la), nb = ncart_d(lb);
+  const double A[3] = {ctr1[3 * si], ctr1[3 * si + 1], ctr1[3 * si + 2]};
+  const double B[3] = {ctr1[3 * sj], ctr1[3 * sj + 1], ctr1[3 * sj + 2]};
+  double s_acc[36], t_acc[36], v_acc[36];
+  for (int k = 0; k < 36; ++k) s_acc[k] = t_acc[k] = v_acc[k] = 0.0;
+  double E[3][ONE_EMAX + 1][ONE_EMAX + 1][2 * ONE_EMAX + 1];
+  double R[ONE_RMAX + 1][ONE_RMAX + 1][ONE_RMAX + 1][ONE_RMAX + 1];
+  double F[ONE_RMAX + 1];
+  for (int ia = 0; ia < snp[si]; ++ia) {
+    const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
+    for (int ib = 0; ib < snp[sj]; ++ib) {
+      const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
+      const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
+      double P[3], r2 = 0.0;
+      for (int x = 0; x < 3; ++x) {
+        P[x] = (a * A[x] + b * B[x]) / p;
+        r2 += (A[x] - B[x]) * (A[x] - B[x]);
+        hermite_e(la, lb + 2, P[x] - A[x], P[x] - B[x], o2p, E[x]);
+      }
+      const double Kab = exp(-mu * r2) * ca * cb;
+      const double s00 = Kab * pow(M_PI / p, 1.5);
+      // overlap and kinetic
+      for (int ka = 0; ka < na; ++ka) {
+        int ea[3];
+        cart_d(la, ka, ea);
+        for (int kb = 0; kb < nb; ++kb) {
+          int eb[3];
+          cart_d(lb, kb, eb);
+          const double sx = E[0][ea[0]][eb[0]][0], sy = E[1][ea[1]][eb[1]][0],
+                       sz = E[2][ea[2]][eb[2]][0];
+          const double sv = sx * sy * sz;
+          const int lbt = eb[0] + eb[1] + eb[2];
+          double tv = b * (2 * lbt + 3) * sv;
+          tv -= 2.0 * b * b * (E[0][ea[0]][eb[0] + 2][0] * sy * sz + sx * E[1][ea[1]][eb[1] + 2][0] * sz +
+                               sx * sy * E[2][ea[2]][eb[2] + 2][0]);
+          if (eb[0] >= 2) tv -= 0.5 * eb[0] * (eb[0] - 1) * E[0][ea[0]][eb[0] - 2][0] * sy * sz;
+          if (eb[1] >= 2) tv -= 0.5 * eb[1] * (eb[1] - 1) * sx * E[1][ea[1]][eb[1] - 2][0] * sz;
+          if (eb[2] >= 2) tv -= 0.5 * eb[2] * (eb[2] - 1) * sx * sy * E[2][ea[2]][eb[2] - 2][0];
+          s_acc[ka * nb + kb] += s00 * sv;
+          t_acc[ka * nb + kb] += s00 * tv;
+        }
+      }
+      // nuclear attraction
+      const int L = la + lb;
+      for (int c = 0; c < natom; ++c) {
+        const double PC[3] = {P[0] - nxyz[3 * c], P[1] - nxyz[3 * c + 1], P[2] - nxyz[3 * c + 2]};
+        const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+        boys_master(L, Tb, master, F);
+        // R^{(n)}_{tuv}: R[n][t][u][v]
+        double m2p = 1.0;
+        for (int n = 0; n <= L; ++n) {
+          R[n][0][0][0] = m2p * F[n];
+          m2p *= -2.0 * p;
+        }
+        for (int n = L - 1; n >= 0; --n) {
+          const int top = L - n;
+          for (int tt = 0; tt <= top; ++tt)
+            for (int u = 0; u <= top - tt; ++u)
+              for (int v = 0; v <= top - tt - u; ++v) {
+                if (tt + u + v == 0) continue;
+                double r;
+                if (tt > 0) {
+                  r = PC[0] * R[n + 1][tt - 1][u][v];
+                  if (tt > 1) r += (tt - 1) * R[n + 1][tt - 2][u][v];
+                } else if (u > 0) {
+                  r = PC[1] * R[n + 1][tt][u - 1][v];
+                  if (u > 1) r += (u - 1) * R[n + 1][tt][u - 2][v];
+                } else {
+                  r = PC[2] * R[n + 1][tt][u][v - 1];
+                  if (v > 1) r += (v - 1) * R[n + 1][tt][u][v - 2];
+                }
+                R[n][tt][u][v] = r;
+              }
+        }
+        const double pref = -nZ[c] * 2.0 * M_PI / p * Kab;
+        for (int ka = 0; ka < na; ++ka) {
+          int ea[3];
+          cart_d(la, ka, ea);
+          for (int kb = 0; kb < nb; ++kb) {
+            int eb[3];
+            cart_d(lb, kb, eb);
+            double sum = 0.0;
+            for (int tt = 0; tt <= ea[0] + eb[0]; ++tt)
+              for (int u = 0; u <= ea[1] + eb[1]; ++u)
+                for (int v = 0; v <= ea[2] + eb[2]; ++v)
+                  sum += E[0][ea[0]][eb[0]][tt] * E[1][ea[1]][eb[1]][u] * E[2][ea[2]][eb[2]][v] *
+                         R[0][tt][u][v];
+            v_acc[ka * nb + kb] += pref * sum;
+          }
+        }
+      }
+    }
+  }
+  const int ao = sbf[si], bo = sbf[sj];
+  for (int ka = 0; ka < na; ++ka)
+    for (int kb = 0; kb < nb; ++kb) {
+      const long i = ao + ka, j = bo + kb;
+      const double s = scale[i] * scale[j];
+      const double sv = s * s_acc[ka * nb + kb], tv = s * t_acc[ka * nb + kb],
+                   vv = s * v_acc[ka * nb + kb];
+      S[i * N + j] = sv;
+      S[j * N + i] = sv;
+      T[i * N + j] = tv;
+      T[j * N + i] = tv;
+      if (V) {
+        V[i * N + j] = vv;
+        V[j * N + i] = vv;
+      }
+    }
+}
+
+}  // namespace gx

@@ -1,0 +1,139 @@
+"""Python handle of a gx context: basis upload, ERIs, direct J/K, one-electron.
+
+Device buffers are torch tensors (FP64) whose ``data_ptr()`` is handed to the
+C ABI; the context launches on torch's current stream so that the ordering
+with surrounding torch work needs no extra synchronisation.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .basis import ShellSet, pack_shells
+
+
+def _p(a):
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(a.data_ptr())
+
+
+class Engine:
+    """One gx context bound to a CUDA/HIP device.
+
+    ``Engine(basis, nuclei)`` packs the contracted functions into shells,
+    builds primitive pairs and Schwarz bounds on the device and keeps them for
+    every later call.
+    """
+
+    def __init__(self, basis=None, nuclei=None, device=None):
+        if not torch.cuda.is_available():
+            raise _lib.GxError('gaussium_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
+        self.device = torch.device('cuda', torch.cuda.current_device() if device is None else device)
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.gx_create(ctypes.byref(h), self.device.index), 'gx_create')
+        self.ctx = h
+        self.shells = None
+        self.nuclei = None
+        if basis is not None:
+            self.set_basis(basis, nuclei)
+        if nuclei is not None:
+            self.set_nuclei(nuclei)
+
+    def close(self):
+        if getattr(self, 'ctx', None) is not None and self.ctx.value:
+            self.lib.gx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up -------------------------------------------------------------
+    def _bind_stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib.gx_set_stream(self.ctx, ctypes.c_void_p(s)), 'gx_set_stream')
+
+    def set_basis(self, basis, nuclei=None):
+        sh = basis if isinstance(basis, ShellSet) else pack_shells(basis, nuclei)
+        self._bind_stream()
+        _lib.check(self.lib.gx_set_basis(
+            self.ctx, sh.nshell, _p(sh.l), _p(sh.nprim), _p(sh.prim_off), _p(sh.bf_off), _p(sh.ctr),
+            _p(sh.ctr1), _p(sh.atom), len(sh.exps), _p(sh.exps), _p(sh.coefs), sh.nbf, _p(sh.bf_scale)),
+            'gx_set_basis')
+        self.shells = sh
+        return sh
+
+    def set_nuclei(self, nuclei):
+        xyz = np.ascontiguousarray([n.coordinates for n in nuclei], dtype=np.float64)
+        z = np.ascontiguousarray([float(n.charge) for n in nuclei], dtype=np.float64)
+        _lib.check(self.lib.gx_set_nuclei(self.ctx, len(nuclei), _p(xyz), _p(z)), 'gx_set_nuclei')
+        self.nuclei = nuclei
+
+    @property
+    def nbf(self):
+        return self.shells.nbf
+
+    def _empty(self, *shape):
+        return torch.empty(*shape, dtype=torch.float64, device=self.device)
+
+    def _dev(self, x):
+        t = torch.as_tensor(x, dtype=torch.float64, device=self.device)
+        return t.contiguous()
+
+    # -- integrals -------------------------------------------------------------
+    def schwarz(self):
+        n = self.lib.gx_npair(self.ctx)
+        q = np.zeros(n)
+        a = np.zeros(n, dtype=np.int32)
+        b = np.zeros(n, dtype=np.int32)
+        _lib.check(self.lib.gx_schwarz(self.ctx, _p(q), _p(a), _p(b)), 'gx_schwarz')
+        return q, a, b
+
+    def eri_quartets(self, keys):
+        """Values of basis-function quartets ``keys`` (int array [n, 4]) as a numpy array."""
+        k = np.ascontiguousarray(np.asarray(keys, dtype=np.int32).reshape(-1, 4))
+        out = np.zeros(len(k))
+        self._bind_stream()
+        _lib.check(self.lib.gx_eri_quartets(self.ctx, len(k), _p(k), _p(out)), 'gx_eri_quartets')
+        return out
+
+    def eri_dense(self):
+        """Dense (ij|kl) tensor [N,N,N,N] on the device."""
+        n = self.nbf
+        out = self._empty(n, n, n, n)
+        self._bind_stream()
+        _lib.check(self.lib.gx_eri_dense(self.ctx, _p(out)), 'gx_eri_dense')
+        return out
+
+    def jk(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False, out=None):
+        """Direct (J, K) from density ``P`` (FP64 device tensors)."""
+        n = self.nbf
+        P = self._dev(P)
+        if P.shape != (n, n):
+            raise ValueError('density has shape %s, expected (%d, %d)' % (tuple(P.shape), n, n))
+        J, K = (self._empty(n, n), self._empty(n, n)) if out is None else out
+        self._bind_stream()
+        _lib.check(self.lib.gx_jk(self.ctx, _p(P), _p(J), _p(K), float(cutoff), float(fp32_bound),
+                                  _lib.GX_JK_MIXED if mixed else 0), 'gx_jk')
+        return J, K
+
+    def jk_stats(self, cutoff=0.0, fp32_bound=0.0, mixed=False):
+        st = _lib.GxStats()
+        _lib.check(self.lib.gx_jk_stats(self.ctx, float(cutoff), float(fp32_bound),
+                                        _lib.GX_JK_MIXED if mixed else 0, ctypes.byref(st)), 'gx_jk_stats')
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def one_electron(self):
+        """(S, T, V) on the device; V needs nuclei."""
+        n = self.nbf
+        S, T = self._empty(n, n), self._empty(n, n)
+        V = self._empty(n, n) if self.nuclei is not None else None
+        self._bind_stream()
+        _lib.check(self.lib.gx_one_electron(self.ctx, _p(S), _p(T), _p(V) if V is not None else None),
+                   'gx_one_electron')
+        return S, T, V

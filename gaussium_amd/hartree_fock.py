@@ -1,0 +1,115 @@
+"""Drop-in RHF driver and the reference's operator call sites.
+
+* ``TwoElectronRepulsionMatrixOS`` -- same constructor and methods as
+  matrixelements/electron_repulsion_matrix.py:9-53 of the reference
+  (``calculate_integral(i, j, k, l)``, ``create_repulsion_matrix()``), backed
+  by the gx class kernels instead of a ``multiprocessing.Pool``.
+* ``FockMatrixRestricted`` -- the Fock-builder plugin of
+  hartreefock/fock_matrix.py:5-22 (``create(P) -> F``, ``.core_hamiltonian``),
+  computing G = J - K/2 by the direct fused J/K kernel; no N^4 tensor.
+* ``RestrictedHF`` -- hartreefock/hartree_fock.py:18-84: S, T, V on the
+  device, canonical orthogonalisation, core-H guess, DIIS SCF.
+
+Options the reference does not have (``schwarz_cutoff``, ``fp32_bound``,
+``precision``, ``max_iter``) default to the reference's behaviour: no
+screening, all FP64, no iteration cap.
+"""
+import time
+
+import numpy as np
+import torch
+
+from .engine import Engine
+from .scf import LinearAlgebra, RestrictedSCF
+
+
+class TwoElectronRepulsionMatrixOS:
+    """ERI operator with the reference's interface (electron_repulsion_matrix.py:9-53)."""
+
+    def __init__(self, basis_set_array, symmetry=None, processes=1, engine=None):
+        self.basis_set_array = basis_set_array
+        self.matrix_size = len(basis_set_array)
+        self.symmetry = symmetry
+        self.processes = processes  # accepted for signature parity; the GPU path ignores it
+        self.engine = engine if engine is not None else Engine(basis_set_array)
+
+    def calculate_integral(self, i, j, k, l):
+        if self.symmetry is not None and hasattr(self.symmetry, 'none_zero_integral') \
+                and not self.symmetry.none_zero_integral((i, j, k, l)):
+            return 0.0
+        return float(self.engine.eri_quartets([(i, j, k, l)])[0])
+
+    def create_repulsion_matrix(self):
+        """Dense [N,N,N,N] numpy tensor (small N; the SCF path never forms it)."""
+        return self.engine.eri_dense().cpu().numpy()
+
+
+class FockMatrixRestricted:
+    """F = H + J - K/2 from the direct fused J/K build (fock_matrix.py:12-22)."""
+
+    def __init__(self, core_hamiltonian, engine, schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64'):
+        self.core_hamiltonian = core_hamiltonian
+        self.engine = engine
+        self.schwarz_cutoff = schwarz_cutoff
+        self.fp32_bound = fp32_bound
+        self.mixed = precision == 'mixed'
+        self.matrix_size = core_hamiltonian.shape[0]
+        self.builds = 0
+
+    def create(self, density_matrix):
+        J, K = self.engine.jk(density_matrix, cutoff=self.schwarz_cutoff, fp32_bound=self.fp32_bound,
+                              mixed=self.mixed)
+        self.builds += 1
+        return self.core_hamiltonian + J - 0.5 * K
+
+
+class RestrictedHF:
+    """Restricted Hartree-Fock (hartree_fock.py:18-84) on one GPU."""
+
+    def __init__(self, nuclei_array, basis_set_array, electrons, symmetry=None, processes=1,
+                 schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64', max_iter=None, engine=None,
+                 verbose=False):
+        self.nuclei_array = nuclei_array
+        self.basis_set_array = basis_set_array
+        self.electrons = electrons
+        self.symmetry = symmetry
+        self.verbose = verbose
+        t0 = time.perf_counter()
+        self.engine = engine if engine is not None else Engine(basis_set_array, nuclei_array)
+        if engine is not None:
+            self.engine.set_basis(basis_set_array, nuclei_array)
+            self.engine.set_nuclei(nuclei_array)
+        S, T, V = self.engine.one_electron()
+        self.orbital_overlap = S
+        self.kinetic_energy = T
+        self.nuclear_attraction = V
+        self.core_hamiltonian = T + V
+        self.linear_algebra = LinearAlgebra(S)
+        self.fock_builder = FockMatrixRestricted(self.core_hamiltonian, self.engine, schwarz_cutoff,
+                                                 fp32_bound, precision)
+        self.scf_method = RestrictedSCF(self.linear_algebra, electrons, S, self.fock_builder,
+                                        max_iter=max_iter)
+        self.setup_time = time.perf_counter() - t0
+        self._repulsion = None
+
+    @property
+    def repulsion(self):
+        """Dense ERI tensor (numpy), formed only on request (post-HF consumers, small N)."""
+        if self._repulsion is None:
+            self._repulsion = self.engine.eri_dense().cpu().numpy()
+        return self._repulsion
+
+    def initial_guess(self):
+        return self.linear_algebra.diagonalize(self.core_hamiltonian)
+
+    def begin_scf(self):
+        e0, c0 = self.initial_guess()
+        t0 = time.perf_counter()
+        energy, eps, c = self.scf_method.begin_iterations(e0, c0)
+        self.scf_time = time.perf_counter() - t0
+        if self.verbose:
+            print('SCF converged in %d iterations, %.3f s' % (self.scf_method.iterations, self.scf_time))
+        return energy, eps, c
+
+    def energies(self):
+        return self.begin_scf()[0], 0.0

@@ -1,0 +1,48 @@
+"""``start()`` and ``Energy`` with the reference's signatures.
+
+main.py:11-34 and energy/energy.py:17-81 of the reference.  Only the 'RHF'
+method runs on this engine (the other branches of ``Energy.calculate_energy``
+are outside the ERI + J/K path and raise ``NotImplementedError``).
+``symmetry=True`` is accepted; point-group quartet skipping is not
+performed (the result is the same energy, as in the reference).
+"""
+import time
+
+from .basis import nuclear_repulsion, read_basis_set_file, read_mol_file
+from .hartree_fock import RestrictedHF
+
+
+class Energy:
+    def __init__(self, electrons, multiplicity, processors, method, **options):
+        self.electrons = electrons
+        self.multiplicity = multiplicity
+        self.processors = processors
+        self.method = method
+        self.options = options
+        self.symmetry_object = None
+        self.last = None
+
+    def calculate_energy(self, nuclei_array, basis_set):
+        e_nuc = nuclear_repulsion(nuclei_array)
+        if self.method != 'RHF':
+            raise NotImplementedError('method %r is not provided by gaussium_amd (RHF only)' % (self.method,))
+        hf = RestrictedHF(nuclei_array, basis_set, self.electrons, self.symmetry_object, self.processors,
+                          **self.options)
+        electron_energy, correlation = hf.energies()
+        self.last = hf
+        return electron_energy + e_nuc + correlation
+
+
+def start(mol_file, basis_file, method, processors, symmetry=False, geometry_optimization=None, **options):
+    """Total energy of the molecule in ``mol_file`` with ``basis_file`` (main.py:11-34)."""
+    t0 = time.perf_counter()
+    nuclei, electrons, multiplicity = read_mol_file(mol_file)
+    energy_object = Energy(electrons, multiplicity, processors, method, **options)
+    if geometry_optimization is not None:
+        from .geometry import optimize
+        energy = optimize(basis_file, energy_object, nuclei, method=geometry_optimization)
+    else:
+        basis = read_basis_set_file(basis_file, nuclei)
+        energy = energy_object.calculate_energy(nuclei, basis)
+    energy_object.wall_time = time.perf_counter() - t0
+    return energy

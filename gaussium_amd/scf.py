@@ -1,0 +1,141 @@
+"""Restricted Hartree-Fock SCF on the device (FP64 torch tensors).
+
+Mirrors the reference's classes and their numerical conventions:
+
+* ``density_matrix_restricted``  P = 2 C_occ C_occ^T (matrixelements/density_matrix.py:4-9)
+* ``TotalEnergy.restricted``     1/2 sum P (H + F) (hartreefock/total_energy.py:16-35)
+* ``LinearAlgebra``              X = U s^-1/2 without cutoff, X^T F X, eigh,
+                                 ascending order, C = X C' (hartreefock/linear_algebra.py:4-31)
+* ``DIIS``                       e = S P F - F P S in the orthonormal basis; signed
+                                 on/off tests any(e < 0.1) / all(e < 1e-6); reset
+                                 after 8 vectors; pop the oldest vector when B is
+                                 singular (diismethod/diis.py:30-112).  B is also
+                                 treated as singular when cond(B) > 1e12: with very
+                                 few basis functions the error matrices are
+                                 parallel, B is singular to working precision and
+                                 a plain solve returns arbitrary coefficients.
+* ``RestrictedSCF``              core-H guess, stop at |dE| < 1e-12
+                                 (hartreefock/scf_procedure.py:17-42); ``max_iter``
+                                 adds the iteration cap the reference lacks.
+"""
+import numpy as np
+import torch
+
+DIIS_COND_LIMIT = 1e12
+
+
+def density_matrix_restricted(c, electrons):
+    occ = c[:, :electrons // 2]
+    return 2.0 * (occ @ occ.T)
+
+
+class TotalEnergy:
+    def __init__(self, core_hamiltonian):
+        self.core_hamiltonian = core_hamiltonian
+
+    def restricted(self, density_matrix, hamiltonian):
+        return 0.5 * float(torch.sum(density_matrix * (self.core_hamiltonian + hamiltonian)))
+
+
+class LinearAlgebra:
+    def __init__(self, orbital_overlap):
+        w, u = torch.linalg.eigh(orbital_overlap)
+        order = torch.argsort(w)
+        w, u = w[order], u[:, order]
+        self.smallest_overlap_eigenvalue = float(w[0])
+        self.transformation_matrix = u * w.rsqrt()
+
+    def diagonalize(self, fock_matrix):
+        x = self.transformation_matrix
+        e, c = torch.linalg.eigh(x.T @ fock_matrix @ x)
+        order = torch.argsort(e)
+        return e[order], x @ c[:, order]
+
+    def orthonormalize(self, matrix):
+        x = self.transformation_matrix
+        return x.T @ matrix @ x
+
+
+class DIIS:
+    def __init__(self, overlap, linear_algebra, diis_subspace=8, cond_limit=DIIS_COND_LIMIT):
+        self.cond_limit = cond_limit
+        self.overlap = overlap
+        self.linear_algebra = linear_algebra
+        self.fock_array = []
+        self.error_array = []
+        self.diis_subspace = diis_subspace
+        self.begin = False
+
+    def fock_matrix(self, fock, density):
+        error = self.overlap @ density @ fock - fock @ density @ self.overlap
+        error = self.linear_algebra.orthonormalize(error)
+        if bool(torch.any(error < 0.1)):
+            self.begin = True
+        if bool(torch.all(error < 1e-6)):
+            self.begin = False
+        if self.begin:
+            self.error_array.append(error)
+            self.fock_array.append(fock)
+            if len(self.fock_array) > self.diis_subspace:
+                self.fock_array = []
+                self.error_array = []
+                return fock
+            if len(self.fock_array) > 1:
+                coef = self.create_b_matrix()
+                fock = torch.zeros_like(fock)
+                for c, f in zip(coef.tolist(), self.fock_array):
+                    fock = fock + c * f
+        return fock
+
+    def create_b_matrix(self):
+        # B_ij = tr(e_i e_j^T) on the device; the (n+1)^2 <= 81-element solve on the host.
+        n = len(self.fock_array)
+        e = torch.stack(self.error_array).reshape(n, -1)
+        b = np.ones((n + 1, n + 1))
+        b[:n, :n] = (e @ e.T).cpu().numpy()
+        b[n, n] = 0.0
+        rhs = np.zeros(n + 1)
+        rhs[n] = 1.0
+        try:
+            if self.cond_limit is not None and np.linalg.cond(b) > self.cond_limit:
+                raise np.linalg.LinAlgError('ill-conditioned DIIS matrix')
+            return np.linalg.solve(b, rhs)[:n]
+        except np.linalg.LinAlgError:
+            self.fock_array.pop(0)
+            self.error_array.pop(0)
+            return self.create_b_matrix()
+
+
+class RestrictedSCF:
+    def __init__(self, linear_algebra, electrons, overlap, hamiltonian_matrix_factory, threshold=1e-12,
+                 max_iter=None, use_diis=True):
+        self.linear_algebra = linear_algebra
+        self.electrons = electrons
+        self.hamiltonian_matrix_factory = hamiltonian_matrix_factory
+        self.threshold = threshold
+        self.max_iter = max_iter
+        self.calculate = TotalEnergy(hamiltonian_matrix_factory.core_hamiltonian)
+        self.diis = DIIS(overlap, linear_algebra) if use_diis else None
+        self.iterations = 0
+        self.energy_history = []
+
+    def begin_iterations(self, orbital_energies, orbital_coefficients):
+        previous = 0.0
+        while True:
+            density = density_matrix_restricted(orbital_coefficients, self.electrons)
+            fock = self.hamiltonian_matrix_factory.create(density)
+            total = self.calculate.restricted(density, fock)
+            delta = previous - total
+            previous = total
+            self.iterations += 1
+            self.energy_history.append(total)
+            if abs(delta) < self.threshold:
+                break
+            if self.max_iter is not None and self.iterations >= self.max_iter:
+                break
+            if self.diis is not None:
+                fock = self.diis.fock_matrix(fock, density)
+            orbital_energies, orbital_coefficients = self.linear_algebra.diagonalize(fock)
+        self.density = density
+        self.fock = fock
+        return total, orbital_energies, orbital_coefficients

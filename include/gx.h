@@ -1,0 +1,139 @@
+/*
+ * gx -- MI355X (gfx950) two-electron-integral and J/K Fock-build engine.
+ *
+ * C ABI: plain pointers and sizes only.  Every call returns GX_OK (0) or a
+ * negative error code and then leaves a message for gx_last_error() (per
+ * host thread).  A context is bound to one device and one HIP stream; it is
+ * not thread-safe.  Unless stated otherwise a call returns after its GPU
+ * work has been enqueued on the context stream and, for host outputs, after
+ * that stream has drained.
+ *
+ * Reference interfaces replaced (paths relative to src/gaussium/ of the
+ * reference, ChiCheng45/Gaussium):
+ *   gx_set_basis      Basis / PrimitiveBasis data model,
+ *                     objects/basis_function.py:6-53,
+ *                     objects/primitive_basis_function.py:6-42 (normalisation
+ *                     is folded into coefs and bf_scale by the caller)
+ *   gx_eri_quartets   TwoElectronRepulsion.calculate_integral
+ *                     matrixelements/electron_repulsion_matrix.py:18-26 and
+ *                     ObaraSaika.integrate
+ *                     integrals/twoelectronrepulsion/obara_saika_scheme.py:17-82
+ *   gx_eri_dense      TwoElectronRepulsion.create_repulsion_matrix
+ *                     matrixelements/electron_repulsion_matrix.py:28-47
+ *   gx_jk             FockMatrixRestricted.create (the G = J - K/2 part)
+ *                     hartreefock/fock_matrix.py:12-22
+ *   gx_one_electron   OrbitalOverlapMatrix / KineticEnergyMatrix /
+ *                     NuclearAttractionMatrix .create
+ *                     matrixelements/orbital_overlap_matrix.py:15-27,
+ *                     matrixelements/kinetic_energy_matrix.py:16-33,
+ *                     matrixelements/nuclear_attraction_matrix.py:16-29
+ *   gx_grad_*         no counterpart (the reference has no gradients; they
+ *                     replace the function-count-driven Nelder-Mead search,
+ *                     geometryoptimization/nelder_mead.py:28-144)
+ */
+#ifndef GX_H
+#define GX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GX_OK 0
+#define GX_ERR_ARG (-1)      /* null pointer, bad size, unsupported input   */
+#define GX_ERR_HIP (-2)      /* a HIP runtime call failed                    */
+#define GX_ERR_STATE (-3)    /* call out of order (e.g. no basis set yet)    */
+#define GX_ERR_NOMEM (-4)    /* device allocation failed                     */
+
+/* gx_jk flags */
+#define GX_JK_MIXED 1        /* quartets with Schwarz bound < fp32_bound in FP32 */
+
+typedef struct gx_ctx gx_ctx;
+
+typedef struct gx_stats {
+  int64_t shell_quartets;       /* canonical shell quartets                       */
+  int64_t bf_quartets;          /* canonical basis-function quartets               */
+  int64_t shell_quartets_fp64;  /* surviving the cutoff, evaluated in FP64         */
+  int64_t shell_quartets_fp32;  /* surviving the cutoff, evaluated in FP32         */
+  int64_t bf_quartets_fp64;
+  int64_t bf_quartets_fp32;
+  double flops_fp64;            /* model flops of the evaluated FP64 quartets      */
+  double flops_fp32;            /* model flops of the evaluated FP32 quartets      */
+} gx_stats;
+
+/* Last error message of this host thread ("" if none). */
+const char* gx_last_error(void);
+
+/* Create a context on HIP device `device` with its own stream. */
+int gx_create(gx_ctx** ctx, int device);
+int gx_destroy(gx_ctx* ctx);
+
+/* Launch on `stream` (a hipStream_t, e.g. torch's current stream) instead
+ * of the context's own stream; NULL restores the context stream. */
+int gx_set_stream(gx_ctx* ctx, void* stream);
+
+/* Basis set as shells of Cartesian functions (all host arrays).
+ *   l[s], nprim[s], prim_off[s]  angular momentum, primitive count/offset
+ *   bf_off[s]                    index of the shell's first basis function;
+ *                                components follow in the order
+ *                                s | x,y,z | xx,yy,zz,xy,xz,yz
+ *   ctr[3s]                      centre used by the two-electron integrals
+ *   ctr1[3s]                     centre used by the one-electron integrals
+ *   atom[s]                      atom index of the shell (gradients)
+ *   exps[p], coefs[p]            primitive exponent and radial coefficient
+ *                                (contraction x primitive norm x contracted norm
+ *                                of the shell's first component)
+ *   bf_scale[nbf]                per-function factor relative to that component
+ * Builds primitive pairs and Schwarz bounds on the device. l <= 1 for all
+ * shells in this build. */
+int gx_set_basis(gx_ctx* ctx, int nshell, const int* l, const int* nprim, const int* prim_off,
+                 const int* bf_off, const double* ctr, const double* ctr1, const int* atom,
+                 int nprim_total, const double* exps, const double* coefs, int nbf,
+                 const double* bf_scale);
+
+/* Nuclei (host arrays): xyz[3*natom] in bohr, charge[natom]. */
+int gx_set_nuclei(gx_ctx* ctx, int natom, const double* xyz, const double* charge);
+
+/* Number of basis functions / shell pairs of the current basis. */
+int gx_nbf(gx_ctx* ctx);
+int gx_npair(gx_ctx* ctx);
+
+/* Schwarz bounds sqrt(max |(ab|ab)|) of all shell pairs, and the pair's
+ * shells, copied to host arrays of length gx_npair(). */
+int gx_schwarz(gx_ctx* ctx, double* Q, int* shellA, int* shellB);
+
+/* Values of n basis-function quartets ijkl[4n] (host) -> out[n] (host). */
+int gx_eri_quartets(gx_ctx* ctx, int64_t n, const int32_t* ijkl, double* out);
+
+/* Dense (ij|kl) tensor into a device buffer of nbf^4 doubles (row-major). */
+int gx_eri_dense(gx_ctx* ctx, double* out_dev);
+
+/* Direct J and K from the density P (all nbf x nbf FP64 device buffers):
+ *   J_ij = sum_kl (ij|kl) P_kl,  K_ij = sum_kl (ik|jl) P_kl.
+ * Shell quartets with Q_ab Q_cd < cutoff are skipped; with GX_JK_MIXED,
+ * those with Q_ab Q_cd < fp32_bound are evaluated in FP32 and accumulated
+ * into the FP64 J and K. */
+int gx_jk(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double cutoff,
+          double fp32_bound, int flags);
+
+/* Shell-quartet and flop counts that gx_jk would evaluate (host-side count). */
+int gx_jk_stats(gx_ctx* ctx, double cutoff, double fp32_bound, int flags, gx_stats* out);
+
+/* Per class combination (la lb|lc ld) of the last gx_jk-style sweep with these
+ * thresholds, one row of 10 int64 in `out` (at most `cap` rows):
+ *   la, lb, lc, ld, shell quartets FP64, shell quartets FP32,
+ *   primitive quartets FP64, primitive quartets FP32,
+ *   basis-function quartets FP64, basis-function quartets FP32.
+ * Returns the number of rows (>= 0) or an error code. */
+int gx_jk_class_counts(gx_ctx* ctx, double cutoff, double fp32_bound, int flags, int64_t* out, int cap);
+
+/* One-electron matrices S, T, V (nbf x nbf FP64 device buffers); V needs
+ * gx_set_nuclei first. */
+int gx_one_electron(gx_ctx* ctx, double* S_dev, double* T_dev, double* V_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GX_H */

@@ -1,0 +1,152 @@
+"""Restricted Hartree-Fock with DIIS, restated from the reference.
+
+* density  P = 2 sum_{i < n/2} c_i c_i^T        ``matrixelements/density_matrix.py:4-9``
+* Fock     F = H + G, G_ij = sum_ab P_ab [(ij|ab) - 1/2 (ib|aj)]
+                                               ``hartreefock/fock_matrix.py:5-22``
+* energy   E = 1/2 sum_ij P_ij (H_ij + F_ij)   ``hartreefock/total_energy.py:16-35``
+* X = U s^-1/2 (canonical, no cutoff), diagonalise via X^T F X, eigh,
+  ascending sort, C = X C'                     ``hartreefock/linear_algebra.py:4-31``
+* DIIS (signed on/off tests, 8-vector reset, pop-oldest on singular B)
+                                               ``diismethod/diis.py:30-112``
+* SCF loop: core-H guess, stop at |dE| < 1e-12 ``hartreefock/scf_procedure.py:17-42``,
+                                               ``hartreefock/hartree_fock.py:45-73``
+
+Test infrastructure only; see ``oracle/__init__.py``.
+"""
+import numpy as np
+
+
+def density_matrix_restricted(c, electrons):
+    """density_matrix.py:4-9."""
+    occ = c[:, :electrons // 2]
+    return 2.0 * occ @ occ.T
+
+
+def jk_from_dense(eri, p):
+    """J_ij = sum_kl (ij|kl) P_kl and K_ij = sum_kl (ik|jl) P_kl from a dense tensor."""
+    j = np.einsum('ijkl,kl->ij', eri, p, optimize=True)
+    k = np.einsum('ikjl,kl->ij', eri, p, optimize=True)
+    return j, k
+
+
+def fock_restricted_loop(h, eri, p):
+    """F = H + G with the reference's element loop (fock_matrix.py:12-22); small N only."""
+    n = h.shape[0]
+    g = np.zeros_like(h)
+    for i in range(n):
+        for j in range(i, n):
+            s = 0.0
+            for a in range(n):
+                for b in range(n):
+                    s += p[a, b] * (eri[i, j, a, b] - 0.5 * eri[i, b, a, j])
+            g[i, j] = g[j, i] = s
+    return h + g
+
+
+def total_energy_restricted(h, p, f):
+    """total_energy.py:16-35."""
+    return 0.5 * float(np.sum(p * (h + f)))
+
+
+class LinearAlgebra:
+    """linear_algebra.py:4-31."""
+
+    def __init__(self, s):
+        w, u = np.linalg.eigh(s)
+        order = np.argsort(w)
+        w, u = w[order], u[:, order]
+        self.x = u @ np.diag(w ** -0.5)
+
+    def diagonalize(self, f):
+        fp = self.x.T @ f @ self.x
+        e, c = np.linalg.eigh(fp)
+        order = np.argsort(e)
+        return e[order], self.x @ c[:, order]
+
+    def orthonormalize(self, m):
+        return self.x.T @ m @ self.x
+
+
+class DIIS:
+    """diis.py:30-112.
+
+    ``cond_limit``: None keeps the reference's test (only an exactly singular
+    B raises LinAlgError); a number also treats cond(B) > cond_limit as
+    singular, which triggers the reference's own recovery (drop the oldest
+    vector, diis.py:105-112).  With two basis functions the error matrices
+    are all parallel and B is singular to working precision (cond ~ 1e18),
+    where LAPACK returns arbitrary coefficients.
+    """
+
+    def __init__(self, s, la, subspace=8, cond_limit=None):
+        self.cond_limit = cond_limit
+        self.s = s
+        self.la = la
+        self.focks = []
+        self.errors = []
+        self.subspace = subspace
+        self.begin = False
+
+    def fock_matrix(self, f, p):
+        e = self.s @ p @ f - f @ p @ self.s
+        e = self.la.orthonormalize(e)
+        if np.any(e < 0.1):
+            self.begin = True
+        if np.all(e < 1e-6):
+            self.begin = False
+        if self.begin:
+            self.errors.append(e)
+            self.focks.append(f)
+            if len(self.focks) > self.subspace:
+                self.focks = []
+                self.errors = []
+                return f
+            if len(self.focks) > 1:
+                coef = self._coefficients()
+                f = np.zeros_like(f)
+                for c, fk in zip(coef, self.focks):
+                    f = f + c * fk
+        return f
+
+    def _coefficients(self):
+        n = len(self.focks)
+        b = np.zeros((n + 1, n + 1))
+        rhs = np.zeros(n + 1)
+        rhs[n] = 1.0
+        for i in range(n):
+            b[i, n] = b[n, i] = 1.0
+            for j in range(n):
+                b[i, j] = np.trace(self.errors[i] @ self.errors[j].T)
+        try:
+            if self.cond_limit is not None and np.linalg.cond(b) > self.cond_limit:
+                raise np.linalg.LinAlgError('ill-conditioned DIIS matrix')
+            return np.linalg.solve(b, rhs)[:n]
+        except np.linalg.LinAlgError:
+            self.focks.pop(0)
+            self.errors.pop(0)
+            return self._coefficients()
+
+
+def rhf(h, s, electrons, fock_builder, threshold=1e-12, max_iter=None, diis_cond_limit=None):
+    """Restricted SCF (scf_procedure.py:24-42, hartree_fock.py:45-73).
+
+    ``fock_builder(P) -> F``.  Returns (E_elec, orbital energies, C, iterations).
+    ``max_iter`` is None for the reference behaviour (no cap).
+    """
+    la = LinearAlgebra(s)
+    diis = DIIS(s, la, cond_limit=diis_cond_limit)
+    eps, c = la.diagonalize(h)
+    prev = 0.0
+    it = 0
+    while True:
+        p = density_matrix_restricted(c, electrons)
+        f = fock_builder(p)
+        e = total_energy_restricted(h, p, f)
+        de = prev - e
+        prev = e
+        it += 1
+        if abs(de) < threshold or (max_iter is not None and it >= max_iter):
+            break
+        f = diis.fock_matrix(f, p)
+        eps, c = la.diagonalize(f)
+    return e, eps, c, it

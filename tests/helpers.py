@@ -1,0 +1,88 @@
+"""Shared builders for the tests: reference mock systems and oracle wrappers."""
+import os
+
+import numpy as np
+
+from tests.conftest import GOLDEN
+
+
+class Obj:
+    """Attribute bag standing in for the reference's MagicMock fixtures."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def heh_sto3g_mock(he=(0.0, 0.0, 0.7316), h=(0.0, 0.0, -0.7316), first='He'):
+    """HeH+ STO-3G with zeta-scaled He exponents, as in the reference ERI tests
+    (tests/testsfast/matrixelements/test_electron_repulsion_matrix.py:11-49)."""
+    h_rows = [(0.15432897, 3.42525091, 1.794441832218435), (0.53532814, 0.62391373, 0.5003264923314032),
+         (0.44463454, 0.16885540, 0.18773545851092535)]
+    he_rows = [(0.15432897, 9.75393461, 3.9336432656254527), (0.53532814, 1.77669115, 1.0967787981767012),
+          (0.44463454, 0.48084429, 0.41154131374122654)]
+
+    def fn(rows, c):
+        prims = [Obj(contraction=a, exponent=b, coordinates=c, integral_exponents=(0, 0, 0), normalisation=n)
+                 for a, b, n in rows]
+        return Obj(primitive_gaussian_array=prims, coordinates=c, integral_exponents=(0, 0, 0), normalisation=1)
+
+    helium = fn(he_rows, he)
+    hydrogen = fn(h_rows, h)
+    return [helium, hydrogen] if first == 'He' else [hydrogen, helium]
+
+
+def golden(name):
+    return os.path.join(GOLDEN, name)
+
+
+def load_system(mol, gbs):
+    from gaussium_amd import read_basis_set_file, read_mol_file
+    nuc, ne, mult = read_mol_file(golden(mol))
+    basis = read_basis_set_file(golden(gbs), nuc)
+    return nuc, basis, ne
+
+
+def oracle_rhf(nuclei, basis, electrons, eri=None, max_iter=None, exact_boys=True, diis_cond_limit=1e12):
+    """Oracle RHF total energy from the C-oracle dense tensor and the oracle S/T/V.
+
+    ``exact_boys`` evaluates the Boys function exactly instead of with the
+    reference's truncated series; ``diis_cond_limit`` matches the product's
+    treatment of a numerically singular DIIS matrix."""
+    from oracle import eri_c
+    from oracle.basis import Nuclei, nuclear_repulsion
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from oracle.scf import jk_from_dense, rhf
+
+    from oracle import boys as oboys
+    if eri is None:
+        if exact_boys:
+            with eri_c.exact_boys():
+                eri = eri_c.eri_dense(basis)
+        else:
+            eri = eri_c.eri_dense(basis)
+    onuc = [Nuclei(n.element, n.charge, n.mass, n.coordinates) for n in nuclei]
+    S = overlap_matrix(basis)
+    with oboys.exact(exact_boys):
+        H = kinetic_matrix(basis) + nuclear_matrix(basis, onuc)
+
+    def fb(P):
+        J, K = jk_from_dense(eri, P)
+        return H + J - 0.5 * K
+
+    e, eps, c, it = rhf(H, S, electrons, fb, max_iter=max_iter, diis_cond_limit=diis_cond_limit)
+    return e + nuclear_repulsion(onuc), it
+
+
+def canonical_sample(n, count, rng):
+    """Random canonical (a<=b, c<=d, (a,b)<=(c,d)) quartets."""
+    out = set()
+    while len(out) < count:
+        a, b, c, d = (int(x) for x in rng.integers(0, n, 4))
+        if a > b:
+            a, b = b, a
+        if c > d:
+            c, d = d, c
+        if (a, b) > (c, d):
+            a, b, c, d = c, d, a, b
+        out.add((a, b, c, d))
+    return np.array(sorted(out), dtype=np.int32)

@@ -1,0 +1,100 @@
+// Host (CPU) build of the generated class code: evaluates contracted shell
+// quartets with the same VRR/HRR source the GPU kernels compile, so that the
+// generated recursions can be checked against the oracle without a GPU.
+// Boys values come from a long-double series here (the device table is
+// checked separately on the GPU).
+#include <cmath>
+#include <cstdint>
+
+#define __device__
+#define __forceinline__ inline
+#include "eri_gen.h"
+
+static void boys_host(int L, double T, double scale, double* F) {
+  const int M = L;
+  long double term = 1.0L / (2 * M + 1), sum = term;
+  for (int k = 1; k < 5000; ++k) {
+    term *= 2.0L * T / (2 * M + 2 * k + 1);
+    sum += term;
+    if (term < sum * 1e-22L) break;
+  }
+  const long double e = expl(-(long double)T);
+  long double f = e * sum;
+  F[M] = (double)(f * scale);
+  for (int m = M; m >= 1; --m) {
+    f = (2.0L * T * f + e) / (2 * m - 1);
+    F[m - 1] = (double)(f * scale);
+  }
+}
+
+struct Shell {
+  double c[3];
+  int n;
+  const double* a;
+  const double* w;
+};
+
+template <int LA, int LB, int LC, int LD>
+static void quartet(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
+  using K = EriClass<LA, LB, LC, LD>;
+  double acc[K::NACC] = {0};
+  const double two_pi_52 = 34.98683665524972;
+  for (int i = 0; i < A.n; ++i)
+    for (int j = 0; j < B.n; ++j) {
+      const double a = A.a[i], b = B.a[j], p = a + b;
+      double P[3], ab2 = 0;
+      for (int x = 0; x < 3; ++x) {
+        P[x] = (a * A.c[x] + b * B.c[x]) / p;
+        ab2 += (A.c[x] - B.c[x]) * (A.c[x] - B.c[x]);
+      }
+      const double Kab = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2);
+      for (int k = 0; k < C.n; ++k)
+        for (int l = 0; l < D.n; ++l) {
+          const double c = C.a[k], d = D.a[l], q = c + d;
+          double Q[3], cd2 = 0, pq2 = 0;
+          for (int x = 0; x < 3; ++x) {
+            Q[x] = (c * C.c[x] + d * D.c[x]) / q;
+            cd2 += (C.c[x] - D.c[x]) * (C.c[x] - D.c[x]);
+            pq2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+          }
+          const double Kcd = C.w[k] * D.w[l] * std::exp(-c * d / q * cd2);
+          const double rho = p * q / (p + q);
+          const double pref = two_pi_52 / (p * q * std::sqrt(p + q)) * Kab * Kcd;
+          double F[K::L + 1];
+          boys_host(K::L, rho * pq2, pref, F);
+          const double rp = rho / p, rq = rho / q;
+          double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+          K::template prim<double>(acc, F, P[0] - A.c[0], P[1] - A.c[1], P[2] - A.c[2], -rp * PQ[0],
+                                   -rp * PQ[1], -rp * PQ[2], Q[0] - C.c[0], Q[1] - C.c[1],
+                                   Q[2] - C.c[2], rq * PQ[0], rq * PQ[1], rq * PQ[2], 0.5 / p,
+                                   0.5 / q, 0.5 / (p + q), rp, rq);
+        }
+    }
+  double o[K::NOUT];
+  K::template hrr<double>(acc, A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2], C.c[0] - D.c[0],
+                          C.c[1] - D.c[1], C.c[2] - D.c[2], o);
+  for (int t = 0; t < K::NOUT; ++t) out[t] = o[t];
+}
+
+extern "C" int host_quartet(const int* l, const double* ctr, const int* n, const double* a,
+                            const double* w, double* out) {
+  Shell s[4];
+  int off = 0;
+  for (int k = 0; k < 4; ++k) {
+    s[k].c[0] = ctr[3 * k];
+    s[k].c[1] = ctr[3 * k + 1];
+    s[k].c[2] = ctr[3 * k + 2];
+    s[k].n = n[k];
+    s[k].a = a + off;
+    s[k].w = w + off;
+    off += n[k];
+  }
+#define GX_CASE(A, B, C, D)                                                         \
+  if (l[0] == A && l[1] == B && l[2] == C && l[3] == D) {                           \
+    quartet<A, B, C, D>(s[0], s[1], s[2], s[3], out);                               \
+    return 0;                                                                       \
+  }
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return -1;
+}

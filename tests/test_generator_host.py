@@ -1,0 +1,105 @@
+"""The generated VRR/HRR class code, compiled for the host with g++, against the C oracle.
+
+Every class up to (dd|dd) with random centres, exponents and contractions;
+the oracle is the independent C restatement of the reference's recursive
+Obara-Saika scheme (oracle/c/os_eri.c).  Tolerance 1e-10 relative to the
+largest integral of the quartet.
+"""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+HARNESS = os.path.join(ROOT, 'tests', 'host_harness')
+BUILD = os.path.join(ROOT, 'tests', 'host_harness', '_build')
+
+COMPS = {0: [(0, 0, 0)], 1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
+         2: [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)]}
+
+
+@pytest.fixture(scope='module')
+def host_lib():
+    os.makedirs(BUILD, exist_ok=True)
+    gen = os.path.join(BUILD, 'eri_gen.h')
+    subprocess.run([sys.executable, os.path.join(ROOT, 'gaussium_amd', 'csrc', 'gen_eri.py'), gen, '2'],
+                   check=True)
+    so = os.path.join(BUILD, 'liberi_host.so')
+    subprocess.run(['g++', '-O1', '-std=c++17', '-shared', '-fPIC', '-I', BUILD,
+                    os.path.join(HARNESS, 'eri_host.cpp'), '-o', so], check=True)
+    lib = ctypes.CDLL(so)
+    lib.host_quartet.restype = ctypes.c_int
+    return lib
+
+
+def _shell(rng, l, center):
+    n = int(rng.integers(1, 4))
+    a = np.exp(rng.uniform(np.log(0.1), np.log(20.0), n))
+    w = rng.uniform(0.1, 1.0, n)
+    return l, np.asarray(center, float), a, w
+
+
+def _oracle(shells):
+    """Dense block via the C oracle: one pseudo-basis function per component."""
+    from oracle import eri_c
+
+    class P:
+        pass
+
+    basis = []
+    for (l, c, a, w) in shells:
+        for comp in COMPS[l]:
+            b = P()
+            b.coordinates = tuple(c)
+            b.integral_exponents = comp
+            b.normalisation = 1.0
+            prims = []
+            for ai, wi in zip(a, w):
+                g = P()
+                g.exponent = ai
+                g.contraction = wi
+                g.normalisation = 1.0
+                prims.append(g)
+            b.primitive_gaussian_array = prims
+            basis.append(b)
+    offs = np.cumsum([0] + [len(COMPS[s[0]]) for s in shells])
+    keys = []
+    for i in range(offs[0], offs[1]):
+        for j in range(offs[1], offs[2]):
+            for k in range(offs[2], offs[3]):
+                for m in range(offs[3], offs[4]):
+                    keys.append((i, j, k, m))
+    return eri_c.eri_keys(basis, keys)
+
+
+CLASSES = [(la, lb, lc, ld) for la in range(3) for lb in range(la + 1)
+           for lc in range(3) for ld in range(lc + 1)
+           if (la * (la + 1) // 2 + lb) >= (lc * (lc + 1) // 2 + ld)]
+
+
+@pytest.mark.parametrize('cls', CLASSES, ids=lambda c: '%d%d%d%d' % c)
+def test_generated_class_matches_oracle(host_lib, cls):
+    rng = np.random.default_rng(sum(v * 10 ** i for i, v in enumerate(cls)) + 7)
+    for trial in range(2):
+        centers = rng.uniform(-1.5, 1.5, (4, 3))
+        if trial == 1:
+            centers[1] = centers[0]  # same-centre pair exercises the zero-skips
+        shells = [_shell(rng, l, c) for l, c in zip(cls, centers)]
+        l = np.array([s[0] for s in shells], dtype=np.int32)
+        ctr = np.ascontiguousarray(np.concatenate([s[1] for s in shells]))
+        n = np.array([len(s[2]) for s in shells], dtype=np.int32)
+        a = np.ascontiguousarray(np.concatenate([s[2] for s in shells]))
+        w = np.ascontiguousarray(np.concatenate([s[3] for s in shells]))
+        nout = int(np.prod([len(COMPS[x]) for x in cls]))
+        out = np.zeros(nout)
+        rc = host_lib.host_quartet(l.ctypes.data_as(ctypes.c_void_p), ctr.ctypes.data_as(ctypes.c_void_p),
+                                   n.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
+                                   w.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+        assert rc == 0
+        ref = _oracle(shells)
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(out - ref).max() <= 1e-10 * scale, (cls, trial, np.abs(out - ref).max())

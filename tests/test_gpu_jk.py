@@ -1,0 +1,155 @@
+"""GPU parity of the fused direct J/K build, one-electron integrals and RHF.
+
+Bars: J/K within 1e-10 of the oracle (einsum over the oracle's dense
+tensor, equivalent to hartreefock/fock_matrix.py:17-19 of the reference);
+RHF energies within 1e-8 Eh of the oracle SCF; the reference's published
+6-significant-digit energies; mixed precision within 1e-6 Eh of FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+from tests.helpers import heh_sto3g_mock, load_system, oracle_rhf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def Engine():
+    from gaussium_amd.engine import Engine
+    return Engine
+
+
+def _random_density(n, rng):
+    c = rng.standard_normal((n, n // 2 + 1))
+    return c @ c.T / n
+
+
+def _oracle_jk(basis, P, exact=True):
+    """J/K from the oracle's dense tensor (exact Boys function unless ``exact`` is False)."""
+    from oracle import eri_c
+    from oracle.scf import jk_from_dense
+    if exact:
+        with eri_c.exact_boys():
+            return jk_from_dense(eri_c.eri_dense(basis), P)
+    return jk_from_dense(eri_c.eri_dense(basis), P)
+
+
+@pytest.mark.parametrize('system', [('HeH+.mol', '6-311+GPP.gbs'), ('C2H4.mol', '3-21G.gbs')])
+def test_jk_matches_oracle(Engine, system):
+    nuc, basis, _ = load_system(*system)
+    eng = Engine(basis, nuc)
+    P = _random_density(len(basis), np.random.default_rng(3))
+    J, K = eng.jk(torch.from_numpy(P).cuda())
+    Jr, Kr = _oracle_jk(basis, P)
+    assert np.abs(J.cpu().numpy() - Jr).max() < 1e-10
+    assert np.abs(K.cpu().numpy() - Kr).max() < 1e-10
+    # against the reference's truncated Boys series the contraction sums its
+    # ~1e-11 per-integral deviations
+    Jr, Kr = _oracle_jk(basis, P, exact=False)
+    assert np.abs(J.cpu().numpy() - Jr).max() < 1e-8
+    assert np.abs(K.cpu().numpy() - Kr).max() < 1e-8
+
+
+def test_jk_cluster_66bf(Engine):
+    from gaussium_amd import workloads
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    P = _random_density(len(basis), np.random.default_rng(5))
+    J, K = eng.jk(torch.from_numpy(P).cuda())
+    Jr, Kr = _oracle_jk(basis, P)
+    assert np.abs(J.cpu().numpy() - Jr).max() < 1e-10
+    assert np.abs(K.cpu().numpy() - Kr).max() < 1e-10
+    # with the Schwarz cutoff the error stays at the cutoff's scale
+    Js, Ks = eng.jk(torch.from_numpy(P).cuda(), cutoff=1e-11)
+    assert np.abs(Js.cpu().numpy() - Jr).max() < 1e-8
+    assert np.abs(Ks.cpu().numpy() - Kr).max() < 1e-8
+
+
+def test_one_electron_reference_values(Engine):
+    """S, T, V of the reference's HeH+ mocks (tests/testsfast/matrixelements/test_*_matrix.py)."""
+    from gaussium_amd.basis import Nuclei
+    basis = heh_sto3g_mock()
+    # the reference's S/T tests place the contracted centres opposite to the
+    # primitive centres; the one-electron integrals use the primitive centres
+    nuc = [Nuclei('HELIUM', 2, 4, (0.0, 0.0, 0.7316)), Nuclei('HYDROGEN', 1, 1, (0.0, 0.0, -0.7316))]
+    eng = Engine(basis, nuc)
+    S, T, V = (x.cpu().numpy() for x in eng.one_electron())
+    np.testing.assert_approx_equal(S[0, 1], 0.4508, 3)
+    np.testing.assert_approx_equal(T[0, 0], 2.1643, 5)
+    np.testing.assert_approx_equal(T[0, 1], 0.167, 3)
+    np.testing.assert_approx_equal(T[1, 1], 0.7600, 4)
+    np.testing.assert_approx_equal(V[0, 0], -4.817, 4)
+    np.testing.assert_approx_equal(V[0, 1], -1.5142, 3)
+    np.testing.assert_approx_equal(V[1, 1], -2.4912, 4)
+
+
+@pytest.mark.parametrize('system', [('HeH+.mol', '6-311+GPP.gbs'), ('C2H4.mol', '3-21G.gbs')])
+def test_one_electron_matches_oracle(Engine, system):
+    from oracle.basis import Nuclei as ONuc
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    nuc, basis, _ = load_system(*system)
+    eng = Engine(basis, nuc)
+    S, T, V = (x.cpu().numpy() for x in eng.one_electron())
+    from oracle.boys import exact
+    onuc = [ONuc(n.element, n.charge, n.mass, n.coordinates) for n in nuc]
+    assert np.abs(S - overlap_matrix(basis)).max() < 1e-12
+    assert np.abs(T - kinetic_matrix(basis)).max() < 1e-11
+    with exact():
+        assert np.abs(V - nuclear_matrix(basis, onuc)).max() < 1e-10
+    # the reference's Boys series (terms below 1e-10 dropped) moves V by < 1e-9
+    assert np.abs(V - nuclear_matrix(basis, onuc)).max() < 1e-9
+
+
+@pytest.mark.parametrize('system,published', [
+    (('HeH+.mol', 'STO-3G.gbs'), -2.84183608212),
+    (('HeH+.mol', '6-311+GPP.gbs'), -2.92922773384),
+    (('C2H4.mol', '3-21G.gbs'), -77.600460844),
+])
+def test_start_rhf_energy(system, published):
+    """start() end to end (tests/testslow/start/test_start.py:9-25 of the reference)."""
+    from gaussium_amd import start
+    from tests.helpers import golden
+    e = start(golden(system[0]), golden(system[1]), 'RHF', 1)
+    np.testing.assert_approx_equal(e, published, 6)
+    nuc, basis, ne = load_system(*system)
+    e_ref, _ = oracle_rhf(nuc, basis, ne)
+    assert abs(e - e_ref) < 1e-8
+
+
+def test_mock_heh_electronic_energy():
+    """tests/testsfast/hartreefock/test_hatree_fock.py:59-61 of the reference: -4.227529."""
+    from gaussium_amd.basis import Nuclei
+    from gaussium_amd.hartree_fock import RestrictedHF
+    basis = heh_sto3g_mock(he=(0.0, 0.0, 0.0), h=(0.0, 0.0, 1.4632), first='H')
+    nuc = [Nuclei('HYDROGEN', 1, 1, (0.0, 0.0, 1.4632)), Nuclei('HELIUM', 2, 4, (0.0, 0.0, 0.0))]
+    hf = RestrictedHF(nuc, basis, 2)
+    np.testing.assert_approx_equal(hf.begin_scf()[0], -4.227529, 6)
+
+
+def test_triatomic_rhf_20_iterations_matches_oracle():
+    from gaussium_amd import workloads
+    from gaussium_amd.basis import nuclear_repulsion
+    from gaussium_amd.hartree_fock import RestrictedHF
+    nuc, basis, ne, info = workloads.triatomic_7bf()
+    hf = RestrictedHF(nuc, basis, ne, max_iter=20)
+    e = hf.energies()[0] + nuclear_repulsion(nuc)
+    e_ref, it = oracle_rhf(nuc, basis, ne, max_iter=20)
+    assert abs(e - e_ref) < 1e-8
+    hf2 = RestrictedHF(nuc, basis, ne)
+    e2 = hf2.energies()[0] + nuclear_repulsion(nuc)
+    e_ref2, it2 = oracle_rhf(nuc, basis, ne)
+    assert abs(e2 - e_ref2) < 1e-8
+
+
+def test_mixed_precision_jk_error_is_small(Engine):
+    from gaussium_amd import workloads
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    P = torch.from_numpy(_random_density(len(basis), np.random.default_rng(9))).cuda()
+    J, K = eng.jk(P, cutoff=1e-11)
+    Jm, Km = eng.jk(P, cutoff=1e-11, fp32_bound=1e-4, mixed=True)
+    st = eng.jk_stats(cutoff=1e-11, fp32_bound=1e-4, mixed=True)
+    assert st['shell_quartets_fp32'] > 0
+    assert (Jm - J).abs().max().item() < 1e-8
+    assert (Km - K).abs().max().item() < 1e-8
