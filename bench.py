@@ -100,7 +100,8 @@ def main():
     scf = {}
     for mode, kw in (('fp64', dict(precision='fp64')), ('mixed', dict(precision='mixed', fp32_bound=FP32_BOUND))):
         t0 = time.perf_counter()
-        hf = RestrictedHF(nuc, basis, ne, engine=eng, schwarz_cutoff=CUTOFF, max_iter=100, **kw)
+        hf = RestrictedHF(nuc, basis, ne, engine=eng, schwarz_cutoff=CUTOFF, max_iter=100, threshold=1e-9,
+                          **kw)
         e = hf.energies()[0] + e_nuc
         scf[mode] = {'energy': e, 'iterations': hf.scf_method.iterations, 'wall_s': time.perf_counter() - t0,
                      'density': hf.scf_method.density}
@@ -111,8 +112,9 @@ def main():
     mixed = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
     st64 = eng.jk_stats(cutoff=CUTOFF)
     stmx = eng.jk_stats(cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
-    f64_a, _ = jk_model_flops(class_rows(eng, False))
-    fm64, fm32 = jk_model_flops(class_rows(eng, True))
+    rows64, rowsmx = class_rows(eng, False), class_rows(eng, True)
+    f64_a, _ = jk_model_flops(rows64)
+    fm64, fm32 = jk_model_flops(rowsmx)
 
     ok = dE < 1e-6
     head = mixed if ok else fp64
@@ -161,6 +163,8 @@ def main():
                                                           'frac': f64_a / (fp64['median_ms'] * 1e-3) / 1e12
                                                           / PEAK_FP64_TFLOPS}},
     }
+    line['class_counts'] = {'columns': ['la', 'lb', 'lc', 'ld', 'shq64', 'shq32', 'primq64', 'primq32', 'bfq64',
+                                        'bfq32'], 'fp64_build': rows64, 'mixed_build': rowsmx}
     traffic_file = os.path.join(ROOT, 'profiles', 'jk_220bf_traffic.json')
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:

@@ -128,7 +128,8 @@ struct gx_ctx {
   double* d_Q = nullptr;
   gx::PrimPair64* d_pp64 = nullptr;
   gx::PrimPair32* d_pp32 = nullptr;
-  std::vector<int*> d_order;  // per class combo
+  std::vector<int*> d_order;  // per class combo: bra positions, small-Kp first
+  std::vector<int> n_small;    // per class combo: bras with <= JK_MAXBRA primitive pairs
   // ket segments: per class, runs of pairs with the same primitive-count key
   std::vector<int> seg_lo, seg_hi, cls_seg;  // cls_seg: NCLS + 1 offsets into seg_*
   int *d_seg_lo = nullptr, *d_seg_hi = nullptr;
@@ -229,19 +230,20 @@ hipError_t launch_quartets(int la, int lb, int lc, int ld, dim3 g, hipStream_t s
   return hipErrorInvalidValue;
 }
 
+template <bool BIG>
 hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hipStream_t s,
                      const gx::JKArgs& a) {
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
-    const void* fn = (const void*)gx::k_jk<A, B, C, D>;                                      \
+    const void* fn = (const void*)gx::k_jk<A, B, C, D, BIG>;                                 \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
     int per_cu = 0;                                                                           \
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, JK_THREADS, lds);           \
     if (e != hipSuccess) return e;                                                            \
     if (per_cu < 1) per_cu = 1;                                                               \
-    const int grid = std::min(a.nbra, per_cu * num_cu);                                      \
-    hipLaunchKernelGGL((gx::k_jk<A, B, C, D>), dim3(grid), dim3(JK_THREADS), lds, s, a);      \
+    const int grid = std::min(a.nbra, per_cu * num_cu);                                       \
+    hipLaunchKernelGGL((gx::k_jk<A, B, C, D, BIG>), dim3(grid), dim3(JK_THREADS), lds, s, a); \
     return hipGetLastError();                                                                 \
   }
   GX_FOR_EACH_CLASS(GX_CASE)
@@ -249,10 +251,11 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
   return hipErrorInvalidValue;
 }
 
-size_t jk_lds_bytes(int la, int lb, int N, int mixed) {
+size_t jk_lds_bytes(int la, int lb, int N, int mixed, bool big) {
   const int na = ncart(la), nb = ncart(lb);
   size_t b = (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
   if (mixed) b += (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float);
+  if (!big) b += (size_t)JK_MAXBRA * (sizeof(gx::BraRec64) + (mixed ? sizeof(gx::BraRec32) : 0));
   b += 2 * (size_t)(na + nb) * N * sizeof(double);
   return b;
 }
@@ -360,6 +363,7 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
     return GX_OK;
   for (auto& p : c->d_order) dfree(&p);
   c->d_order.assign(combo_count(), nullptr);
+  c->n_small.assign(combo_count(), 0);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
@@ -374,7 +378,15 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
       });
       std::vector<int> order(nb);
       std::iota(order.begin(), order.end(), 0);
-      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+      const int b0 = c->cls_start[X];
+      auto big = [&](int i) { return c->pnp[b0 + i] > JK_MAXBRA; };
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (big(a) != big(b)) return !big(a);
+        return cost[a] > cost[b];
+      });
+      int ns = 0;
+      while (ns < nb && !big(order[ns])) ++ns;
+      c->n_small[combo] = ns;
       GX_TRY(upload(&c->d_order[combo], order, c->stream));
     }
   c->ord_cutoff = cutoff;
@@ -755,7 +767,7 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   const long nn = (long)N * N;
   const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
   for (int X = 0; X < NCLS; ++X) {
-    const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), N, mixed);
+    const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), N, mixed, false);
     if (c->cls_start[X + 1] > c->cls_start[X] && lds > (size_t)c->max_lds)
       return fail(GX_ERR_ARG, "gx_jk: basis too large for the LDS row cache (nbf=" + std::to_string(N) +
                                   ", needs " + std::to_string(lds) + " bytes)");
@@ -791,8 +803,18 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       a.cutoff = cutoff;
       a.fp32_bound = fp32_bound;
       a.mixed = mixed;
-      GX_HIP(launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
-                       jk_lds_bytes(class_la(X), class_lb(X), N, mixed), c->stream, a));
+      const int ns = c->n_small[combo];
+      if (ns > 0) {
+        a.nbra = ns;
+        GX_HIP(launch_jk<false>(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
+                                jk_lds_bytes(class_la(X), class_lb(X), N, mixed, false), c->stream, a));
+      }
+      if (ns < nb) {
+        a.order = c->d_order[combo] + ns;
+        a.nbra = nb - ns;
+        GX_HIP(launch_jk<true>(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
+                               jk_lds_bytes(class_la(X), class_lb(X), N, mixed, true), c->stream, a));
+      }
     }
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, c->d_Jt,
                      c->d_Kt, J, K);

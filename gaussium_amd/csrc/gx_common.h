@@ -21,8 +21,8 @@
 // Boys-function interpolation grid.
 #define BOYS_DT 0.1
 #define BOYS_INV_DT 10.0
-#define BOYS_NT 481          // rows T_n = 0, 0.1, .., 48.0 (nearest row of any T < 48)
-#define BOYS_TMAX 48.0
+#define BOYS_NT 361          // rows T_n = 0, 0.1, .., 36.0 (nearest row of any T < 36)
+#define BOYS_TMAX 36.0
 #define BOYS_MCOL 20         // F_0 .. F_19 stored (L + 7 <= 19)
 #define BOYS_TAYLOR64 8      // FP64: 8 Taylor terms, |dT| <= 0.05 -> < 1e-15 rel.
 #define BOYS_TAYLOR32 5      // FP32: 5 Taylor terms -> < 3e-9 rel.
@@ -87,9 +87,10 @@ __device__ __forceinline__ float gx_rsqrt(float x) { return __builtin_amdgcn_rsq
 //   T < BOYS_TMAX : F_L by Taylor about the nearest grid point, exp(-T) from
 //                   exp(-T_n) times a Taylor series, then the downward
 //                   recursion F_{m-1} = (2T F_m + e^-T) / (2m - 1)
-//   T >= BOYS_TMAX: F_0 = sqrt(pi / T) / 2 (erf(sqrt T) = 1 to < 1e-21) and
-//                   the upward recursion F_{m+1} = (2m + 1) F_m / (2T)
-//                   (the dropped e^-T term is < 2e-21)
+//   T >= BOYS_TMAX: F_0 = sqrt(pi / T) / 2 (relative error erfc(sqrt T) < 3e-17)
+//                   and the upward recursion F_{m+1} = (2m + 1) F_m / (2T); the
+//                   dropped e^-T / (2T) term leaves every F_m within 4e-18
+//                   absolute of the exact value
 // Every F_m is multiplied by `scale` (the quartet prefactor).
 template <int L>
 __device__ __forceinline__ void boys64(double T, double scale, const double* __restrict__ tab,
@@ -123,8 +124,9 @@ __device__ __forceinline__ void boys64(double T, double scale, const double* __r
       F[m - 1] = f * scale;
     }
   } else {
-    const double it = gx_rcp(T);
-    double f = 0.88622692545275801365 * sqrt(it);  // sqrt(pi)/2
+    const double rt = gx_rsqrt(T);
+    const double it = rt * rt;
+    double f = 0.88622692545275801365 * rt;  // sqrt(pi)/2 / sqrt(T)
     F[0] = f * scale;
 #pragma unroll
     for (int m = 0; m < L; ++m) {
@@ -155,8 +157,9 @@ __device__ __forceinline__ void boys32(float T, float scale, const float* __rest
       }
     }
   } else {
-    const float it = gx_rcp(T);
-    float f = 0.886226925f * sqrtf(it);
+    const float rt = gx_rsqrt(T);
+    const float it = rt * rt;
+    float f = 0.886226925f * rt;
     F[0] = f * scale;
 #pragma unroll
     for (int m = 0; m < L; ++m) {

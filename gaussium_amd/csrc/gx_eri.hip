@@ -140,6 +140,89 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 }
 
 // ---------------------------------------------------------------------------
+// J/K evaluators.  The bra shell pair is uniform across a workgroup, so its
+// primitive pairs are staged once per bra in LDS (with P - A and K/p
+// precomputed) and read as broadcasts; each lane loads its own ket
+// primitive pairs once, in the outer loop.
+struct BraRec64 {
+  double p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad;
+};
+struct BraRec32 {
+  float p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad0, pad1, pad2;
+};
+#define JK_MAXBRA 64  // primitive pairs of a bra staged in LDS, one per lane (BIG kernels read HBM)
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void quartet64_jk(const BraRec64* __restrict__ bra, int nb,
+                                             const PrimPair64* __restrict__ kp, int nk, double Cx,
+                                             double Cy, double Cz, double ABx, double ABy, double ABz,
+                                             double CDx, double CDy, double CDz,
+                                             const double* __restrict__ tab,
+                                             double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
+  using C = EriClass<LA, LB, LC, LD>;
+  double acc[C::NACC];
+#pragma unroll
+  for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
+  for (int ik = 0; ik < nk; ++ik) {
+    const PrimPair64 k = kp[ik];
+    const double Kiq = k.K * k.ip;
+    const double QCx = k.Px - Cx, QCy = k.Py - Cy, QCz = k.Pz - Cz;
+#pragma unroll 2
+    for (int ib = 0; ib < nb; ++ib) {
+      const BraRec64 b = bra[ib];
+      const double rs = gx_rsqrt(b.p + k.p);
+      const double is = rs * rs;
+      const double rho = b.p * k.p * is;
+      const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = b.Kip * Kiq * rs;
+      double F[C::L + 1];
+      boys64<C::L>(T, pref, tab, F);
+      const double rho_p = rho * b.ip, rho_q = rho * k.ip;
+      C::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
+                               QCx, QCy, QCz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip,
+                               0.5 * k.ip, 0.5 * is, rho_p, rho_q);
+    }
+  }
+  C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void quartet32_jk(const BraRec32* __restrict__ bra, int nb,
+                                             const PrimPair32* __restrict__ kp, int nk, float Cx,
+                                             float Cy, float Cz, float ABx, float ABy, float ABz,
+                                             float CDx, float CDy, float CDz,
+                                             const float* __restrict__ tab,
+                                             float (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
+  using C = EriClass<LA, LB, LC, LD>;
+  float acc[C::NACC];
+#pragma unroll
+  for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0f;
+  for (int ik = 0; ik < nk; ++ik) {
+    const PrimPair32 k = kp[ik];
+    const float Kiq = k.K * k.ip;
+    const float QCx = k.Px - Cx, QCy = k.Py - Cy, QCz = k.Pz - Cz;
+#pragma unroll 2
+    for (int ib = 0; ib < nb; ++ib) {
+      const BraRec32 b = bra[ib];
+      const float rs = gx_rsqrt(b.p + k.p);
+      const float is = rs * rs;
+      const float rho = b.p * k.p * is;
+      const float PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+      const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const float pref = b.Kip * Kiq * rs;
+      float F[C::L + 1];
+      boys32<C::L>(T, pref, tab, F);
+      const float rho_p = rho * b.ip, rho_q = rho * k.ip;
+      C::template prim<float>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
+                              QCx, QCy, QCz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5f * b.ip,
+                              0.5f * k.ip, 0.5f * is, rho_p, rho_q);
+    }
+  }
+  C::template hrr<float>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+// ---------------------------------------------------------------------------
 // Schwarz bounds: Q_AB = sqrt(max_ab |(ab|ab)| s_a^2 s_b^2), one thread per pair.
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void k_schwarz(ShellView sh, PairView pr, int p0, int np,
@@ -238,17 +321,18 @@ __global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int 
 // Fused direct J/K.
 //
 // Persistent workgroups stride over the bra shell pairs of one class (in
-// decreasing-work order).  For each bra the ket class is split into
-// segments of equal primitive-pair count, each sorted by decreasing Schwarz
-// bound, so the kets that survive the cutoff -- and, in mixed mode, the ones
-// that stay in FP64 -- are prefixes found by binary search.  The surviving
-// kets form a compact virtual index space that the waves sweep 64 at a time
-// (one ket pair per lane); a chunk only mixes precisions or primitive counts
-// at range boundaries.  Integrals are digested at once:
+// decreasing-work order); all waves of a workgroup share one bra at a time.
+// For each bra the ket class is split into segments of equal primitive-pair
+// count, each sorted by decreasing Schwarz bound, so the kets surviving the
+// cutoff -- and, in mixed mode, those that stay in FP64 -- are prefixes
+// found by binary search.  The surviving kets form a compact index space the
+// waves sweep 64 at a time (one ket pair per lane).  The bra's primitive
+// pairs and the density rows of its shells sit in LDS.  Digestion:
 //   Jt_ab += 2 f v P_cd      (register sums, wave-reduced, one atomic per element)
 //   Jt_cd += 2 f v P_ab      (global FP64 atomics)
 //   Kt_ac += f v P_bd, Kt_ad += f v P_bc, Kt_bc += f v P_ad, Kt_bd += f v P_ac
-//                            (LDS rows of A and B, flushed once per bra)
+//                            (LDS rows of A and B, flushed once per bra: the
+//                            whole wave would otherwise hit the same two rows)
 // with f the 8-fold degeneracy weight; J = Jt + Jt^T, K = Kt + Kt^T.
 #define JK_MAXSEG 16
 #define JK_THREADS 256
@@ -283,7 +367,7 @@ __device__ __forceinline__ int count_at_least(const double* __restrict__ Q, int 
   return a - lo;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool BIG>
 __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
@@ -291,9 +375,12 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
   __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1];
   __shared__ double jab_s[NA * NB];
   const int N = a.N;
+  // dynamic LDS: Boys FP64 | Boys FP32 (mixed) | bra FP64 | bra FP32 (mixed) | P rows | K rows
   double* tab64 = smem;
   float* tab32 = (float*)(tab64 + BOYS_NT * BOYS_ROW64);
-  double* prow = (double*)(tab32 + (a.mixed ? BOYS_NT * BOYS_ROW32 : 0));
+  BraRec64* bra64 = (BraRec64*)(tab32 + (a.mixed ? BOYS_NT * BOYS_ROW32 : 0));
+  BraRec32* bra32 = (BraRec32*)(bra64 + (BIG ? 0 : JK_MAXBRA));
+  double* prow = (double*)(bra32 + ((a.mixed && !BIG) ? JK_MAXBRA : 0));
   double* krow = prow + (NA + NB) * N;
   const int nr = 2 * a.nseg;  // range r: segment r/2, FP64 (even) or FP32 (odd)
 
@@ -311,6 +398,9 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
     const int ao = a.sh.bf[A], bo = a.sh.bf[B];
     const int rowB = (A == B) ? 0 : NA;
     const double QA = a.pr.Q[ip];
+    const int nbp = a.pr.np[ip];
+    const double Ax = a.sh.ctr[3 * A], Ay = a.sh.ctr[3 * A + 1], Az = a.sh.ctr[3 * A + 2];
+    const double ABx = Ax - a.sh.ctr[3 * B], ABy = Ay - a.sh.ctr[3 * B + 1], ABz = Az - a.sh.ctr[3 * B + 2];
     __syncthreads();  // the previous bra's flush is complete
     for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
       const int r = i / N, c = i - r * N;
@@ -319,6 +409,36 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
       krow[i] = 0.0;
     }
     if (threadIdx.x < NA * NB) jab_s[threadIdx.x] = 0.0;
+    if (!BIG && threadIdx.x < nbp) {
+      const int t = threadIdx.x;
+      const PrimPair64 q = a.pr.pp64[a.pr.poff[ip] + t];
+      BraRec64 r;
+      r.p = q.p;
+      r.ip = q.ip;
+      r.Px = q.Px;
+      r.Py = q.Py;
+      r.Pz = q.Pz;
+      r.Kip = q.K * q.ip;
+      r.PAx = q.Px - Ax;
+      r.PAy = q.Py - Ay;
+      r.PAz = q.Pz - Az;
+      r.pad = 0.0;
+      bra64[t] = r;
+      if (a.mixed) {
+        BraRec32 r32;
+        r32.p = (float)r.p;
+        r32.ip = (float)r.ip;
+        r32.Px = (float)r.Px;
+        r32.Py = (float)r.Py;
+        r32.Pz = (float)r.Pz;
+        r32.Kip = (float)r.Kip;
+        r32.PAx = (float)r.PAx;
+        r32.PAy = (float)r.PAy;
+        r32.PAz = (float)r.PAz;
+        r32.pad0 = r32.pad1 = r32.pad2 = 0.0f;
+        bra32[t] = r32;
+      }
+    }
     if (threadIdx.x < a.nseg) {
       const int sg = threadIdx.x;
       const int lo = a.seg_lo[sg];
@@ -329,7 +449,7 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
       const int n64 = a.mixed ? min(nsurv, count_at_least(a.pr.Q, lo, hi, QA, a.fp32_bound)) : nsurv;
       r_start[2 * sg] = lo;
       r_start[2 * sg + 1] = lo + n64;
-      r_pre[2 * sg + 1] = n64;            // lengths for now
+      r_pre[2 * sg + 1] = n64;  // lengths for now
       r_pre[2 * sg + 2] = nsurv - n64;
     }
     __syncthreads();
@@ -358,15 +478,37 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
       const bool lo = act && (r & 1);
       const bool hi = act && !(r & 1);
       double val[C::NOUT];
-      if (hi) quartet64<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab64, val);
+      int Cs = 0, Ds = 0;
+      double Cx = 0.0, Cy = 0.0, Cz = 0.0, CDx = 0.0, CDy = 0.0, CDz = 0.0;
+      if (act) {
+        Cs = a.pr.A[jp];
+        Ds = a.pr.B[jp];
+        Cx = a.sh.ctr[3 * Cs];
+        Cy = a.sh.ctr[3 * Cs + 1];
+        Cz = a.sh.ctr[3 * Cs + 2];
+        CDx = Cx - a.sh.ctr[3 * Ds];
+        CDy = Cy - a.sh.ctr[3 * Ds + 1];
+        CDz = Cz - a.sh.ctr[3 * Ds + 2];
+      }
+      if (hi) {
+        if (!BIG)
+          quartet64_jk<LA, LB, LC, LD>(bra64, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], Cx, Cy, Cz,
+                                       ABx, ABy, ABz, CDx, CDy, CDz, tab64, val);
+        else
+          quartet64<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab64, val);
+      }
       if (lo) {
         float vf[C::NOUT];
-        quartet32<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab32, vf);
+        if (!BIG)
+          quartet32_jk<LA, LB, LC, LD>(bra32, nbp, a.pr.pp32 + a.pr.poff[jp], a.pr.np[jp], (float)Cx,
+                                       (float)Cy, (float)Cz, (float)ABx, (float)ABy, (float)ABz,
+                                       (float)CDx, (float)CDy, (float)CDz, tab32, vf);
+        else
+          quartet32<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab32, vf);
 #pragma unroll
         for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
       }
       if (act) {
-        const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
         const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
         double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
         if (jp == ip) f *= 0.5;
@@ -426,11 +568,9 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib) {
 #pragma unroll
-          for (int ic = 0; ic < NC; ++ic)
-            atomicAdd(&krow[(rowB + ib) * N + co + ic], kbc[ib * NC + ic]);
+          for (int ic = 0; ic < NC; ++ic) atomicAdd(&krow[(rowB + ib) * N + co + ic], kbc[ib * NC + ic]);
 #pragma unroll
-          for (int id = 0; id < ND; ++id)
-            atomicAdd(&krow[(rowB + ib) * N + dof + id], kbd[ib * ND + id]);
+          for (int id = 0; id < ND; ++id) atomicAdd(&krow[(rowB + ib) * N + dof + id], kbd[ib * ND + id]);
         }
       }
     }

@@ -11,8 +11,8 @@
   device, canonical orthogonalisation, core-H guess, DIIS SCF.
 
 Options the reference does not have (``schwarz_cutoff``, ``fp32_bound``,
-``precision``, ``max_iter``) default to the reference's behaviour: no
-screening, all FP64, no iteration cap.
+``precision``, ``max_iter``, ``threshold``) default to the reference's
+behaviour: no screening, all FP64, no iteration cap, |dE| < 1e-12.
 """
 import time
 
@@ -68,7 +68,7 @@ class RestrictedHF:
 
     def __init__(self, nuclei_array, basis_set_array, electrons, symmetry=None, processes=1,
                  schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64', max_iter=None, engine=None,
-                 verbose=False):
+                 verbose=False, threshold=1e-12):
         self.nuclei_array = nuclei_array
         self.basis_set_array = basis_set_array
         self.electrons = electrons
@@ -88,7 +88,7 @@ class RestrictedHF:
         self.fock_builder = FockMatrixRestricted(self.core_hamiltonian, self.engine, schwarz_cutoff,
                                                  fp32_bound, precision)
         self.scf_method = RestrictedSCF(self.linear_algebra, electrons, S, self.fock_builder,
-                                        max_iter=max_iter)
+                                        threshold=threshold, max_iter=max_iter)
         self.setup_time = time.perf_counter() - t0
         self._repulsion = None
 

@@ -63,6 +63,7 @@ class DIIS:
         self.linear_algebra = linear_algebra
         self.fock_array = []
         self.error_array = []
+        self.dots = np.zeros((0, 0))  # tr(e_i e_j^T) of the stored error matrices
         self.diis_subspace = diis_subspace
         self.begin = False
 
@@ -74,11 +75,21 @@ class DIIS:
         if bool(torch.all(error < 1e-6)):
             self.begin = False
         if self.begin:
+            # new row of tr(e_i e_j^T): one elementwise product and reduction on the device
+            row = (torch.stack(self.error_array + [error]) * error).sum(dim=(1, 2)).cpu().numpy() \
+                if self.error_array else (error * error).sum().reshape(1).cpu().numpy()
+            n = len(self.error_array)
+            dots = np.zeros((n + 1, n + 1))
+            dots[:n, :n] = self.dots
+            dots[n, :] = row
+            dots[:, n] = row
+            self.dots = dots
             self.error_array.append(error)
             self.fock_array.append(fock)
             if len(self.fock_array) > self.diis_subspace:
                 self.fock_array = []
                 self.error_array = []
+                self.dots = np.zeros((0, 0))
                 return fock
             if len(self.fock_array) > 1:
                 coef = self.create_b_matrix()
@@ -88,11 +99,10 @@ class DIIS:
         return fock
 
     def create_b_matrix(self):
-        # B_ij = tr(e_i e_j^T) on the device; the (n+1)^2 <= 81-element solve on the host.
+        # B_ij = tr(e_i e_j^T) (kept incrementally); the (n+1)^2 <= 81-element solve on the host.
         n = len(self.fock_array)
-        e = torch.stack(self.error_array).reshape(n, -1)
         b = np.ones((n + 1, n + 1))
-        b[:n, :n] = (e @ e.T).cpu().numpy()
+        b[:n, :n] = self.dots
         b[n, n] = 0.0
         rhs = np.zeros(n + 1)
         rhs[n] = 1.0
@@ -103,6 +113,7 @@ class DIIS:
         except np.linalg.LinAlgError:
             self.fock_array.pop(0)
             self.error_array.pop(0)
+            self.dots = self.dots[1:, 1:]
             return self.create_b_matrix()
 
 

@@ -1,0 +1,330 @@
+"""Pin the CPU oracle: reference golden values, closed forms and invariants (no GPU).
+
+Golden values are the reference's own test fixtures (tests/testsfast/... of
+ChiCheng45/Gaussium, cited per test).  Closed forms use mpmath: the Boys
+function via the lower incomplete gamma function; (ss|ss) in closed form; p
+and d integrals from derivatives of the (ss|ss) closed form with respect to
+the centre, (p_x s|ss) = 1/(2a) d/dA_x (ss|ss) and
+(d_xx s|ss) = [d^2/dA_x^2 (ss|ss) + 2a (ss|ss)] / (4a^2), d_xy likewise
+without the 2a term -- a formula independent of the recursion it checks.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.helpers import Obj, canonical_sample, golden, heh_sto3g_mock, load_system
+
+
+# --------------------------------------------------------------------------- Boys
+BOYS_REF = [  # tests/testsfast/integral/test_boys_function.py:8-66
+    (0, 0, 1), (0, 4.802164876034457e-31, 1), (0, 4.4757221852728435, 0.417742),
+    (0, 12.0769452374367, 0.255015), (0, 130.80051761256559, 0.077489),
+    (1, 0, 0.333333), (1, 4.802164876034457e-31, 0.333333), (1, 4.4757221852728435, 0.045396),
+    (1, 12.0769452374367, 0.0105576), (1, 130.80051761256559, 0.000296210),
+    (2, 0, 0.200000), (2, 4.802164876034457e-31, 0.200000), (2, 4.4757221852728435, 0.0139425),
+    (2, 12.0769452374367, 0.00131106), (2, 130.80051761256559, 3.39689e-6),
+]
+
+
+@pytest.mark.parametrize('v,x,want', BOYS_REF)
+def test_boys_reference_values(v, x, want):
+    from oracle.boys import boys_function
+    from oracle import eri_c
+    np.testing.assert_approx_equal(boys_function(v, x), want, 6)
+    np.testing.assert_approx_equal(eri_c.boys(v, x), want, 6)
+
+
+def test_boys_closed_forms_and_mpmath():
+    from oracle.boys import boys_exact, boys_f0_closed_form, boys_function, boys_mpmath, exact
+    for m in range(10):
+        assert boys_function(m, 0.0) == pytest.approx(1.0 / (2 * m + 1), rel=1e-12)
+    for x in (1e-6, 0.3, 2.0, 11.0, 24.9, 25.1, 40.0, 90.0):
+        assert boys_function(0, x) == pytest.approx(boys_f0_closed_form(x), abs=1e-10)
+        for m in range(9):
+            ref = float(boys_mpmath(m, x))
+            # the reference's series truncates terms below 1e-10
+            assert abs(boys_function(m, x) - ref) < 2e-10
+            assert boys_exact(m, x) == pytest.approx(ref, rel=1e-13)
+            with exact():
+                assert boys_function(m, x) == pytest.approx(ref, rel=1e-13)
+    # downward recursion identity F_{m-1} = (2x F_m + e^-x) / (2m - 1)
+    from oracle.boys import boys_function_recursion
+    for x in (0.5, 7.0, 30.0):
+        for m in range(1, 8):
+            assert boys_function_recursion(m, x, float(boys_mpmath(m, x))) == pytest.approx(
+                float(boys_mpmath(m - 1, x)), rel=1e-13)
+
+
+def test_c_oracle_exact_boys_matches_mpmath():
+    from oracle import eri_c
+    from oracle.boys import boys_mpmath
+    with eri_c.exact_boys():
+        for m in range(12):
+            for x in (0.0, 1e-9, 0.05, 3.3, 25.0, 47.99, 59.9, 60.0, 250.0, 4.0e4):
+                assert eri_c.boys(m, x) == pytest.approx(float(boys_mpmath(m, x)), rel=1e-14)
+
+
+# --------------------------------------------------------------------------- normalisation
+@pytest.mark.parametrize('c,a,want', [  # tests/testsfast/factory/test_primitive_basis_function.py:8-30
+    (0.15432897, 3.42525091, 1.794441832218435), (0.53532814, 0.62391373, 0.5003264923314032),
+    (0.44463454, 0.16885540, 0.18773545851092535), (0.15432897, 9.75393461, 3.9336432656254527),
+    (0.53532814, 1.77669115, 1.0967787981767012), (0.44463454, 0.48084429, 0.41154131374122654)])
+def test_primitive_normalisation(c, a, want):
+    from oracle.basis import PrimitiveBasis
+    from gaussium_amd.basis import PrimitiveBasis as P2
+    np.testing.assert_approx_equal(PrimitiveBasis(c, a, (0, 0, 0.7316), (0, 0, 0)).normalisation, want, 7)
+    np.testing.assert_approx_equal(P2(c, a, (0, 0, 0.7316), (0, 0, 0)).normalisation, want, 7)
+
+
+def test_contracted_functions_are_normalised():
+    """<phi|phi> = 1 for every contracted function with >= 2 primitives (s, p, d)."""
+    from oracle.basis import expand_basis_set
+    from oracle.one_electron import overlap_matrix
+    shells = [['S', [0.3, 4.0], [0.6, 0.9], [0.4, 0.2]], ['L', [0.2, 1.5, 0.5], [0.8, 0.3, 0.7]],
+              ['D', [0.6, 1.1], [0.5, 0.35]]]
+    basis = expand_basis_set(shells, (0.1, -0.2, 0.3))
+    s = overlap_matrix(basis)
+    assert np.allclose(np.diag(s), 1.0, atol=1e-12)
+
+
+# --------------------------------------------------------------------------- ERIs
+HEH_ERI = [((0, 0, 0, 0), 1.3072), ((1, 0, 0, 0), 0.4373), ((1, 0, 1, 0), 0.1773),
+           ((1, 1, 0, 0), 0.6057), ((1, 1, 1, 0), 0.3118), ((1, 1, 1, 1), 0.7746)]
+
+
+def test_heh_eri_reference_values():
+    """tests/testsfast/matrixelements/test_electron_repulsion_matrix.py:124-146 (OS)."""
+    from oracle import eri_c
+    from oracle.os_eri import eri_contracted
+    basis = heh_sto3g_mock()
+    keys = [k for k, _ in HEH_ERI]
+    c_vals = eri_c.eri_keys(basis, keys)
+    for (k, want), cv in zip(HEH_ERI, c_vals):
+        pv = eri_contracted(*(basis[i] for i in k))
+        np.testing.assert_approx_equal(pv, want, 4)
+        np.testing.assert_approx_equal(cv, want, 4)
+        assert abs(pv - cv) < 1e-14
+
+
+def _prim_basis(l, a, ctr):
+    g = Obj(contraction=1.0, exponent=a, coordinates=tuple(ctr), integral_exponents=l, normalisation=1.0)
+    return Obj(primitive_gaussian_array=[g], coordinates=tuple(ctr), integral_exponents=l, normalisation=1.0)
+
+
+def _ssss_mp(a, A, b, B, c, C, d, D):
+    import mpmath
+    p, q = a + b, c + d
+    P = [(a * A[i] + b * B[i]) / p for i in range(3)]
+    Q = [(c * C[i] + d * D[i]) / q for i in range(3)]
+    rho = p * q / (p + q)
+    rab = sum((A[i] - B[i]) ** 2 for i in range(3))
+    rcd = sum((C[i] - D[i]) ** 2 for i in range(3))
+    T = rho * sum((P[i] - Q[i]) ** 2 for i in range(3))
+    F0 = mpmath.mpf(1) if T == 0 else mpmath.sqrt(mpmath.pi / (4 * T)) * mpmath.erf(mpmath.sqrt(T))
+    return (2 * mpmath.pi ** 2.5 / (p * q * mpmath.sqrt(p + q)) * mpmath.exp(-a * b / p * rab - c * d / q * rcd)
+            * F0)
+
+
+def test_ssss_closed_form():
+    import mpmath
+    from oracle import eri_c
+    rng = np.random.default_rng(1)
+    with mpmath.workdps(30):
+        for _ in range(10):
+            ex = rng.uniform(0.1, 8.0, 4)
+            ctr = rng.uniform(-1.5, 1.5, (4, 3))
+            basis = [_prim_basis((0, 0, 0), ex[i], ctr[i]) for i in range(4)]
+            want = float(_ssss_mp(*[mpmath.mpf(float(ex[i])) if k == 0 else [mpmath.mpf(float(v)) for v in ctr[i]]
+                                    for i in range(4) for k in range(2)]))
+            with eri_c.exact_boys():
+                got = eri_c.eri_keys(basis, [(0, 1, 2, 3)])[0]
+            assert got == pytest.approx(want, rel=1e-12, abs=1e-15)
+
+
+def _deriv_eri(lA, ex, ctr):
+    """(lA s|ss) with lA in {x, xx, xy} (as Cartesian tuples) by differentiating the closed form."""
+    import mpmath
+    a = mpmath.mpf(float(ex[0]))
+    rest = []
+    for i in range(1, 4):
+        rest += [mpmath.mpf(float(ex[i])), [mpmath.mpf(float(v)) for v in ctr[i]]]
+    A0 = [mpmath.mpf(float(v)) for v in ctr[0]]
+
+    def f(x, y, z):
+        return _ssss_mp(a, [x, y, z], *rest)
+
+    order = list(lA)
+    d = mpmath.diff(f, A0, tuple(order))
+    if lA == (1, 0, 0):
+        return d / (2 * a)
+    if lA == (2, 0, 0):
+        return (d + 2 * a * f(*A0)) / (4 * a * a)
+    if lA == (1, 1, 0):
+        return d / (4 * a * a)
+    raise ValueError(lA)
+
+
+@pytest.mark.parametrize('lA', [(1, 0, 0), (2, 0, 0), (1, 1, 0)])
+def test_p_and_d_by_derivative_identity(lA):
+    import mpmath
+    from oracle import eri_c
+    from oracle.os_eri import eri_primitive
+    rng = np.random.default_rng(sum(lA) + 5 * lA[1])
+    with mpmath.workdps(40):
+        for _ in range(4):
+            ex = rng.uniform(0.2, 5.0, 4)
+            ctr = rng.uniform(-1.2, 1.2, (4, 3))
+            want = float(_deriv_eri(lA, ex, ctr))
+            basis = [_prim_basis(lA, ex[0], ctr[0])] + [_prim_basis((0, 0, 0), ex[i], ctr[i]) for i in range(1, 4)]
+            with eri_c.exact_boys():
+                got = eri_c.eri_keys(basis, [(0, 1, 2, 3)])[0]
+            assert got == pytest.approx(want, rel=1e-10, abs=1e-13)
+            # the Python restatement agrees with the C one (reference Boys series in both)
+            py = eri_primitive(ex[0], tuple(ctr[0]), lA, ex[1], tuple(ctr[1]), (0, 0, 0),
+                               ex[2], tuple(ctr[2]), (0, 0, 0), ex[3], tuple(ctr[3]), (0, 0, 0))
+            cv = eri_c.eri_keys(basis, [(0, 1, 2, 3)])[0]
+            assert py == pytest.approx(cv, rel=1e-12, abs=1e-14)
+
+
+def test_python_and_c_oracles_agree_on_p_and_d():
+    """Two independent recursion orders of the reference scheme (slot order restored vs not)."""
+    from oracle import eri_c
+    from oracle.basis import expand_basis_set
+    from oracle.os_eri import eri_contracted
+    shells_a = [['L', [0.3, 2.0, 0.4], [0.7, 0.4, 0.6]], ['D', [1.0, 0.9]]]
+    shells_b = [['S', [0.5, 1.3], [0.5, 0.25]], ['P', [0.6, 0.8], [0.4, 0.2]]]
+    basis = expand_basis_set(shells_a, (0.0, 0.1, -0.3)) + expand_basis_set(shells_b, (1.1, -0.4, 0.6))
+    keys = canonical_sample(len(basis), 150, np.random.default_rng(4))
+    cv = eri_c.eri_keys(basis, keys)
+    pv = np.array([eri_contracted(*(basis[i] for i in k)) for k in keys])
+    assert np.abs(cv - pv).max() < 1e-12
+
+
+def test_dense_symmetry_and_schwarz_inequality():
+    from oracle import eri_c
+    nuc, basis, _ = load_system('HeH+.mol', '6-311+GPP.gbs')
+    d = eri_c.eri_dense(basis)
+    n = len(basis)
+    assert np.array_equal(d, d.transpose(1, 0, 2, 3)) and np.array_equal(d, d.transpose(2, 3, 0, 1))
+    diag = np.sqrt(np.abs(np.einsum('ijij->ij', d))).reshape(-1)
+    assert np.all(np.abs(d.reshape(n * n, n * n)) <= np.outer(diag, diag) * (1 + 1e-12) + 1e-15)
+
+
+# --------------------------------------------------------------------------- one-electron
+def test_one_electron_reference_values():
+    """tests/testsfast/matrixelements/test_{overlap_integral,kinetic_energy,nuclear_attraction}_matrix.py."""
+    from oracle.basis import Nuclei
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    basis = heh_sto3g_mock()
+    nuc = [Nuclei('HELIUM', 2, 4, (0, 0, 0.7316)), Nuclei('HYDROGEN', 1, 1, (0, 0, -0.7316))]
+    S, T, V = overlap_matrix(basis), kinetic_matrix(basis), nuclear_matrix(basis, nuc)
+    np.testing.assert_approx_equal(S[0, 1], 0.4508, 3)
+    np.testing.assert_approx_equal(T[0, 0], 2.1643, 5)
+    np.testing.assert_approx_equal(T[0, 1], 0.167, 3)
+    np.testing.assert_approx_equal(T[1, 1], 0.7600, 4)
+    np.testing.assert_approx_equal(V[0, 0], -4.817, 5)
+    np.testing.assert_approx_equal(V[0, 1], -1.5142, 3)
+    np.testing.assert_approx_equal(V[1, 1], -2.4912, 4)
+    # H2 cases of the same files (0.7 bohr): S = 0.6593, T_01 = 0.2365, V_00 = -1.8804, V_01 = -1.1948
+    h2 = heh_sto3g_mock(he=(0, 0, 0.7), h=(0, 0, -0.7))
+    h_rows = h2[1].primitive_gaussian_array
+    b0 = Obj(primitive_gaussian_array=[Obj(**{**g.__dict__, 'coordinates': (0, 0, 0.7)}) for g in h_rows],
+             coordinates=(0, 0, 0.7), integral_exponents=(0, 0, 0), normalisation=1)
+    b1 = Obj(primitive_gaussian_array=[Obj(**{**g.__dict__, 'coordinates': (0, 0, -0.7)}) for g in h_rows],
+             coordinates=(0, 0, -0.7), integral_exponents=(0, 0, 0), normalisation=1)
+    nh = [Nuclei('HYDROGEN', 1, 1, (0, 0, 0.7)), Nuclei('HYDROGEN', 1, 1, (0, 0, -0.7))]
+    S, T, V = overlap_matrix([b0, b1]), kinetic_matrix([b0, b1]), nuclear_matrix([b0, b1], nh)
+    np.testing.assert_approx_equal(S[0, 1], 0.6593, 4)
+    np.testing.assert_approx_equal(T[0, 1], 0.2365, 4)
+    np.testing.assert_approx_equal(V[0, 0], -1.8804, 4)
+    np.testing.assert_approx_equal(V[0, 1], -1.1948, 4)
+
+
+# --------------------------------------------------------------------------- SCF pieces
+def test_nuclear_repulsion_reference_values():
+    """tests/testsfast/common/test_nuclear_repulsion.py:9-21."""
+    from oracle.basis import Nuclei, nuclear_repulsion
+    np.testing.assert_approx_equal(
+        nuclear_repulsion([Nuclei('H', 1, 1, (0, 0, 0.7)), Nuclei('H', 1, 1, (0, 0, -0.7))]), 0.7143, 4)
+    np.testing.assert_approx_equal(
+        nuclear_repulsion([Nuclei('H', 1, 1, (0, 0, 0.7316)), Nuclei('He', 2, 4, (0, 0, -0.7316))]), 1.366867, 7)
+
+
+def test_density_energy_linear_algebra_fixtures():
+    """test_density_matrix.py:15-25, test_total_energy.py:25-28, test_linear_algebra.py:11-29."""
+    from oracle.scf import LinearAlgebra, density_matrix_restricted, total_energy_restricted
+    c = np.array([[0.9291, -0.6259], [0.1398, 1.1115]])
+    p = density_matrix_restricted(c, 2)
+    np.testing.assert_approx_equal(p[0, 0], 1.7266, 4)
+    np.testing.assert_approx_equal(p[1, 0], 0.2599, 3)
+    np.testing.assert_approx_equal(p[1, 1], 0.0391, 3)
+    P = np.array([[1.7266, 0.2599], [0.2599, 0.0391]])
+    H = np.array([[-2.6527, -1.3472], [-1.3472, -1.7318]])
+    F = np.array([[-1.3904, -0.9732], [-0.9732, -0.7429]])
+    np.testing.assert_approx_equal(total_energy_restricted(H, P, F), -4.14186, 6)
+    la = LinearAlgebra(np.array([[1.0, 0.4508], [0.4508, 1.0]]))
+    # eigenvector signs are free; compare up to a sign per column
+    want = np.array([[-0.9541, 0.5871], [0.9541, 0.5871]])
+    for k in range(2):
+        col = la.x[:, k] * np.sign(la.x[0, k] * want[0, k])
+        np.testing.assert_array_almost_equal(col, want[:, k], decimal=4)
+    la.x = want
+    e, cc = la.diagonalize(H)
+    np.testing.assert_array_almost_equal(e, [-2.6742, -1.3043], decimal=4)
+    for k, w in enumerate(([0.9291, 0.1398], [0.6259, -1.1115])):
+        col = cc[:, k] * np.sign(cc[0, k] * w[0])
+        np.testing.assert_array_almost_equal(col, w, decimal=3)
+
+
+@pytest.mark.parametrize('args,want', [  # tests/testsfast/common/test_symmetry.py:14-48
+    ((6, 2, 1, 0), (0, 1, 2, 6)), ((0, 0, 1, 0), (0, 0, 0, 1)), ((1, 0, 0, 0), (0, 0, 0, 1)),
+    ((1, 0, 1, 0), (0, 1, 0, 1)), ((2, 2, 1, 0), (0, 1, 2, 2)), ((2, 1, 1, 0), (0, 1, 1, 2)),
+    ((1, 1, 1, 0), (0, 1, 1, 1)), ((2, 1, 2, 0), (0, 2, 1, 2)), ((1, 1, 0, 2), (0, 2, 1, 1))])
+def test_sort_index(args, want):
+    from oracle.basis import sort_index
+    assert sort_index(*args) == want
+
+
+def test_jk_loop_matches_einsum():
+    """The reference's element loop (fock_matrix.py:12-22) equals the einsum J - K/2."""
+    from oracle import eri_c
+    from oracle.scf import fock_restricted_loop, jk_from_dense
+    nuc, basis, _ = load_system('HeH+.mol', '6-311+GPP.gbs')
+    eri = eri_c.eri_dense(basis)
+    rng = np.random.default_rng(0)
+    c = rng.standard_normal((len(basis), 3))
+    P = c @ c.T
+    H = np.zeros_like(P)
+    J, K = jk_from_dense(eri, P)
+    assert np.abs(fock_restricted_loop(H, eri, P) - (J - 0.5 * K)).max() < 1e-12
+
+
+def test_mock_heh_electronic_energy():
+    """tests/testsfast/hartreefock/test_hatree_fock.py:59-61: -4.227529."""
+    from oracle import eri_c
+    from oracle.basis import Nuclei
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from oracle.scf import jk_from_dense, rhf
+    basis = heh_sto3g_mock(he=(0.0, 0.0, 0.0), h=(0.0, 0.0, 1.4632), first='H')
+    nuc = [Nuclei('HYDROGEN', 1, 1, (0.0, 0.0, 1.4632)), Nuclei('HELIUM', 2, 4, (0.0, 0.0, 0.0))]
+    eri = eri_c.eri_dense(basis)
+    H = kinetic_matrix(basis) + nuclear_matrix(basis, nuc)
+    e, *_ = rhf(H, overlap_matrix(basis), 2, lambda P: H + jk_from_dense(eri, P)[0] - 0.5 * jk_from_dense(eri, P)[1])
+    np.testing.assert_approx_equal(e, -4.227529, 6)
+
+
+@pytest.mark.parametrize('system,published', [  # tests/testslow/start/test_start.py:9-25
+    (('HeH+.mol', 'STO-3G.gbs'), -2.84183608212),
+    (('HeH+.mol', '6-311+GPP.gbs'), -2.92922773384),
+    (('C2H4.mol', '3-21G.gbs'), -77.600460844),
+])
+def test_oracle_rhf_published_energies(system, published):
+    from tests.helpers import oracle_rhf
+    nuc, basis, ne = load_system(*system)
+    e, _ = oracle_rhf(nuc, basis, ne, exact_boys=False, diis_cond_limit=None)
+    np.testing.assert_approx_equal(e, published, 6)
+    e2, _ = oracle_rhf(nuc, basis, ne)
+    np.testing.assert_approx_equal(e2, published, 6)
+    # exact vs reference Boys series and the DIIS conditioning rule move E by < 1e-9
+    assert abs(e - e2) < 1e-9
