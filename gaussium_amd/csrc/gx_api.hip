@@ -48,6 +48,7 @@ inline int class_la(int c) {
 }
 inline int class_lb(int c) { return c - class_la(c) * (class_la(c) + 1) / 2; }
 inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+inline int class_L(int X, int Y) { return class_la(X) + class_lb(X) + class_la(Y) + class_lb(Y); }
 
 template <class T>
 int dalloc(T** p, size_t n) {
@@ -128,8 +129,7 @@ struct gx_ctx {
   double* d_Q = nullptr;
   gx::PrimPair64* d_pp64 = nullptr;
   gx::PrimPair32* d_pp32 = nullptr;
-  std::vector<int*> d_order;  // per class combo: bra positions, small-Kp first
-  std::vector<int> n_small;    // per class combo: bras with <= JK_MAXBRA primitive pairs
+  std::vector<int*> d_order;  // per class combo: bra positions, largest work first
   // ket segments: per class, runs of pairs with the same primitive-count key
   std::vector<int> seg_lo, seg_hi, cls_seg;  // cls_seg: NCLS + 1 offsets into seg_*
   int *d_seg_lo = nullptr, *d_seg_hi = nullptr;
@@ -230,20 +230,22 @@ hipError_t launch_quartets(int la, int lb, int lc, int ld, dim3 g, hipStream_t s
   return hipErrorInvalidValue;
 }
 
-template <bool BIG>
 hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hipStream_t s,
                      const gx::JKArgs& a) {
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
-    const void* fn = (const void*)gx::k_jk<A, B, C, D, BIG>;                                 \
+    const void* fn = (const void*)gx::k_jk<A, B, C, D>;                                      \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
+    constexpr int TPB = gx::JKShape<A + B + C + D>::THREADS;                                  \
+    constexpr int TEAMS = gx::JKShape<A + B + C + D>::TEAMS;                                  \
     int per_cu = 0;                                                                           \
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, JK_THREADS, lds);           \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TPB, lds);                  \
     if (e != hipSuccess) return e;                                                            \
     if (per_cu < 1) per_cu = 1;                                                               \
-    const int grid = std::min(a.nbra, per_cu * num_cu);                                       \
-    hipLaunchKernelGGL((gx::k_jk<A, B, C, D, BIG>), dim3(grid), dim3(JK_THREADS), lds, s, a); \
+    const int grid = std::min((a.nbra + TEAMS - 1) / TEAMS, per_cu * num_cu);                 \
+    hipLaunchKernelGGL((gx::k_jk<A, B, C, D>), dim3(grid), dim3(TPB), lds, s, a, a.pr.pp64,   \
+                       a.pr.pp32);                                                            \
     return hipGetLastError();                                                                 \
   }
   GX_FOR_EACH_CLASS(GX_CASE)
@@ -251,13 +253,13 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
   return hipErrorInvalidValue;
 }
 
-size_t jk_lds_bytes(int la, int lb, int N, int mixed, bool big) {
+// Must match the layout at the top of gx::k_jk.
+size_t jk_lds_bytes(int la, int lb, int N, int mixed) {
   const int na = ncart(la), nb = ncart(lb);
+  const int teams = gx::JKShape<0>::TEAMS;  // the same for every L
   size_t b = (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
   if (mixed) b += (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float);
-  if (!big) b += (size_t)JK_MAXBRA * (sizeof(gx::BraRec64) + (mixed ? sizeof(gx::BraRec32) : 0));
-  b += 2 * (size_t)(na + nb) * N * sizeof(double);
-  return b;
+  return b + teams * 2 * (size_t)(na + nb) * N * sizeof(double);
 }
 
 // Build the primitive-pair records of the current pair list on the device.
@@ -363,7 +365,6 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
     return GX_OK;
   for (auto& p : c->d_order) dfree(&p);
   c->d_order.assign(combo_count(), nullptr);
-  c->n_small.assign(combo_count(), 0);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
@@ -378,15 +379,7 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
       });
       std::vector<int> order(nb);
       std::iota(order.begin(), order.end(), 0);
-      const int b0 = c->cls_start[X];
-      auto big = [&](int i) { return c->pnp[b0 + i] > JK_MAXBRA; };
-      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        if (big(a) != big(b)) return !big(a);
-        return cost[a] > cost[b];
-      });
-      int ns = 0;
-      while (ns < nb && !big(order[ns])) ++ns;
-      c->n_small[combo] = ns;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
       GX_TRY(upload(&c->d_order[combo], order, c->stream));
     }
   c->ord_cutoff = cutoff;
@@ -767,7 +760,7 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   const long nn = (long)N * N;
   const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
   for (int X = 0; X < NCLS; ++X) {
-    const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), N, mixed, false);
+    const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), N, mixed);
     if (c->cls_start[X + 1] > c->cls_start[X] && lds > (size_t)c->max_lds)
       return fail(GX_ERR_ARG, "gx_jk: basis too large for the LDS row cache (nbf=" + std::to_string(N) +
                                   ", needs " + std::to_string(lds) + " bytes)");
@@ -803,18 +796,8 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       a.cutoff = cutoff;
       a.fp32_bound = fp32_bound;
       a.mixed = mixed;
-      const int ns = c->n_small[combo];
-      if (ns > 0) {
-        a.nbra = ns;
-        GX_HIP(launch_jk<false>(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
-                                jk_lds_bytes(class_la(X), class_lb(X), N, mixed, false), c->stream, a));
-      }
-      if (ns < nb) {
-        a.order = c->d_order[combo] + ns;
-        a.nbra = nb - ns;
-        GX_HIP(launch_jk<true>(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
-                               jk_lds_bytes(class_la(X), class_lb(X), N, mixed, true), c->stream, a));
-      }
+      GX_HIP(launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
+                       jk_lds_bytes(class_la(X), class_lb(X), N, mixed), c->stream, a));
     }
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, c->d_Jt,
                      c->d_Kt, J, K);

@@ -31,15 +31,16 @@
 
 namespace gx {
 
+// One primitive pair (a on A, b on B) of an oriented shell pair (A, B):
+//   p = a + b, ip = 1/p, P the Gaussian product centre, PA = P - A,
+//   Kip = sqrt(2 pi^(5/2)) c_a c_b exp(-a b / p |AB|^2) / p
+// so that a primitive quartet's prefactor is Kip_bra Kip_ket / sqrt(p + q).
 struct __align__(16) PrimPair64 {
-  double p, ip;       // p = a + b, 1/p
-  double Px, Py, Pz;  // Gaussian product centre
-  double K;           // 2 pi^(5/2)-scaled c_a c_b exp(-a b / p |AB|^2)
-  double pad0, pad1;
+  double p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad;
 };
 
 struct __align__(16) PrimPair32 {
-  float p, ip, Px, Py, Pz, K, pad0, pad1;
+  float p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad0, pad1, pad2;
 };
 
 // Device-side views passed to kernels by value.

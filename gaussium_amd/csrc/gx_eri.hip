@@ -41,9 +41,11 @@ __global__ void k_primpairs(int npp, const int* __restrict__ pair_of, const int*
   r.Px = (a * Ax + b * Bx) * ip_;
   r.Py = (a * Ay + b * By) * ip_;
   r.Pz = (a * Az + b * Bz) * ip_;
-  r.K = K;
-  r.pad0 = 0.0;
-  r.pad1 = 0.0;
+  r.Kip = K * ip_;
+  r.PAx = r.Px - Ax;
+  r.PAy = r.Py - Ay;
+  r.PAz = r.Pz - Az;
+  r.pad = 0.0;
   out64[t] = r;
   PrimPair32 s;
   s.p = (float)p;
@@ -51,9 +53,11 @@ __global__ void k_primpairs(int npp, const int* __restrict__ pair_of, const int*
   s.Px = (float)r.Px;
   s.Py = (float)r.Py;
   s.Pz = (float)r.Pz;
-  s.K = (float)K;
-  s.pad0 = 0.0f;
-  s.pad1 = 0.0f;
+  s.Kip = (float)r.Kip;
+  s.PAx = (float)r.PAx;
+  s.PAy = (float)r.PAy;
+  s.PAz = (float)r.PAz;
+  s.pad0 = s.pad1 = s.pad2 = 0.0f;
   out32[t] = s;
 }
 
@@ -65,10 +69,10 @@ __device__ __forceinline__ void quartet64(const ShellView& sh, const PairView& p
                                           double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
   using C = EriClass<LA, LB, LC, LD>;
   const int A = pr.A[ip], B = pr.B[ip], Cs = pr.A[jp], Ds = pr.B[jp];
-  const double Ax = sh.ctr[3 * A], Ay = sh.ctr[3 * A + 1], Az = sh.ctr[3 * A + 2];
-  const double Cx = sh.ctr[3 * Cs], Cy = sh.ctr[3 * Cs + 1], Cz = sh.ctr[3 * Cs + 2];
-  const double ABx = Ax - sh.ctr[3 * B], ABy = Ay - sh.ctr[3 * B + 1], ABz = Az - sh.ctr[3 * B + 2];
-  const double CDx = Cx - sh.ctr[3 * Ds], CDy = Cy - sh.ctr[3 * Ds + 1], CDz = Cz - sh.ctr[3 * Ds + 2];
+  const double ABx = sh.ctr[3 * A] - sh.ctr[3 * B], ABy = sh.ctr[3 * A + 1] - sh.ctr[3 * B + 1],
+               ABz = sh.ctr[3 * A + 2] - sh.ctr[3 * B + 2];
+  const double CDx = sh.ctr[3 * Cs] - sh.ctr[3 * Ds], CDy = sh.ctr[3 * Cs + 1] - sh.ctr[3 * Ds + 1],
+               CDz = sh.ctr[3 * Cs + 2] - sh.ctr[3 * Ds + 2];
   const PrimPair64* __restrict__ bp = pr.pp64 + pr.poff[ip];
   const PrimPair64* __restrict__ kp = pr.pp64 + pr.poff[jp];
   const int nb = pr.np[ip], nk = pr.np[jp];
@@ -77,7 +81,6 @@ __device__ __forceinline__ void quartet64(const ShellView& sh, const PairView& p
   for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
   for (int ib = 0; ib < nb; ++ib) {
     const PrimPair64 b = bp[ib];
-    const double PAx = b.Px - Ax, PAy = b.Py - Ay, PAz = b.Pz - Az;
     for (int ik = 0; ik < nk; ++ik) {
       const PrimPair64 k = kp[ik];
       const double rs = gx_rsqrt(b.p + k.p);
@@ -85,13 +88,13 @@ __device__ __forceinline__ void quartet64(const ShellView& sh, const PairView& p
       const double rho = b.p * k.p * is;
       const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
       const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = b.K * k.K * b.ip * k.ip * rs;
+      const double pref = b.Kip * k.Kip * rs;
       double F[C::L + 1];
       boys64<C::L>(T, pref, tab, F);
       const double rho_p = rho * b.ip, rho_q = rho * k.ip;
-      C::template prim<double>(acc, F, PAx, PAy, PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
-                               k.Px - Cx, k.Py - Cy, k.Pz - Cz, rho_q * PQx, rho_q * PQy,
-                               rho_q * PQz, 0.5 * b.ip, 0.5 * k.ip, 0.5 * is, rho_p, rho_q);
+      C::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
+                               k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip,
+                               0.5 * k.ip, 0.5 * is, rho_p, rho_q);
     }
   }
   C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
@@ -103,8 +106,6 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
                                           float (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
   using C = EriClass<LA, LB, LC, LD>;
   const int A = pr.A[ip], B = pr.B[ip], Cs = pr.A[jp], Ds = pr.B[jp];
-  const float Ax = (float)sh.ctr[3 * A], Ay = (float)sh.ctr[3 * A + 1], Az = (float)sh.ctr[3 * A + 2];
-  const float Cx = (float)sh.ctr[3 * Cs], Cy = (float)sh.ctr[3 * Cs + 1], Cz = (float)sh.ctr[3 * Cs + 2];
   const float ABx = (float)(sh.ctr[3 * A] - sh.ctr[3 * B]);
   const float ABy = (float)(sh.ctr[3 * A + 1] - sh.ctr[3 * B + 1]);
   const float ABz = (float)(sh.ctr[3 * A + 2] - sh.ctr[3 * B + 2]);
@@ -119,7 +120,6 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
   for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0f;
   for (int ib = 0; ib < nb; ++ib) {
     const PrimPair32 b = bp[ib];
-    const float PAx = b.Px - Ax, PAy = b.Py - Ay, PAz = b.Pz - Az;
     for (int ik = 0; ik < nk; ++ik) {
       const PrimPair32 k = kp[ik];
       const float rs = gx_rsqrt(b.p + k.p);
@@ -127,36 +127,27 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
       const float rho = b.p * k.p * is;
       const float PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
       const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const float pref = b.K * k.K * b.ip * k.ip * rs;
+      const float pref = b.Kip * k.Kip * rs;
       float F[C::L + 1];
       boys32<C::L>(T, pref, tab, F);
       const float rho_p = rho * b.ip, rho_q = rho * k.ip;
-      C::template prim<float>(acc, F, PAx, PAy, PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
-                              k.Px - Cx, k.Py - Cy, k.Pz - Cz, rho_q * PQx, rho_q * PQy,
-                              rho_q * PQz, 0.5f * b.ip, 0.5f * k.ip, 0.5f * is, rho_p, rho_q);
+      C::template prim<float>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
+                              k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5f * b.ip,
+                              0.5f * k.ip, 0.5f * is, rho_p, rho_q);
     }
   }
   C::template hrr<float>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
 }
 
 // ---------------------------------------------------------------------------
-// J/K evaluators.  The bra shell pair is uniform across a workgroup, so its
-// primitive pairs are staged once per bra in LDS (with P - A and K/p
-// precomputed) and read as broadcasts; each lane loads its own ket
-// primitive pairs once, in the outer loop.
-struct BraRec64 {
-  double p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad;
-};
-struct BraRec32 {
-  float p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad0, pad1, pad2;
-};
-#define JK_MAXBRA 64  // primitive pairs of a bra staged in LDS, one per lane (BIG kernels read HBM)
-
+// J/K evaluators.  The bra shell pair is uniform across a wave, so its
+// primitive pairs come through the scalar data path (kernel-argument
+// pointers marked __restrict__ let the compiler use scalar loads); each lane
+// loads its own ket primitive pairs once, in the outer loop.
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void quartet64_jk(const BraRec64* __restrict__ bra, int nb,
-                                             const PrimPair64* __restrict__ kp, int nk, double Cx,
-                                             double Cy, double Cz, double ABx, double ABy, double ABz,
-                                             double CDx, double CDy, double CDz,
+__device__ __forceinline__ void quartet64_jk(const PrimPair64* __restrict__ bra, int nb,
+                                             const PrimPair64* __restrict__ kp, int nk, double ABx,
+                                             double ABy, double ABz, double CDx, double CDy, double CDz,
                                              const double* __restrict__ tab,
                                              double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
   using C = EriClass<LA, LB, LC, LD>;
@@ -165,22 +156,20 @@ __device__ __forceinline__ void quartet64_jk(const BraRec64* __restrict__ bra, i
   for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
   for (int ik = 0; ik < nk; ++ik) {
     const PrimPair64 k = kp[ik];
-    const double Kiq = k.K * k.ip;
-    const double QCx = k.Px - Cx, QCy = k.Py - Cy, QCz = k.Pz - Cz;
 #pragma unroll 2
     for (int ib = 0; ib < nb; ++ib) {
-      const BraRec64 b = bra[ib];
+      const PrimPair64 b = bra[ib];
       const double rs = gx_rsqrt(b.p + k.p);
       const double is = rs * rs;
       const double rho = b.p * k.p * is;
       const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
       const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = b.Kip * Kiq * rs;
+      const double pref = b.Kip * k.Kip * rs;
       double F[C::L + 1];
       boys64<C::L>(T, pref, tab, F);
       const double rho_p = rho * b.ip, rho_q = rho * k.ip;
       C::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
-                               QCx, QCy, QCz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip,
+                               k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip,
                                0.5 * k.ip, 0.5 * is, rho_p, rho_q);
     }
   }
@@ -188,10 +177,9 @@ __device__ __forceinline__ void quartet64_jk(const BraRec64* __restrict__ bra, i
 }
 
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void quartet32_jk(const BraRec32* __restrict__ bra, int nb,
-                                             const PrimPair32* __restrict__ kp, int nk, float Cx,
-                                             float Cy, float Cz, float ABx, float ABy, float ABz,
-                                             float CDx, float CDy, float CDz,
+__device__ __forceinline__ void quartet32_jk(const PrimPair32* __restrict__ bra, int nb,
+                                             const PrimPair32* __restrict__ kp, int nk, float ABx,
+                                             float ABy, float ABz, float CDx, float CDy, float CDz,
                                              const float* __restrict__ tab,
                                              float (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
   using C = EriClass<LA, LB, LC, LD>;
@@ -200,22 +188,20 @@ __device__ __forceinline__ void quartet32_jk(const BraRec32* __restrict__ bra, i
   for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0f;
   for (int ik = 0; ik < nk; ++ik) {
     const PrimPair32 k = kp[ik];
-    const float Kiq = k.K * k.ip;
-    const float QCx = k.Px - Cx, QCy = k.Py - Cy, QCz = k.Pz - Cz;
 #pragma unroll 2
     for (int ib = 0; ib < nb; ++ib) {
-      const BraRec32 b = bra[ib];
+      const PrimPair32 b = bra[ib];
       const float rs = gx_rsqrt(b.p + k.p);
       const float is = rs * rs;
       const float rho = b.p * k.p * is;
       const float PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
       const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const float pref = b.Kip * Kiq * rs;
+      const float pref = b.Kip * k.Kip * rs;
       float F[C::L + 1];
       boys32<C::L>(T, pref, tab, F);
       const float rho_p = rho * b.ip, rho_q = rho * k.ip;
       C::template prim<float>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
-                              QCx, QCy, QCz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5f * b.ip,
+                              k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5f * b.ip,
                               0.5f * k.ip, 0.5f * is, rho_p, rho_q);
     }
   }
@@ -335,7 +321,16 @@ __global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int 
 //                            whole wave would otherwise hit the same two rows)
 // with f the 8-fold degeneracy weight; J = Jt + Jt^T, K = Kt + Kt^T.
 #define JK_MAXSEG 16
-#define JK_THREADS 256
+#define JK_TEAM 256  // threads working on one bra
+
+// Teams (bras in flight) per workgroup.  Two teams share one Boys table but
+// couple their bras through the workgroup barriers; measured on MI355X at
+// 220 functions one team is faster, so every class uses one.
+template <int L>
+struct JKShape {
+  static constexpr int TEAMS = 1;
+  static constexpr int THREADS = TEAMS * JK_TEAM;
+};
 
 struct JKArgs {
   ShellView sh;
@@ -367,84 +362,69 @@ __device__ __forceinline__ int count_at_least(const double* __restrict__ Q, int 
   return a - lo;
 }
 
-template <int LA, int LB, int LC, int LD, bool BIG>
-__global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
+    JKArgs a, const PrimPair64* __restrict__ pp64, const PrimPair32* __restrict__ pp32) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+  constexpr int TEAMS = JKShape<C::L>::TEAMS;
   extern __shared__ double smem[];
-  __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1];
-  __shared__ double jab_s[NA * NB];
+  __shared__ int r_start_t[TEAMS][2 * JK_MAXSEG], r_pre_t[TEAMS][2 * JK_MAXSEG + 1];
+  __shared__ double jab_t[TEAMS][NA * NB];
   const int N = a.N;
-  // dynamic LDS: Boys FP64 | Boys FP32 (mixed) | bra FP64 | bra FP32 (mixed) | P rows | K rows
+  const int team = threadIdx.x / JK_TEAM;
+  const int tid = threadIdx.x - team * JK_TEAM;  // thread index inside the team
+  // dynamic LDS: Boys FP64 | Boys FP32 (mixed) | per team: P rows | K rows
   double* tab64 = smem;
   float* tab32 = (float*)(tab64 + BOYS_NT * BOYS_ROW64);
-  BraRec64* bra64 = (BraRec64*)(tab32 + (a.mixed ? BOYS_NT * BOYS_ROW32 : 0));
-  BraRec32* bra32 = (BraRec32*)(bra64 + (BIG ? 0 : JK_MAXBRA));
-  double* prow = (double*)(bra32 + ((a.mixed && !BIG) ? JK_MAXBRA : 0));
+  double* prow = (double*)(tab32 + (a.mixed ? BOYS_NT * BOYS_ROW32 : 0)) + team * 2 * (NA + NB) * N;
   double* krow = prow + (NA + NB) * N;
+  int* r_start = r_start_t[team];
+  int* r_pre = r_pre_t[team];
+  double* jab_s = jab_t[team];
   const int nr = 2 * a.nseg;  // range r: segment r/2, FP64 (even) or FP32 (odd)
 
   boys_stage64(C::L, a.master, tab64);
   if (a.mixed) boys_stage32(C::L, a.master, tab32);
 
   const int lane = threadIdx.x & (GX_WAVE - 1);
-  const int wave = threadIdx.x / GX_WAVE;
-  const int nwave = blockDim.x / GX_WAVE;
+  const int wave = tid / GX_WAVE;
+  constexpr int nwave = JK_TEAM / GX_WAVE;
 
-  for (int w = blockIdx.x; w < a.nbra; w += gridDim.x) {
-    const int bpos = a.order[w];
+  // Every team runs the same number of iterations so that the workgroup
+  // barriers line up; a team without a bra idles through its iteration.
+  for (int w0 = blockIdx.x * TEAMS; w0 < a.nbra; w0 += gridDim.x * TEAMS) {
+    const int w = w0 + team;
+    const bool has = w < a.nbra;
+    const int bpos = has ? a.order[w] : 0;
     const int ip = a.bra0 + bpos;
     const int A = a.pr.A[ip], B = a.pr.B[ip];
     const int ao = a.sh.bf[A], bo = a.sh.bf[B];
     const int rowB = (A == B) ? 0 : NA;
     const double QA = a.pr.Q[ip];
-    const int nbp = a.pr.np[ip];
-    const double Ax = a.sh.ctr[3 * A], Ay = a.sh.ctr[3 * A + 1], Az = a.sh.ctr[3 * A + 2];
-    const double ABx = Ax - a.sh.ctr[3 * B], ABy = Ay - a.sh.ctr[3 * B + 1], ABz = Az - a.sh.ctr[3 * B + 2];
+    // wave-uniform: kept in scalar registers so that the bra loop reads go
+    // through the scalar data path
+    const int nbp = __builtin_amdgcn_readfirstlane(a.pr.np[ip]);
+    const int bpoff = __builtin_amdgcn_readfirstlane(a.pr.poff[ip]);
+    const PrimPair64* __restrict__ bra64 = pp64 + bpoff;
+    const PrimPair32* __restrict__ bra32 = pp32 + bpoff;
+    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
+                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
     __syncthreads();  // the previous bra's flush is complete
-    for (int i = threadIdx.x; i < (NA + NB) * N; i += blockDim.x) {
+    for (int i = tid; has && i < (NA + NB) * N; i += JK_TEAM) {
       const int r = i / N, c = i - r * N;
       const int g = r < NA ? ao + r : bo + (r - NA);
       prow[i] = a.P[(long)g * N + c];
       krow[i] = 0.0;
     }
-    if (threadIdx.x < NA * NB) jab_s[threadIdx.x] = 0.0;
-    if (!BIG && threadIdx.x < nbp) {
-      const int t = threadIdx.x;
-      const PrimPair64 q = a.pr.pp64[a.pr.poff[ip] + t];
-      BraRec64 r;
-      r.p = q.p;
-      r.ip = q.ip;
-      r.Px = q.Px;
-      r.Py = q.Py;
-      r.Pz = q.Pz;
-      r.Kip = q.K * q.ip;
-      r.PAx = q.Px - Ax;
-      r.PAy = q.Py - Ay;
-      r.PAz = q.Pz - Az;
-      r.pad = 0.0;
-      bra64[t] = r;
-      if (a.mixed) {
-        BraRec32 r32;
-        r32.p = (float)r.p;
-        r32.ip = (float)r.ip;
-        r32.Px = (float)r.Px;
-        r32.Py = (float)r.Py;
-        r32.Pz = (float)r.Pz;
-        r32.Kip = (float)r.Kip;
-        r32.PAx = (float)r.PAx;
-        r32.PAy = (float)r.PAy;
-        r32.PAz = (float)r.PAz;
-        r32.pad0 = r32.pad1 = r32.pad2 = 0.0f;
-        bra32[t] = r32;
-      }
-    }
-    if (threadIdx.x < a.nseg) {
-      const int sg = threadIdx.x;
+    if (tid < NA * NB) jab_s[tid] = 0.0;
+    if (tid < a.nseg) {
+      const int sg = tid;
       const int lo = a.seg_lo[sg];
       int hi = a.seg_hi[sg];
       if (a.same && hi > ip + 1) hi = ip + 1;
       if (hi < lo) hi = lo;
+      if (!has) hi = lo;
       const int nsurv = a.cutoff > 0.0 ? count_at_least(a.pr.Q, lo, hi, QA, a.cutoff) : hi - lo;
       const int n64 = a.mixed ? min(nsurv, count_at_least(a.pr.Q, lo, hi, QA, a.fp32_bound)) : nsurv;
       r_start[2 * sg] = lo;
@@ -453,7 +433,7 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
       r_pre[2 * sg + 2] = nsurv - n64;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       int acc = 0;
       r_pre[0] = 0;
       for (int r = 0; r < nr; ++r) {
@@ -479,32 +459,21 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
       const bool hi = act && !(r & 1);
       double val[C::NOUT];
       int Cs = 0, Ds = 0;
-      double Cx = 0.0, Cy = 0.0, Cz = 0.0, CDx = 0.0, CDy = 0.0, CDz = 0.0;
+      double CDx = 0.0, CDy = 0.0, CDz = 0.0;
       if (act) {
         Cs = a.pr.A[jp];
         Ds = a.pr.B[jp];
-        Cx = a.sh.ctr[3 * Cs];
-        Cy = a.sh.ctr[3 * Cs + 1];
-        Cz = a.sh.ctr[3 * Cs + 2];
-        CDx = Cx - a.sh.ctr[3 * Ds];
-        CDy = Cy - a.sh.ctr[3 * Ds + 1];
-        CDz = Cz - a.sh.ctr[3 * Ds + 2];
+        CDx = a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds];
+        CDy = a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1];
+        CDz = a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2];
       }
-      if (hi) {
-        if (!BIG)
-          quartet64_jk<LA, LB, LC, LD>(bra64, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], Cx, Cy, Cz,
-                                       ABx, ABy, ABz, CDx, CDy, CDz, tab64, val);
-        else
-          quartet64<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab64, val);
-      }
+      if (hi)
+        quartet64_jk<LA, LB, LC, LD>(bra64, nbp, pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz, CDx,
+                                     CDy, CDz, tab64, val);
       if (lo) {
         float vf[C::NOUT];
-        if (!BIG)
-          quartet32_jk<LA, LB, LC, LD>(bra32, nbp, a.pr.pp32 + a.pr.poff[jp], a.pr.np[jp], (float)Cx,
-                                       (float)Cy, (float)Cz, (float)ABx, (float)ABy, (float)ABz,
-                                       (float)CDx, (float)CDy, (float)CDz, tab32, vf);
-        else
-          quartet32<LA, LB, LC, LD>(a.sh, a.pr, ip, jp, tab32, vf);
+        quartet32_jk<LA, LB, LC, LD>(bra32, nbp, pp32 + a.pr.poff[jp], a.pr.np[jp], (float)ABx,
+                                     (float)ABy, (float)ABz, (float)CDx, (float)CDy, (float)CDz, tab32, vf);
 #pragma unroll
         for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
       }
@@ -584,15 +553,15 @@ __global__ __launch_bounds__(JK_THREADS) void k_jk(JKArgs a) {
       if (lane == 0 && x != 0.0) atomicAdd(&jab_s[i], x);
     }
     __syncthreads();
-    if (threadIdx.x < NA * NB) {
-      const double x = jab_s[threadIdx.x];
+    if (tid < NA * NB) {
+      const double x = jab_s[tid];
       if (x != 0.0) {
-        const int ia = threadIdx.x / NB, ib = threadIdx.x - ia * NB;
+        const int ia = tid / NB, ib = tid - ia * NB;
         atomicAdd(&a.Jt[(long)(ao + ia) * N + bo + ib], 2.0 * x);
       }
     }
-    const int nrow = (A == B) ? NA : NA + NB;
-    for (int i = threadIdx.x; i < nrow * N; i += blockDim.x) {
+    const int nrow = !has ? 0 : (A == B) ? NA : NA + NB;
+    for (int i = tid; i < nrow * N; i += JK_TEAM) {
       const double x = krow[i];
       if (x != 0.0) {
         const int r = i / N, c = i - r * N;
