@@ -23,6 +23,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0
 CUTOFF = 1e-11
 FP32_BOUND = 1e-4
+COPY_BYTES = 256 << 20
 
 
 def parse():
@@ -33,25 +34,38 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=12.0,
                     help='wall seconds of the sampled CPU baseline (0 to skip)')
     ap.add_argument('--seed', type=int, default=2)
+    ap.add_argument('--kernels-only', action='store_true',
+                    help='counter runs: no SCF; time the J/K kernels on the core-guess density and run '
+                         'one %d MiB device copy as the HBM-counter calibration' % (COPY_BYTES >> 20))
     return ap.parse_args()
 
 
 def timed_builds(eng, P, steps, warmup, **kw):
+    """Median build time (torch events around each gx_jk call) and the per-kernel
+    durations from HIP events that gx_jk records on its own stream."""
     import torch
     for _ in range(warmup):
         eng.jk(P, **kw)
     torch.cuda.synchronize()
+    eng.jk_launch_times()  # forget earlier timed launches
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     t0 = time.perf_counter()
     for i in range(steps):
         starts[i].record()
-        eng.jk(P, **kw)
+        eng.jk(P, timing=True, **kw)
         ends[i].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
-    return {'median_ms': ms[len(ms) // 2], 'mean_ms': sum(ms) / len(ms), 'min_ms': ms[0], 'wall_s': wall}
+    kernels = {k: v[0] / v[1] for k, v in eng.jk_launch_times().items()}
+    return {'median_ms': ms[len(ms) // 2], 'mean_ms': sum(ms) / len(ms), 'min_ms': ms[0], 'wall_s': wall,
+            'kernel_ms': kernels}
+
+
+def kernel_name(combo, mixed):
+    la, lb, lc, ld = combo
+    return 'k_jk<%d, %d, %d, %d, %s>' % (la, lb, lc, ld, 'true' if mixed else 'false')
 
 
 def class_rows(eng, mixed):
@@ -79,12 +93,40 @@ def cpu_baseline(basis, seconds):
             'extrapolated_total_s': b['total_s'], 'eri_quartets_per_s': b['eri_quartets_per_s']}
 
 
+def kernels_only(args):
+    """The J/K kernels of both modes on the core-Hamiltonian guess density (the
+    work per kernel depends on the Schwarz bounds only, not on P), plus one
+    plain device copy of COPY_BYTES whose FETCH_SIZE / WRITE_SIZE calibrate the
+    HBM counters."""
+    import torch
+    from gaussium_amd import workloads
+    from gaussium_amd.engine import Engine
+    from gaussium_amd.scf import LinearAlgebra, density_matrix_restricted
+    nuc, basis, ne, info = workloads.cluster_220bf(seed=args.seed)
+    eng = Engine(basis, nuc)
+    S, T, V = eng.one_electron()
+    _, C = LinearAlgebra(S).diagonalize(T + V)
+    P = density_matrix_restricted(C, ne)
+    out = {'workload': info['workload'], 'copy_bytes': COPY_BYTES}
+    for name, kw in (('fp64_build', {}), ('mixed_build', dict(fp32_bound=FP32_BOUND, mixed=True))):
+        r = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF, **kw)
+        out[name] = {kernel_name(k, name == 'mixed_build'): v for k, v in r['kernel_ms'].items()}
+        out[name]['median_build_ms'] = r['median_ms']
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print('bench.py measures one GPU; --gpus %d is not supported (run with --gpus 1)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.kernels_only:
+        return kernels_only(args)
     import torch
     from gaussium_amd import workloads
     from gaussium_amd.basis import nuclear_repulsion
@@ -119,15 +161,25 @@ def main():
     ok = dE < 1e-6
     head = mixed if ok else fp64
     head_stats = stmx if ok else st64
+    head_rows = rowsmx if ok else rows64
     t = head['median_ms'] * 1e-3
+
+    # Roofline of the dominant kernel: the class-combination kernel with the
+    # largest share of the build, its model flops per launch over its mean
+    # event-timed duration.  Mixed kernels are priced against the blend of the
+    # FP64 and FP32 peaks that their own flop split implies.
+    combo = max(head['kernel_ms'], key=lambda k: head['kernel_ms'][k])
+    row = next(r for r in head_rows if tuple(r[:4]) == combo)
+    k64, k32 = jk_model_flops([row])
+    k_s = head['kernel_ms'][combo] * 1e-3
+    k_floor = k64 / (PEAK_FP64_TFLOPS * 1e12) + k32 / (PEAK_FP32_TFLOPS * 1e12)
+    k_peak = (k64 + k32) / k_floor / 1e12
+    k_achieved = (k64 + k32) / k_s / 1e12
     if ok:
-        flops = fm64 + fm32
-        floor_s = fm64 / (PEAK_FP64_TFLOPS * 1e12) + fm32 / (PEAK_FP32_TFLOPS * 1e12)
-        peak = flops / floor_s / 1e12
+        b_flops = fm64 + fm32
+        b_peak = b_flops / (fm64 / (PEAK_FP64_TFLOPS * 1e12) + fm32 / (PEAK_FP32_TFLOPS * 1e12)) / 1e12
     else:
-        flops = f64_a
-        peak = PEAK_FP64_TFLOPS
-    achieved = flops / t / 1e12
+        b_flops, b_peak = f64_a, PEAK_FP64_TFLOPS
     line = {
         'metric': 'ERI quartets/s + J/K build ms (220 bf), mixed vs FP64 at dE<1e-6 Eh',
         'value': head_stats['bf_quartets'] / t,
@@ -156,12 +208,17 @@ def main():
                      'evaluated_bf_mixed_build': [stmx['bf_quartets_fp64'], stmx['bf_quartets_fp32']],
                      'evaluated_bf_per_s': (stmx['bf_quartets_fp64'] + stmx['bf_quartets_fp32']) / t
                      if ok else st64['bf_quartets_fp64'] / t},
-        'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
-                     'frac': achieved / peak, 'traffic': None,
-                     'model_flops': flops, 'fp64_build': {'flops': f64_a,
-                                                          'achieved': f64_a / (fp64['median_ms'] * 1e-3) / 1e12,
-                                                          'frac': f64_a / (fp64['median_ms'] * 1e-3) / 1e12
-                                                          / PEAK_FP64_TFLOPS}},
+        'roofline': {'bound': 'valu', 'achieved': k_achieved, 'peak': k_peak, 'unit': 'TFLOP/s',
+                     'frac': k_achieved / k_peak, 'traffic': None,
+                     'kernel': kernel_name(combo, ok), 'kernel_ms': head['kernel_ms'][combo],
+                     'kernel_share': head['kernel_ms'][combo] / head['median_ms'],
+                     'flops_per_launch': [k64, k32],
+                     'build': {'flops': b_flops, 'achieved': b_flops / t / 1e12, 'peak': b_peak,
+                               'frac': b_flops / t / 1e12 / b_peak},
+                     'fp64_build': {'flops': f64_a, 'achieved': f64_a / (fp64['median_ms'] * 1e-3) / 1e12,
+                                    'frac': f64_a / (fp64['median_ms'] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}},
+        'kernel_ms': {'fp64_build': {kernel_name(k, False): v for k, v in fp64['kernel_ms'].items()},
+                      'mixed_build': {kernel_name(k, True): v for k, v in mixed['kernel_ms'].items()}},
     }
     line['class_counts'] = {'columns': ['la', 'lb', 'lc', 'ld', 'shq64', 'shq32', 'primq64', 'primq32', 'bfq64',
                                         'bfq32'], 'fp64_build': rows64, 'mixed_build': rowsmx}
@@ -169,7 +226,8 @@ def main():
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tr = json.load(f)
-        line['roofline']['traffic'] = tr.get('bytes_per_build_mixed' if ok else 'bytes_per_build_fp64')
+        per_kernel = tr.get('hbm_bytes_per_launch', {})
+        line['roofline']['traffic'] = per_kernel.get(line['roofline']['kernel'])
     if args.cpu_seconds > 0:
         cb = cpu_baseline(basis, args.cpu_seconds)
         line['cpu_baseline'] = cb

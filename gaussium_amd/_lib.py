@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libgx.so')
 
 GX_JK_MIXED = 1
+GX_JK_TIMING = 2
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -44,6 +45,7 @@ SIGNATURES = {
                                    ctypes.POINTER(GxStats)]),
     'gx_jk_class_counts': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                           ctypes.c_void_p, ctypes.c_int]),
+    'gx_jk_launch_times': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     'gx_one_electron': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 

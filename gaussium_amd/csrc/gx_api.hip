@@ -112,6 +112,8 @@ struct gx_ctx {
   hipStream_t stream = nullptr;
   int max_lds = 65536;
   double* d_boys = nullptr;
+  double* d_tab64 = nullptr;  // per-L Taylor tables (L <= BOYS_MCOL - 8)
+  float* d_tab32 = nullptr;
   // basis
   int nshell = 0, nbf = 0, nprim = 0;
   std::vector<int> l, np, poff, bf, atom;
@@ -145,6 +147,10 @@ struct gx_ctx {
   int natom = 0;
   std::vector<double> nxyz, nZ;
   double *d_nxyz = nullptr, *d_nZ = nullptr;
+  // launch events of gx_jk calls made with GX_JK_TIMING since the last read:
+  // pairs (ev[2i], ev[2i+1]) around the launch of class combo ev_combo[i]
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_combo;
 
   gx::ShellView shells() const { return gx::ShellView{d_l, d_bf, d_ctr}; }
   gx::PairView pairs() const { return gx::PairView{d_pA, d_pB, d_ppoff, d_pnp, d_Q, d_pp64, d_pp32}; }
@@ -181,6 +187,11 @@ struct gx_ctx {
     dfree(&d_nxyz);
     dfree(&d_nZ);
     dfree(&d_boys);
+    dfree(&d_tab64);
+    dfree(&d_tab32);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    ev_combo.clear();
   }
 };
 
@@ -234,7 +245,8 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
                      const gx::JKArgs& a) {
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
-    const void* fn = (const void*)gx::k_jk<A, B, C, D>;                                      \
+    const void* fn = a.mixed ? (const void*)gx::k_jk<A, B, C, D, true>                       \
+                             : (const void*)gx::k_jk<A, B, C, D, false>;                      \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
     constexpr int TPB = gx::JKShape<A + B + C + D>::THREADS;                                  \
@@ -244,8 +256,12 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
     if (e != hipSuccess) return e;                                                            \
     if (per_cu < 1) per_cu = 1;                                                               \
     const int grid = std::min((a.nbra + TEAMS - 1) / TEAMS, per_cu * num_cu);                 \
-    hipLaunchKernelGGL((gx::k_jk<A, B, C, D>), dim3(grid), dim3(TPB), lds, s, a, a.pr.pp64,   \
-                       a.pr.pp32);                                                            \
+    if (a.mixed)                                                                              \
+      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, true>), dim3(grid), dim3(TPB), lds, s, a,      \
+                         a.pr.pp64, a.pr.pp32);                                               \
+    else                                                                                      \
+      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, false>), dim3(grid), dim3(TPB), lds, s, a,     \
+                         a.pr.pp64, a.pr.pp32);                                               \
     return hipGetLastError();                                                                 \
   }
   GX_FOR_EACH_CLASS(GX_CASE)
@@ -257,8 +273,7 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
 size_t jk_lds_bytes(int la, int lb, int N, int mixed) {
   const int na = ncart(la), nb = ncart(lb);
   const int teams = gx::JKShape<0>::TEAMS;  // the same for every L
-  size_t b = (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
-  if (mixed) b += (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float);
+  size_t b = mixed ? 0 : (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
   return b + teams * 2 * (size_t)(na + nb) * N * sizeof(double);
 }
 
@@ -418,6 +433,14 @@ int gx_create(gx_ctx** out, int device) {
   std::vector<double> tab;
   boys_master_table(tab);
   int rc = upload(&c->d_boys, tab, c->stream);
+  const int ltab = BOYS_MCOL - BOYS_TAYLOR64;  // highest L whose Taylor columns exist
+  if (rc == GX_OK) rc = dalloc(&c->d_tab64, (size_t)(ltab + 1) * BOYS_NT * BOYS_ROW64);
+  if (rc == GX_OK) rc = dalloc(&c->d_tab32, (size_t)(ltab + 1) * BOYS_NT * BOYS_ROW32);
+  if (rc == GX_OK) {
+    hipLaunchKernelGGL(gx::k_boys_tables, dim3(ltab + 1), dim3(256), 0, c->stream, ltab, c->d_boys,
+                       c->d_tab64, c->d_tab32);
+    if (hipGetLastError() != hipSuccess) rc = fail(GX_ERR_HIP, "k_boys_tables launch");
+  }
   if (rc == GX_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GX_ERR_HIP, "boys upload");
   if (rc != GX_OK) {
     c->free_all();
@@ -771,6 +794,20 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, fp32_bound, mixed));
+  const bool timing = (flags & GX_JK_TIMING) != 0;
+  // one event pair per launch; the pool grows on first use and is reused
+  // after every gx_jk_launch_times read
+  auto next_events = [&](int combo_id) -> hipError_t {
+    const size_t i = c->ev_combo.size();
+    while (c->ev.size() < 2 * i + 2) {
+      hipEvent_t e;
+      hipError_t err = hipEventCreate(&e);
+      if (err != hipSuccess) return err;
+      c->ev.push_back(e);
+    }
+    c->ev_combo.push_back(combo_id);
+    return hipSuccess;
+  };
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
@@ -789,6 +826,8 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       a.nseg = c->cls_seg[Y + 1] - c->cls_seg[Y];
       a.order = c->d_order[combo];
       a.master = c->d_boys;
+      a.tab64 = c->d_tab64;
+      a.tab32 = c->d_tab32;
       a.P = c->d_Ps;
       a.Jt = c->d_Jt;
       a.Kt = c->d_Kt;
@@ -796,8 +835,13 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       a.cutoff = cutoff;
       a.fp32_bound = fp32_bound;
       a.mixed = mixed;
+      if (timing) {
+        GX_HIP(next_events(combo));
+        GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
+      }
       GX_HIP(launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
                        jk_lds_bytes(class_la(X), class_lb(X), N, mixed), c->stream, a));
+      if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
     }
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, c->d_Jt,
                      c->d_Kt, J, K);
@@ -860,6 +904,35 @@ int gx_jk_class_counts(gx_ctx* c, double cutoff, double fp32_bound, int flags, i
         r[9] += b32;
       });
       ++row;
+    }
+  return row;
+}
+
+int gx_jk_launch_times(gx_ctx* c, double* out, int cap) {
+  if (!c || !out) return fail(GX_ERR_ARG, "gx_jk_launch_times: null argument");
+  GX_HIP(hipSetDevice(c->device));
+  const int nc = combo_count();
+  std::vector<double> ms(nc, 0.0), cnt(nc, 0.0);
+  if (!c->ev_combo.empty()) GX_HIP(hipEventSynchronize(c->ev[2 * c->ev_combo.size() - 1]));
+  for (size_t i = 0; i < c->ev_combo.size(); ++i) {
+    float t = 0.f;
+    GX_HIP(hipEventElapsedTime(&t, c->ev[2 * i], c->ev[2 * i + 1]));
+    ms[c->ev_combo[i]] += t;
+    cnt[c->ev_combo[i]] += 1.0;
+  }
+  c->ev_combo.clear();
+  int row = 0, combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      if (cnt[combo] == 0.0) continue;
+      if (row >= cap) return fail(GX_ERR_ARG, "gx_jk_launch_times: output too small");
+      double* r = out + 6 * row++;
+      r[0] = class_la(X);
+      r[1] = class_lb(X);
+      r[2] = class_la(Y);
+      r[3] = class_lb(Y);
+      r[4] = ms[combo];
+      r[5] = cnt[combo];
     }
   return row;
 }

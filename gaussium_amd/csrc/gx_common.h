@@ -170,6 +170,92 @@ __device__ __forceinline__ void boys32(float T, float scale, const float* __rest
   }
 }
 
+// Branch-free variants: both the table and the asymptotic forms are
+// evaluated (index clamped to the table) and one is selected, so that a loop
+// over several primitive quartets stays a single basic block and the table
+// gathers of all of them can be in flight together.
+template <int L>
+__device__ __forceinline__ void boys64_nb(double T, double scale, const double* __restrict__ tab,
+                                          double (&F)[L + 1]) {
+  const bool small = T < BOYS_TMAX;
+  const double Tc = small ? T : 0.0;
+  const int n = (int)(Tc * BOYS_INV_DT + 0.5);
+  const double x = n * BOYS_DT - Tc;
+  const double* row = tab + n * BOYS_ROW64;
+  double f = row[BOYS_TAYLOR64 - 1];
+#pragma unroll
+  for (int k = BOYS_TAYLOR64 - 2; k >= 0; --k) f = fma(f, x, row[k]);
+  const double rt = gx_rsqrt(small ? 1.0 : T);
+  const double it = rt * rt;
+  double g = 0.88622692545275801365 * rt;  // asymptotic F_0
+  if (L == 0) {
+    F[0] = (small ? f : g) * scale;
+    return;
+  }
+  double e = 1.0 / 5040.0;
+  e = fma(e, x, 1.0 / 720.0);
+  e = fma(e, x, 1.0 / 120.0);
+  e = fma(e, x, 1.0 / 24.0);
+  e = fma(e, x, 1.0 / 6.0);
+  e = fma(e, x, 0.5);
+  e = fma(e, x, 1.0);
+  e = fma(e, x, 1.0);
+  e *= row[BOYS_TAYLOR64];
+  double Fs[L + 1], Fa[L + 1];
+  Fs[L] = f;
+  const double t2 = 2.0 * Tc;
+#pragma unroll
+  for (int m = L; m >= 1; --m) {
+    f = fma(t2, f, e) * (1.0 / (2 * m - 1));
+    Fs[m - 1] = f;
+  }
+  Fa[0] = g;
+#pragma unroll
+  for (int m = 0; m < L; ++m) {
+    g = g * ((m + 0.5) * it);
+    Fa[m + 1] = g;
+  }
+#pragma unroll
+  for (int m = 0; m <= L; ++m) F[m] = (small ? Fs[m] : Fa[m]) * scale;
+}
+
+template <int L>
+__device__ __forceinline__ void boys32_nb(float T, float scale, const float* __restrict__ tab,
+                                          float (&F)[L + 1]) {
+  const bool small = T < (float)BOYS_TMAX;
+  const float Tc = small ? T : 0.0f;
+  const int n = (int)(Tc * (float)BOYS_INV_DT + 0.5f);
+  const float x = n * (float)BOYS_DT - Tc;
+  const float* row = tab + n * BOYS_ROW32;
+  float f = row[BOYS_TAYLOR32 - 1];
+#pragma unroll
+  for (int k = BOYS_TAYLOR32 - 2; k >= 0; --k) f = fmaf(f, x, row[k]);
+  const float rt = gx_rsqrt(small ? 1.0f : T);
+  const float it = rt * rt;
+  float g = 0.886226925f * rt;
+  if (L == 0) {
+    F[0] = (small ? f : g) * scale;
+    return;
+  }
+  const float e = __expf(-Tc);
+  float Fs[L + 1], Fa[L + 1];
+  Fs[L] = f;
+  const float t2 = 2.0f * Tc;
+#pragma unroll
+  for (int m = L; m >= 1; --m) {
+    f = fmaf(t2, f, e) * (1.0f / (2 * m - 1));
+    Fs[m - 1] = f;
+  }
+  Fa[0] = g;
+#pragma unroll
+  for (int m = 0; m < L; ++m) {
+    g = g * ((m + 0.5f) * it);
+    Fa[m + 1] = g;
+  }
+#pragma unroll
+  for (int m = 0; m <= L; ++m) F[m] = (small ? Fs[m] : Fa[m]) * scale;
+}
+
 // Stage the class Taylor tables for total angular momentum L into LDS.
 // Master layout: master[n * (BOYS_MCOL + 1) + m] = F_m(T_n), column
 // BOYS_MCOL = exp(-T_n).

@@ -144,36 +144,84 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 // primitive pairs come through the scalar data path (kernel-argument
 // pointers marked __restrict__ let the compiler use scalar loads); each lane
 // loads its own ket primitive pairs once, in the outer loop.
-template <int LA, int LB, int LC, int LD>
+// Bra primitive pairs are consumed in groups of JKG (FP64) / JKG32 (FP32):
+// the group's scalar record loads and table gathers are issued together and
+// the branch-free Boys evaluation keeps the group one basic block.
+#ifndef GX_JKG
+#define GX_JKG 2
+#endif
+#ifndef GX_JKG32
+#define GX_JKG32 2
+#endif
+#ifndef GX_JKGM
+#define GX_JKGM 1  // FP64 group inside mixed builds (few FP64 quartets there)
+#endif
+
+template <class C, typename R, typename PP, int G>
+__device__ __forceinline__ void prim_group(R (&acc)[C::NACC], const PP* __restrict__ b, const PP& k,
+                                           const R* __restrict__ tab) {
+  R rs[G], rho[G], PQx[G], PQy[G], PQz[G], T[G], pref[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    rs[g] = gx_rsqrt(b[g].p + k.p);
+    const R is = rs[g] * rs[g];
+    rho[g] = b[g].p * k.p * is;
+    PQx[g] = b[g].Px - k.Px;
+    PQy[g] = b[g].Py - k.Py;
+    PQz[g] = b[g].Pz - k.Pz;
+    T[g] = rho[g] * (PQx[g] * PQx[g] + PQy[g] * PQy[g] + PQz[g] * PQz[g]);
+    pref[g] = b[g].Kip * k.Kip * rs[g];
+  }
+  R F[G][C::L + 1];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if constexpr (sizeof(R) == 8)
+      boys64_nb<C::L>(T[g], pref[g], tab, F[g]);
+    else
+      boys32_nb<C::L>(T[g], pref[g], tab, F[g]);
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const R is = rs[g] * rs[g];
+    const R rho_p = rho[g] * b[g].ip, rho_q = rho[g] * k.ip;
+    C::template prim<R>(acc, F[g], b[g].PAx, b[g].PAy, b[g].PAz, -rho_p * PQx[g], -rho_p * PQy[g],
+                        -rho_p * PQz[g], k.PAx, k.PAy, k.PAz, rho_q * PQx[g], rho_q * PQy[g],
+                        rho_q * PQz[g], R(0.5) * b[g].ip, R(0.5) * k.ip, R(0.5) * is, rho_p, rho_q);
+  }
+}
+
+template <class C, typename R, typename PP, int G>
+__device__ __forceinline__ void quartet_jk(const PP* __restrict__ bra, int nb, const PP* __restrict__ kp,
+                                           int nk, R ABx, R ABy, R ABz, R CDx, R CDy, R CDz,
+                                           const R* __restrict__ tab, R (&out)[C::NOUT]) {
+  R acc[C::NACC];
+#pragma unroll
+  for (int i = 0; i < C::NACC; ++i) acc[i] = R(0);
+  for (int ik = 0; ik < nk; ++ik) {
+    const PP k = kp[ik];
+    int ib = 0;
+    for (; ib + G <= nb; ib += G) {
+      PP b[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) b[g] = bra[ib + g];
+      prim_group<C, R, PP, G>(acc, b, k, tab);
+    }
+    for (; ib < nb; ++ib) {
+      PP b[1] = {bra[ib]};
+      prim_group<C, R, PP, 1>(acc, b, k, tab);
+    }
+  }
+  C::template hrr<R>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+template <int LA, int LB, int LC, int LD, int G>
 __device__ __forceinline__ void quartet64_jk(const PrimPair64* __restrict__ bra, int nb,
                                              const PrimPair64* __restrict__ kp, int nk, double ABx,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
                                              const double* __restrict__ tab,
                                              double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
-  using C = EriClass<LA, LB, LC, LD>;
-  double acc[C::NACC];
-#pragma unroll
-  for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
-  for (int ik = 0; ik < nk; ++ik) {
-    const PrimPair64 k = kp[ik];
-#pragma unroll 2
-    for (int ib = 0; ib < nb; ++ib) {
-      const PrimPair64 b = bra[ib];
-      const double rs = gx_rsqrt(b.p + k.p);
-      const double is = rs * rs;
-      const double rho = b.p * k.p * is;
-      const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
-      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = b.Kip * k.Kip * rs;
-      double F[C::L + 1];
-      boys64<C::L>(T, pref, tab, F);
-      const double rho_p = rho * b.ip, rho_q = rho * k.ip;
-      C::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
-                               k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip,
-                               0.5 * k.ip, 0.5 * is, rho_p, rho_q);
-    }
-  }
-  C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+  quartet_jk<EriClass<LA, LB, LC, LD>, double, PrimPair64, G>(bra, nb, kp, nk, ABx, ABy, ABz, CDx, CDy,
+                                                               CDz, tab, out);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -182,30 +230,8 @@ __device__ __forceinline__ void quartet32_jk(const PrimPair32* __restrict__ bra,
                                              float ABy, float ABz, float CDx, float CDy, float CDz,
                                              const float* __restrict__ tab,
                                              float (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
-  using C = EriClass<LA, LB, LC, LD>;
-  float acc[C::NACC];
-#pragma unroll
-  for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0f;
-  for (int ik = 0; ik < nk; ++ik) {
-    const PrimPair32 k = kp[ik];
-#pragma unroll 2
-    for (int ib = 0; ib < nb; ++ib) {
-      const PrimPair32 b = bra[ib];
-      const float rs = gx_rsqrt(b.p + k.p);
-      const float is = rs * rs;
-      const float rho = b.p * k.p * is;
-      const float PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
-      const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const float pref = b.Kip * k.Kip * rs;
-      float F[C::L + 1];
-      boys32<C::L>(T, pref, tab, F);
-      const float rho_p = rho * b.ip, rho_q = rho * k.ip;
-      C::template prim<float>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
-                              k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5f * b.ip,
-                              0.5f * k.ip, 0.5f * is, rho_p, rho_q);
-    }
-  }
-  C::template hrr<float>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+  quartet_jk<EriClass<LA, LB, LC, LD>, float, PrimPair32, GX_JKG32>(bra, nb, kp, nk, ABx, ABy, ABz, CDx,
+                                                                    CDy, CDz, tab, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -341,6 +367,8 @@ struct JKArgs {
   int nseg;
   const int* order;      // bra positions, largest estimated work first
   const double* master;  // Boys master table
+  const double* tab64;   // per-L FP64 Taylor tables (BOYS_NT x BOYS_ROW64 each)
+  const float* tab32;    // per-L FP32 Taylor tables
   const double* P;       // density, pre-scaled by s_k s_l
   double* Jt;
   double* Kt;
@@ -362,7 +390,11 @@ __device__ __forceinline__ int count_at_least(const double* __restrict__ Q, int 
   return a - lo;
 }
 
-template <int LA, int LB, int LC, int LD>
+// MIXED = false: pure FP64 build, the FP32 path is compiled out and the FP64
+// Taylor table is staged in LDS.  MIXED = true: most primitive quartets are
+// FP32 and both tables are read through the vector-memory path (L1), which
+// leaves the LDS to the P/K rows and raises occupancy.
+template <int LA, int LB, int LC, int LD, bool MIXED>
 __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
     JKArgs a, const PrimPair64* __restrict__ pp64, const PrimPair32* __restrict__ pp32) {
   using C = EriClass<LA, LB, LC, LD>;
@@ -370,22 +402,23 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
   constexpr int TEAMS = JKShape<C::L>::TEAMS;
   extern __shared__ double smem[];
   __shared__ int r_start_t[TEAMS][2 * JK_MAXSEG], r_pre_t[TEAMS][2 * JK_MAXSEG + 1];
+  __shared__ int r_len_t[TEAMS][2 * JK_MAXSEG];
   __shared__ double jab_t[TEAMS][NA * NB];
   const int N = a.N;
   const int team = threadIdx.x / JK_TEAM;
   const int tid = threadIdx.x - team * JK_TEAM;  // thread index inside the team
-  // dynamic LDS: Boys FP64 | Boys FP32 (mixed) | per team: P rows | K rows
-  double* tab64 = smem;
-  float* tab32 = (float*)(tab64 + BOYS_NT * BOYS_ROW64);
-  double* prow = (double*)(tab32 + (a.mixed ? BOYS_NT * BOYS_ROW32 : 0)) + team * 2 * (NA + NB) * N;
+  // dynamic LDS: Boys FP64 (pure FP64 build) | per team: P rows | K rows
+  const double* __restrict__ tab64 = !MIXED ? smem : a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
+  const float* __restrict__ tab32 = a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
+  double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : 0) + team * 2 * (NA + NB) * N;
   double* krow = prow + (NA + NB) * N;
   int* r_start = r_start_t[team];
   int* r_pre = r_pre_t[team];
+  int* r_len = r_len_t[team];
   double* jab_s = jab_t[team];
-  const int nr = 2 * a.nseg;  // range r: segment r/2, FP64 (even) or FP32 (odd)
+  const int nr = 2 * a.nseg;  // range r < nseg: FP64 part of segment r; else FP32 part of r - nseg
 
-  boys_stage64(C::L, a.master, tab64);
-  if (a.mixed) boys_stage32(C::L, a.master, tab32);
+  if (!MIXED) boys_stage64(C::L, a.master, smem);
 
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = tid / GX_WAVE;
@@ -427,19 +460,28 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
       if (!has) hi = lo;
       const int nsurv = a.cutoff > 0.0 ? count_at_least(a.pr.Q, lo, hi, QA, a.cutoff) : hi - lo;
       const int n64 = a.mixed ? min(nsurv, count_at_least(a.pr.Q, lo, hi, QA, a.fp32_bound)) : nsurv;
-      r_start[2 * sg] = lo;
-      r_start[2 * sg + 1] = lo + n64;
-      r_pre[2 * sg + 1] = n64;  // lengths for now
-      r_pre[2 * sg + 2] = nsurv - n64;
+      // ranges: all FP64 parts first (r = sg), then all FP32 parts (r = nseg + sg),
+      // so that a bra has at most one precision boundary inside a chunk
+      r_start[sg] = lo;
+      r_start[a.nseg + sg] = lo + n64;
+      r_pre[sg + 1] = n64;  // lengths for now
+      r_pre[a.nseg + sg + 1] = nsurv - n64;
     }
     __syncthreads();
     if (tid == 0) {
+      // r_pre[r]: virtual start of range r, r_pre[r + 1] - (pad) its end.  The
+      // FP32 ranges start at the next multiple of 64, so no chunk of 64 mixes
+      // precisions (the lanes of the gap stay idle).
       int acc = 0;
-      r_pre[0] = 0;
       for (int r = 0; r < nr; ++r) {
-        acc += r_pre[r + 1];
-        r_pre[r + 1] = acc;
+        const int len = r_pre[r + 1];
+        if (r == a.nseg) acc = (acc + GX_WAVE - 1) & ~(GX_WAVE - 1);
+        r_pre[r + 1] = acc + len;  // end of range r for now
+        r_len[r] = len;
+        acc += len;
       }
+      for (int r = 0; r < nr; ++r) r_pre[r] = r_pre[r + 1] - r_len[r];  // starts
+      r_pre[nr] = acc;
     }
     __syncthreads();
     const int total = r_pre[nr];
@@ -450,13 +492,13 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
 
     for (int v0 = wave * GX_WAVE; v0 < total; v0 += nwave * GX_WAVE) {
       const int v = v0 + lane;
-      const bool act = v < total;
       int r = 0;
-      if (act)
-        while (v >= r_pre[r + 1]) ++r;
+      if (v < total)
+        while (r + 1 < nr && v >= r_pre[r + 1]) ++r;
+      const bool act = v < total && v < r_pre[r] + r_len[r];
       const int jp = act ? r_start[r] + (v - r_pre[r]) : 0;
-      const bool lo = act && (r & 1);
-      const bool hi = act && !(r & 1);
+      const bool lo = act && r >= a.nseg;
+      const bool hi = act && r < a.nseg;
       double val[C::NOUT];
       int Cs = 0, Ds = 0;
       double CDx = 0.0, CDy = 0.0, CDz = 0.0;
@@ -468,14 +510,18 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
         CDz = a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2];
       }
       if (hi)
-        quartet64_jk<LA, LB, LC, LD>(bra64, nbp, pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz, CDx,
-                                     CDy, CDz, tab64, val);
-      if (lo) {
-        float vf[C::NOUT];
-        quartet32_jk<LA, LB, LC, LD>(bra32, nbp, pp32 + a.pr.poff[jp], a.pr.np[jp], (float)ABx,
-                                     (float)ABy, (float)ABz, (float)CDx, (float)CDy, (float)CDz, tab32, vf);
+        quartet64_jk<LA, LB, LC, LD, MIXED ? GX_JKGM : GX_JKG>(bra64, nbp, pp64 + a.pr.poff[jp],
+                                                              a.pr.np[jp], ABx, ABy, ABz, CDx, CDy, CDz,
+                                                              tab64, val);
+      if constexpr (MIXED) {
+        if (lo) {
+          float vf[C::NOUT];
+          quartet32_jk<LA, LB, LC, LD>(bra32, nbp, pp32 + a.pr.poff[jp], a.pr.np[jp], (float)ABx,
+                                       (float)ABy, (float)ABz, (float)CDx, (float)CDy, (float)CDz, tab32,
+                                       vf);
 #pragma unroll
-        for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
+          for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
+        }
       }
       if (act) {
         const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
@@ -590,6 +636,15 @@ __global__ void k_scale_density(int N, const double* __restrict__ scale, const d
   if (t >= N * N) return;
   const int i = t / N, j = t - i * N;
   Ps[t] = P[t] * scale[i] * scale[j];
+}
+
+// Per-L Taylor tables in HBM, the same layout the kernels stage into LDS.
+__global__ void k_boys_tables(int lmax, const double* __restrict__ master, double* __restrict__ t64,
+                              float* __restrict__ t32) {
+  const int L = blockIdx.x;
+  if (L > lmax) return;
+  boys_stage64(L, master, t64 + (long)L * BOYS_NT * BOYS_ROW64);
+  boys_stage32(L, master, t32 + (long)L * BOYS_NT * BOYS_ROW32);
 }
 
 __global__ void k_fill(long n, double* __restrict__ x, double v) {

@@ -110,17 +110,31 @@ class Engine:
         _lib.check(self.lib.gx_eri_dense(self.ctx, _p(out)), 'gx_eri_dense')
         return out
 
-    def jk(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False, out=None):
-        """Direct (J, K) from density ``P`` (FP64 device tensors)."""
+    def jk(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False, out=None, timing=False):
+        """Direct (J, K) from density ``P`` (FP64 device tensors).
+
+        ``timing=True`` records HIP events around each class-combination
+        kernel on the stream it runs on; read them with ``jk_launch_times``.
+        """
         n = self.nbf
         P = self._dev(P)
         if P.shape != (n, n):
             raise ValueError('density has shape %s, expected (%d, %d)' % (tuple(P.shape), n, n))
         J, K = (self._empty(n, n), self._empty(n, n)) if out is None else out
         self._bind_stream()
-        _lib.check(self.lib.gx_jk(self.ctx, _p(P), _p(J), _p(K), float(cutoff), float(fp32_bound),
-                                  _lib.GX_JK_MIXED if mixed else 0), 'gx_jk')
+        flags = (_lib.GX_JK_MIXED if mixed else 0) | (_lib.GX_JK_TIMING if timing else 0)
+        _lib.check(self.lib.gx_jk(self.ctx, _p(P), _p(J), _p(K), float(cutoff), float(fp32_bound), flags),
+                   'gx_jk')
         return J, K
+
+    def jk_launch_times(self):
+        """{(la, lb, lc, ld): (total ms, launches)} over the ``jk(..., timing=True)``
+        calls since the previous read."""
+        out = np.zeros((64, 6))
+        n = self.lib.gx_jk_launch_times(self.ctx, _p(out), 64)
+        if n < 0:
+            _lib.check(n, 'gx_jk_launch_times')
+        return {tuple(int(x) for x in r[:4]): (float(r[4]), int(r[5])) for r in out[:n]}
 
     def jk_stats(self, cutoff=0.0, fp32_bound=0.0, mixed=False):
         st = _lib.GxStats()

@@ -48,6 +48,7 @@ extern "C" {
 
 /* gx_jk flags */
 #define GX_JK_MIXED 1        /* quartets with Schwarz bound < fp32_bound in FP32 */
+#define GX_JK_TIMING 2       /* record HIP events around each class-combination launch */
 
 typedef struct gx_ctx gx_ctx;
 
@@ -127,6 +128,15 @@ int gx_jk_stats(gx_ctx* ctx, double cutoff, double fp32_bound, int flags, gx_sta
  *   basis-function quartets FP64, basis-function quartets FP32.
  * Returns the number of rows (>= 0) or an error code. */
 int gx_jk_class_counts(gx_ctx* ctx, double cutoff, double fp32_bound, int flags, int64_t* out, int cap);
+
+/* Kernel durations of the gx_jk calls made with GX_JK_TIMING since the
+ * previous call of this function, from HIP events recorded on the context's
+ * stream around each class-combination launch.  One row of 6 doubles per
+ * class combination that ran: la, lb, lc, ld, total milliseconds, launches;
+ * at most `cap` rows.  Waits for the last event, then forgets the recorded
+ * launches.  Returns the number of rows or an error code.  No counterpart in
+ * the reference (measurement). */
+int gx_jk_launch_times(gx_ctx* ctx, double* out, int cap);
 
 /* One-electron matrices S, T, V (nbf x nbf FP64 device buffers); V needs
  * gx_set_nuclei first. */
