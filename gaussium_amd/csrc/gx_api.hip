@@ -131,7 +131,13 @@ struct gx_ctx {
   double* d_Q = nullptr;
   gx::PrimPair64* d_pp64 = nullptr;
   gx::PrimPair32* d_pp32 = nullptr;
-  std::vector<int*> d_order;  // per class combo: bra positions, largest work first
+  // per class combo: work items (4 ints: bra position, virtual ket range
+  // [v0, v1), bra primitive-pair range p0 | p1 << 16), largest first; and per
+  // bra position the surviving ket counts of each ket segment (FP64 parts,
+  // then FP32 parts)
+  std::vector<int*> d_items, d_rng;
+  std::vector<int> n_items;
+  std::vector<int> wg_per_cu;  // [combo][mixed] resident workgroups per CU, 0 = not queried
   // ket segments: per class, runs of pairs with the same primitive-count key
   std::vector<int> seg_lo, seg_hi, cls_seg;  // cls_seg: NCLS + 1 offsets into seg_*
   int *d_seg_lo = nullptr, *d_seg_hi = nullptr;
@@ -163,8 +169,12 @@ struct gx_ctx {
     dfree(&d_Q);
     dfree(&d_pp64);
     dfree(&d_pp32);
-    for (auto& p : d_order) dfree(&p);
-    d_order.clear();
+    for (auto& p : d_items) dfree(&p);
+    for (auto& p : d_rng) dfree(&p);
+    d_items.clear();
+    d_rng.clear();
+    n_items.clear();
+    wg_per_cu.clear();
     dfree(&d_seg_lo);
     dfree(&d_seg_hi);
     ord_cutoff = ord_fp32 = -1.0;
@@ -241,26 +251,28 @@ hipError_t launch_quartets(int la, int lb, int lc, int ld, dim3 g, hipStream_t s
   return hipErrorInvalidValue;
 }
 
+// Persistent J/K launch: as many workgroups as stay resident (queried once
+// per kernel and basis, cached in *per_cu), at most one per work item.
 hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hipStream_t s,
-                     const gx::JKArgs& a) {
+                     const gx::JKArgs& a, int* per_cu) {
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
     const void* fn = a.mixed ? (const void*)gx::k_jk<A, B, C, D, true>                       \
                              : (const void*)gx::k_jk<A, B, C, D, false>;                      \
+    /* the attribute is per function, shared by every context: set it each time */          \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
-    constexpr int TPB = gx::JKShape<A + B + C + D>::THREADS;                                  \
-    constexpr int TEAMS = gx::JKShape<A + B + C + D>::TEAMS;                                  \
-    int per_cu = 0;                                                                           \
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TPB, lds);                  \
-    if (e != hipSuccess) return e;                                                            \
-    if (per_cu < 1) per_cu = 1;                                                               \
-    const int grid = std::min((a.nbra + TEAMS - 1) / TEAMS, per_cu * num_cu);                 \
+    if (*per_cu <= 0) {                                                                       \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, JK_TEAM, lds);             \
+      if (e != hipSuccess) return e;                                                          \
+      if (*per_cu < 1) *per_cu = 1;                                                           \
+    }                                                                                         \
+    const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
     if (a.mixed)                                                                              \
-      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, true>), dim3(grid), dim3(TPB), lds, s, a,      \
+      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, true>), dim3(grid), dim3(JK_TEAM), lds, s, a,  \
                          a.pr.pp64, a.pr.pp32);                                               \
     else                                                                                      \
-      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, false>), dim3(grid), dim3(TPB), lds, s, a,     \
+      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, false>), dim3(grid), dim3(JK_TEAM), lds, s, a, \
                          a.pr.pp64, a.pr.pp32);                                               \
     return hipGetLastError();                                                                 \
   }
@@ -272,9 +284,8 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
 // Must match the layout at the top of gx::k_jk.
 size_t jk_lds_bytes(int la, int lb, int N, int mixed) {
   const int na = ncart(la), nb = ncart(lb);
-  const int teams = gx::JKShape<0>::TEAMS;  // the same for every L
   size_t b = mixed ? 0 : (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
-  return b + teams * 2 * (size_t)(na + nb) * N * sizeof(double);
+  return b + 2 * (size_t)(na + nb) * N * sizeof(double);
 }
 
 // Build the primitive-pair records of the current pair list on the device.
@@ -373,29 +384,124 @@ void for_each_bra_segment(const gx_ctx* c, int X, int Y, double cutoff, double f
   }
 }
 
-// Orders for the J/K workgroups: bra positions by decreasing estimated work
-// (primitive quartets that survive the cutoff, FP32 ones weighted 1/2).
+// Work items of the J/K kernels.  A bra's surviving kets form a virtual
+// index space (FP64 ranges, then FP32 ranges from the next multiple of 64,
+// the same layout the kernel rebuilds); waves take 64 consecutive indices at
+// a time.  Heavy bras are split into items over contiguous runs of those
+// chunks and, if one chunk alone is too heavy, over their primitive pairs
+// too: J and K are linear in the integrals, so each part is digested on its
+// own.  Costs are in primitive-quartet lane iterations; items are sorted by
+// decreasing cost and the persistent workgroups stride over them.
+struct JKItem {
+  int bpos, v0, v1, prims;
+};
+
 int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
-  if (cutoff == c->ord_cutoff && fp32 == c->ord_fp32 && mixed == c->ord_mixed && !c->d_order.empty())
+  if (cutoff == c->ord_cutoff && fp32 == c->ord_fp32 && mixed == c->ord_mixed && !c->d_items.empty())
     return GX_OK;
-  for (auto& p : c->d_order) dfree(&p);
-  c->d_order.assign(combo_count(), nullptr);
+  for (auto& p : c->d_items) dfree(&p);
+  for (auto& p : c->d_rng) dfree(&p);
+  c->d_items.assign(combo_count(), nullptr);
+  c->d_rng.assign(combo_count(), nullptr);
+  c->n_items.assign(combo_count(), 0);
+#ifndef GX_ITEM_OVERHEAD
+#define GX_ITEM_OVERHEAD 256.0
+#endif
+#ifndef GX_ITEMS_PER_WG
+#define GX_ITEMS_PER_WG 4.0
+#endif
+  constexpr double kItemOverhead = GX_ITEM_OVERHEAD;  // setup + flush of one item
+  constexpr double kQuartetCost = 8.0;                // digestion of one shell quartet per lane
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
-      const int nb = c->cls_start[X + 1] - c->cls_start[X];
+      const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
       const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
       if (nb == 0 || nk == 0) continue;
-      std::vector<double> cost(nb, 64.0);
-      for_each_bra_segment(c, X, Y, cutoff, fp32, mixed, [&](int i, int ip, int lo, int, int n64, int n32) {
-        const double k64 = (double)(c->pre_pnp[lo + n64] - c->pre_pnp[lo]);
-        const double k32 = (double)(c->pre_pnp[lo + n64 + n32] - c->pre_pnp[lo + n64]);
-        cost[i] += c->pnp[ip] * (k64 + 0.5 * k32) + 8.0 * (n64 + n32);
+      const int s0 = c->cls_seg[Y], nseg = c->cls_seg[Y + 1] - s0;
+      std::vector<int> rng((size_t)nb * 2 * nseg, 0);
+      for_each_bra_segment(c, X, Y, cutoff, fp32, mixed, [&](int i, int, int lo, int, int n64, int n32) {
+        int sg = 0;
+        while (c->seg_lo[s0 + sg] != lo) ++sg;
+        rng[(size_t)i * 2 * nseg + sg] = n64;
+        rng[(size_t)i * 2 * nseg + nseg + sg] = n32;
       });
-      std::vector<int> order(nb);
+      // per bra: virtual layout and per-chunk cost (widest ket of the chunk)
+      std::vector<std::vector<double>> chunk(nb);
+      double total = 0.0;
+      std::vector<int> vk;  // virtual index -> ket pair id (-1 = padding)
+      for (int i = 0; i < nb; ++i) {
+        const int* r = &rng[(size_t)i * 2 * nseg];
+        vk.clear();
+        for (int h = 0; h < 2; ++h) {
+          if (h == 1) vk.resize((vk.size() + GX_WAVE - 1) / GX_WAVE * GX_WAVE, -1);
+          for (int sg = 0; sg < nseg; ++sg) {
+            const int start = c->seg_lo[s0 + sg] + (h ? r[sg] : 0);
+            for (int k = 0; k < r[h * nseg + sg]; ++k) vk.push_back(start + k);
+          }
+        }
+        while (!vk.empty() && vk.back() < 0) vk.pop_back();
+        const double nbp = c->pnp[b0 + i];
+        for (size_t v0 = 0; v0 < vk.size(); v0 += GX_WAVE) {
+          int widest = 0;
+          for (size_t v = v0; v < std::min(vk.size(), v0 + GX_WAVE); ++v)
+            if (vk[v] >= 0) widest = std::max(widest, c->pnp[vk[v]]);
+          chunk[i].push_back(nbp * widest + kQuartetCost);
+          total += chunk[i].back();
+        }
+      }
+      // item size: about GX_ITEMS_PER_WG items per resident workgroup (four
+      // per CU assumed), each spread over the four waves of its workgroup
+      const double slots = (double)c->num_cu * 4.0;
+      const double target = std::max(total / (slots * 4.0 * GX_ITEMS_PER_WG), kItemOverhead);
+      std::vector<JKItem> items;
+      std::vector<double> icost;
+      for (int i = 0; i < nb; ++i) {
+        const auto& ch = chunk[i];
+        if (ch.empty()) continue;
+        const int nbp = c->pnp[b0 + i];
+        double widest = 0.0;
+        for (double x : ch) widest = std::max(widest, x);
+        // primitive split so that one chunk of each part fits the target
+        const int nsplit = std::max(1, std::min(nbp, (int)std::ceil(widest / target)));
+        for (int sp = 0; sp < nsplit; ++sp) {
+          const int p0 = (int)((long)nbp * sp / nsplit), p1 = (int)((long)nbp * (sp + 1) / nsplit);
+          const double frac = (double)(p1 - p0) / nbp;
+          // runs of chunks of about target x 4 waves
+          size_t k0 = 0;
+          double acc = 0.0;
+          for (size_t k = 0; k < ch.size(); ++k) {
+            acc += ch[k] * frac;
+            const bool last = k + 1 == ch.size();
+            if (last || (acc >= 4.0 * target && k + 1 - k0 >= 4)) {
+              JKItem it;
+              it.bpos = i;
+              it.v0 = (int)(k0 * GX_WAVE);
+              it.v1 = (int)((k + 1) * GX_WAVE);
+              it.prims = p0 | (p1 << 16);
+              items.push_back(it);
+              icost.push_back(acc / 4.0 + kItemOverhead);
+              k0 = k + 1;
+              acc = 0.0;
+            }
+          }
+        }
+      }
+      std::vector<int> order(items.size());
       std::iota(order.begin(), order.end(), 0);
-      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-      GX_TRY(upload(&c->d_order[combo], order, c->stream));
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return icost[a] > icost[b]; });
+      std::vector<int> flat(4 * items.size());
+      for (size_t k = 0; k < order.size(); ++k) {
+        const JKItem& it = items[order[k]];
+        flat[4 * k] = it.bpos;
+        flat[4 * k + 1] = it.v0;
+        flat[4 * k + 2] = it.v1;
+        flat[4 * k + 3] = it.prims;
+      }
+      c->n_items[combo] = (int)items.size();
+      GX_TRY(upload(&c->d_items[combo], flat, c->stream));
+      GX_TRY(upload(&c->d_rng[combo], rng, c->stream));
+      GX_HIP(hipStreamSynchronize(c->stream));  // before the host vectors go out of scope
     }
   c->ord_cutoff = cutoff;
   c->ord_fp32 = fp32;
@@ -814,17 +920,18 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
       const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
       if (nb == 0 || nk == 0) continue;
+      if (c->n_items[combo] == 0) continue;
       gx::JKArgs a;
       a.sh = c->shells();
       a.pr = c->pairs();
       a.bra0 = b0;
-      a.nbra = nb;
       a.ket0 = c->cls_start[Y];
       a.same = (X == Y);
       a.seg_lo = c->d_seg_lo + c->cls_seg[Y];
-      a.seg_hi = c->d_seg_hi + c->cls_seg[Y];
       a.nseg = c->cls_seg[Y + 1] - c->cls_seg[Y];
-      a.order = c->d_order[combo];
+      a.items = reinterpret_cast<const int4*>(c->d_items[combo]);
+      a.nitem = c->n_items[combo];
+      a.rng = c->d_rng[combo];
       a.master = c->d_boys;
       a.tab64 = c->d_tab64;
       a.tab32 = c->d_tab32;
@@ -832,15 +939,15 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       a.Jt = c->d_Jt;
       a.Kt = c->d_Kt;
       a.N = N;
-      a.cutoff = cutoff;
-      a.fp32_bound = fp32_bound;
       a.mixed = mixed;
       if (timing) {
         GX_HIP(next_events(combo));
         GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
       }
+      if (c->wg_per_cu.empty()) c->wg_per_cu.assign(2 * combo_count(), 0);
       GX_HIP(launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
-                       jk_lds_bytes(class_la(X), class_lb(X), N, mixed), c->stream, a));
+                       jk_lds_bytes(class_la(X), class_lb(X), N, mixed), c->stream, a,
+                       &c->wg_per_cu[2 * combo + mixed]));
       if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
     }
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, c->d_Jt,

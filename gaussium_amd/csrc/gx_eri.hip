@@ -153,6 +153,9 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 #ifndef GX_JKG32
 #define GX_JKG32 2
 #endif
+#ifndef GX_DYN_CHUNK
+#define GX_DYN_CHUNK 1
+#endif
 #ifndef GX_JKGM
 #define GX_JKGM 1  // FP64 group inside mixed builds (few FP64 quartets there)
 #endif
@@ -349,23 +352,20 @@ __global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int 
 #define JK_MAXSEG 16
 #define JK_TEAM 256  // threads working on one bra
 
-// Teams (bras in flight) per workgroup.  Two teams share one Boys table but
-// couple their bras through the workgroup barriers; measured on MI355X at
-// 220 functions one team is faster, so every class uses one.
 template <int L>
 struct JKShape {
-  static constexpr int TEAMS = 1;
-  static constexpr int THREADS = TEAMS * JK_TEAM;
+  static constexpr int THREADS = JK_TEAM;
 };
 
 struct JKArgs {
   ShellView sh;
   PairView pr;
-  int bra0, nbra, ket0, same;
-  const int* seg_lo;     // ket segments of the ket class: [seg_lo, seg_hi) global pair ids
-  const int* seg_hi;
+  int bra0, ket0, same;
+  const int* seg_lo;     // ket segments of the ket class: first global pair id of each
   int nseg;
-  const int* order;      // bra positions, largest estimated work first
+  const int4* items;     // work items: bra position, [v0, v1), p0 | p1 << 16; largest first
+  int nitem;
+  const int* rng;        // per bra position: surviving kets per segment, FP64 then FP32
   const double* master;  // Boys master table
   const double* tab64;   // per-L FP64 Taylor tables (BOYS_NT x BOYS_ROW64 each)
   const float* tab32;    // per-L FP32 Taylor tables
@@ -373,50 +373,31 @@ struct JKArgs {
   double* Jt;
   double* Kt;
   int N;
-  double cutoff;
-  double fp32_bound;
   int mixed;
 };
-
-// Number of pairs in [lo, hi) (Q descending) with QA * Q >= thr.
-__device__ __forceinline__ int count_at_least(const double* __restrict__ Q, int lo, int hi, double QA,
-                                              double thr) {
-  int a = lo, b = hi;  // first index in [lo, hi) failing the test
-  while (a < b) {
-    const int m = (a + b) >> 1;
-    if (QA * Q[m] >= thr) a = m + 1;
-    else b = m;
-  }
-  return a - lo;
-}
 
 // MIXED = false: pure FP64 build, the FP32 path is compiled out and the FP64
 // Taylor table is staged in LDS.  MIXED = true: most primitive quartets are
 // FP32 and both tables are read through the vector-memory path (L1), which
 // leaves the LDS to the P/K rows and raises occupancy.
 template <int LA, int LB, int LC, int LD, bool MIXED>
-__global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
-    JKArgs a, const PrimPair64* __restrict__ pp64, const PrimPair32* __restrict__ pp32) {
+__global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
+                                                const PrimPair32* __restrict__ pp32) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
-  constexpr int TEAMS = JKShape<C::L>::TEAMS;
   extern __shared__ double smem[];
-  __shared__ int r_start_t[TEAMS][2 * JK_MAXSEG], r_pre_t[TEAMS][2 * JK_MAXSEG + 1];
-  __shared__ int r_len_t[TEAMS][2 * JK_MAXSEG];
-  __shared__ double jab_t[TEAMS][NA * NB];
+  __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1], r_len[2 * JK_MAXSEG];
+  __shared__ double jab_s[NA * NB];
+  __shared__ int next_chunk;
   const int N = a.N;
-  const int team = threadIdx.x / JK_TEAM;
-  const int tid = threadIdx.x - team * JK_TEAM;  // thread index inside the team
-  // dynamic LDS: Boys FP64 (pure FP64 build) | per team: P rows | K rows
+  const int tid = threadIdx.x;
+  // dynamic LDS: Boys FP64 (pure FP64 build) | P rows | K rows
   const double* __restrict__ tab64 = !MIXED ? smem : a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
   const float* __restrict__ tab32 = a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
-  double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : 0) + team * 2 * (NA + NB) * N;
+  double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : 0);
   double* krow = prow + (NA + NB) * N;
-  int* r_start = r_start_t[team];
-  int* r_pre = r_pre_t[team];
-  int* r_len = r_len_t[team];
-  double* jab_s = jab_t[team];
-  const int nr = 2 * a.nseg;  // range r < nseg: FP64 part of segment r; else FP32 part of r - nseg
+  const int nseg = a.nseg;
+  const int nr = 2 * nseg;  // range r < nseg: FP64 part of segment r; else FP32 part of r - nseg
 
   if (!MIXED) boys_stage64(C::L, a.master, smem);
 
@@ -424,58 +405,50 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
   const int wave = tid / GX_WAVE;
   constexpr int nwave = JK_TEAM / GX_WAVE;
 
-  // Every team runs the same number of iterations so that the workgroup
-  // barriers line up; a team without a bra idles through its iteration.
-  for (int w0 = blockIdx.x * TEAMS; w0 < a.nbra; w0 += gridDim.x * TEAMS) {
-    const int w = w0 + team;
-    const bool has = w < a.nbra;
-    const int bpos = has ? a.order[w] : 0;
+  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
+    const int4 it = a.items[w];
+    const int bpos = it.x;
     const int ip = a.bra0 + bpos;
     const int A = a.pr.A[ip], B = a.pr.B[ip];
     const int ao = a.sh.bf[A], bo = a.sh.bf[B];
     const int rowB = (A == B) ? 0 : NA;
-    const double QA = a.pr.Q[ip];
     // wave-uniform: kept in scalar registers so that the bra loop reads go
     // through the scalar data path
-    const int nbp = __builtin_amdgcn_readfirstlane(a.pr.np[ip]);
-    const int bpoff = __builtin_amdgcn_readfirstlane(a.pr.poff[ip]);
+    const int p0 = __builtin_amdgcn_readfirstlane(it.w & 0xffff);
+    const int nbp = __builtin_amdgcn_readfirstlane((it.w >> 16) - (it.w & 0xffff));
+    const int bpoff = __builtin_amdgcn_readfirstlane(a.pr.poff[ip]) + p0;
     const PrimPair64* __restrict__ bra64 = pp64 + bpoff;
     const PrimPair32* __restrict__ bra32 = pp32 + bpoff;
     const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
                  ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
-    __syncthreads();  // the previous bra's flush is complete
-    for (int i = tid; has && i < (NA + NB) * N; i += JK_TEAM) {
+    __syncthreads();  // the previous item's flush is complete
+    for (int i = tid; i < (NA + NB) * N; i += JK_TEAM) {
       const int r = i / N, c = i - r * N;
       const int g = r < NA ? ao + r : bo + (r - NA);
       prow[i] = a.P[(long)g * N + c];
       krow[i] = 0.0;
     }
     if (tid < NA * NB) jab_s[tid] = 0.0;
-    if (tid < a.nseg) {
-      const int sg = tid;
-      const int lo = a.seg_lo[sg];
-      int hi = a.seg_hi[sg];
-      if (a.same && hi > ip + 1) hi = ip + 1;
-      if (hi < lo) hi = lo;
-      if (!has) hi = lo;
-      const int nsurv = a.cutoff > 0.0 ? count_at_least(a.pr.Q, lo, hi, QA, a.cutoff) : hi - lo;
-      const int n64 = a.mixed ? min(nsurv, count_at_least(a.pr.Q, lo, hi, QA, a.fp32_bound)) : nsurv;
+    if (tid == 0) next_chunk = nwave;
+    if (tid < nseg) {
+      const int* rr = a.rng + (long)bpos * nr;
+      const int n64 = rr[tid], n32 = rr[nseg + tid];
       // ranges: all FP64 parts first (r = sg), then all FP32 parts (r = nseg + sg),
-      // so that a bra has at most one precision boundary inside a chunk
-      r_start[sg] = lo;
-      r_start[a.nseg + sg] = lo + n64;
-      r_pre[sg + 1] = n64;  // lengths for now
-      r_pre[a.nseg + sg + 1] = nsurv - n64;
+      // so that a bra has at most one precision boundary
+      r_start[tid] = a.seg_lo[tid];
+      r_start[nseg + tid] = a.seg_lo[tid] + n64;
+      r_pre[tid + 1] = n64;  // lengths for now
+      r_pre[nseg + tid + 1] = n32;
     }
     __syncthreads();
     if (tid == 0) {
-      // r_pre[r]: virtual start of range r, r_pre[r + 1] - (pad) its end.  The
-      // FP32 ranges start at the next multiple of 64, so no chunk of 64 mixes
-      // precisions (the lanes of the gap stay idle).
+      // r_pre[r]: virtual start of range r.  The FP32 ranges start at the next
+      // multiple of 64, so no chunk of 64 mixes precisions (the lanes of the
+      // gap stay idle).
       int acc = 0;
       for (int r = 0; r < nr; ++r) {
         const int len = r_pre[r + 1];
-        if (r == a.nseg) acc = (acc + GX_WAVE - 1) & ~(GX_WAVE - 1);
+        if (r == nseg) acc = (acc + GX_WAVE - 1) & ~(GX_WAVE - 1);
         r_pre[r + 1] = acc + len;  // end of range r for now
         r_len[r] = len;
         acc += len;
@@ -484,13 +457,18 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
       r_pre[nr] = acc;
     }
     __syncthreads();
-    const int total = r_pre[nr];
+    const int total = min(r_pre[nr], it.z);
     const double fA = (A == B) ? 0.5 : 1.0;
     double jab[NA * NB];
 #pragma unroll
     for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
 
-    for (int v0 = wave * GX_WAVE; v0 < total; v0 += nwave * GX_WAVE) {
+#if GX_DYN_CHUNK
+    // waves take the item's chunks of 64 kets in turn from an LDS counter
+    for (int v0 = it.y + wave * GX_WAVE; v0 < total;) {
+#else
+    for (int v0 = it.y + wave * GX_WAVE; v0 < total; v0 += nwave * GX_WAVE) {
+#endif
       const int v = v0 + lane;
       int r = 0;
       if (v < total)
@@ -588,6 +566,11 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
           for (int id = 0; id < ND; ++id) atomicAdd(&krow[(rowB + ib) * N + dof + id], kbd[ib * ND + id]);
         }
       }
+#if GX_DYN_CHUNK
+      int ck = 0;
+      if (lane == 0) ck = atomicAdd(&next_chunk, 1);
+      v0 = it.y + __builtin_amdgcn_readfirstlane(ck) * GX_WAVE;
+#endif
     }
 
     // J_ab: wave reduction, then one LDS atomic per wave and element.
@@ -606,7 +589,7 @@ __global__ __launch_bounds__(JKShape<LA + LB + LC + LD>::THREADS) void k_jk(
         atomicAdd(&a.Jt[(long)(ao + ia) * N + bo + ib], 2.0 * x);
       }
     }
-    const int nrow = !has ? 0 : (A == B) ? NA : NA + NB;
+    const int nrow = (A == B) ? NA : NA + NB;
     for (int i = tid; i < nrow * N; i += JK_TEAM) {
       const double x = krow[i];
       if (x != 0.0) {
