@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, 'libgx.so')
 ROOT = os.path.dirname(HERE)
 
 # Angular momentum handled by the generated class kernels.
-ERI_LMAX = int(os.environ.get('GX_ERI_LMAX', '1'))
+ERI_LMAX = int(os.environ.get('GX_ERI_LMAX', '2'))
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',

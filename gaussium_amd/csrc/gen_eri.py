@@ -228,18 +228,201 @@ def emit_class(la, lb, lc, ld):
     return '\n'.join(s)
 
 
+# ---------------------------------------------------------------------------
+# Programs for the wave-cooperative evaluator.  Classes whose contracted
+# intermediates do not fit a lane's registers are evaluated by a whole wave
+# on one shell quartet: the VRR table [e0|f0]^(m) and the HRR table live in
+# LDS and the lanes share the entries of each recursion level.  The programs
+# list those entries level by level:
+#
+#   VRR entry (int4):  x = dest | kind << 16 | axis << 17 | k1 << 20 | k2 << 24
+#                      y = s0 | s1 << 16, z = s2 | s3 << 16, w = s4
+#     kind 0 (bra):  T[dest] = PA_i T[s0] + WP_i T[s1] + k1 o2p (T[s2] - rho_p T[s3])
+#     kind 1 (ket):  T[dest] = QC_i T[s0] + WQ_i T[s1] + k1 o2q (T[s2] - rho_q T[s3])
+#                              + k2 o2pq T[s4]
+#     slot 0 holds 0, slots 1..L+1 hold pref * F_m.
+#   accumulator t (t < NACC) adds T[acc[t]] (the [e0|f0]^(0) layout of
+#   EriClass::prim).
+#   HRR entry (int2):  x = dest | kind << 16 | axis << 17, y = s0 | s1 << 16
+#     kind 0: H[dest] = H[s0] + AB_i H[s1];  kind 1: H[dest] = H[s0] + CD_i H[s1]
+#     slots 0..NACC-1 hold the contracted accumulators.
+#   out[k] = H[outslot[k]] in the reference component order.
+# ---------------------------------------------------------------------------
+COOP_NACC = 90  # classes with more contracted intermediates than this are cooperative
+
+
+def coop_program(la, lb, lc, ld):
+    L = la + lb + lc + ld
+    zero = (0, 0, 0)
+    memo = {}
+    entries = []  # (level, packed int4)
+    nslot = [L + 2]
+
+    def node(e, f, m):
+        if e == zero and f == zero:
+            return 1 + m
+        key = (e, f, m)
+        if key in memo:
+            return memo[key]
+        s2 = s3 = s4 = 0
+        k1 = k2 = 0
+        if f == zero:
+            kind = 0
+            i = pick_axis(e)
+            em1 = sub(e, i)
+            s0, s1 = node(em1, f, m), node(em1, f, m + 1)
+            if e[i] >= 2:
+                k1 = e[i] - 1
+                s2, s3 = node(sub(e, i, 2), f, m), node(sub(e, i, 2), f, m + 1)
+        else:
+            kind = 1
+            i = pick_axis(f)
+            fm1 = sub(f, i)
+            s0, s1 = node(e, fm1, m), node(e, fm1, m + 1)
+            if f[i] >= 2:
+                k1 = f[i] - 1
+                s2, s3 = node(e, sub(f, i, 2), m), node(e, sub(f, i, 2), m + 1)
+            if e[i] >= 1:
+                k2 = e[i]
+                s4 = node(sub(e, i), fm1, m + 1)
+        dest = nslot[0]
+        nslot[0] += 1
+        assert dest < 65536 and k1 < 16 and k2 < 16
+        lev = sum(e) + sum(f)
+        entries.append((lev, (dest | kind << 16 | i << 17 | k1 << 20 | k2 << 24, s0 | s1 << 16,
+                              s2 | s3 << 16, s4)))
+        memo[key] = dest
+        return dest
+
+    layout = []
+    for le in range(la, la + lb + 1):
+        for e in cart(le):
+            for lf in range(lc, lc + ld + 1):
+                for f in cart(lf):
+                    layout.append((e, f))
+    acc = [node(e, f, 0) for (e, f) in layout]
+    entries.sort(key=lambda t: t[0])  # stable: creation order inside a level
+    vlev = [0] * (L + 2)
+    for lev, _ in entries:
+        vlev[lev + 1] += 1
+    for k in range(1, L + 2):
+        vlev[k] += vlev[k - 1]
+    vrr = [e for _, e in entries]
+
+    index = {ef: k for k, ef in enumerate(layout)}
+    hslot = [len(layout)]
+    hentries = []
+    bra_memo = {}
+
+    def bra(a, b, f):
+        key = (a, b, f)
+        if key in bra_memo:
+            return bra_memo[key]
+        if b == zero:
+            v = index[(a, f)]
+        else:
+            i = [k for k in range(3) if b[k] > 0][0]
+            s0, s1 = bra(add(a, i), sub(b, i), f), bra(a, sub(b, i), f)
+            v = hslot[0]
+            hslot[0] += 1
+            hentries.append((sum(b), (v | 0 << 16 | i << 17, s0 | s1 << 16)))
+        bra_memo[key] = v
+        return v
+
+    ket_memo = {}
+
+    def ket(a, b, c, d):
+        key = (a, b, c, d)
+        if key in ket_memo:
+            return ket_memo[key]
+        if d == zero:
+            v = bra(a, b, c)
+        else:
+            i = [k for k in range(3) if d[k] > 0][0]
+            s0, s1 = ket(a, b, add(c, i), sub(d, i)), ket(a, b, c, sub(d, i))
+            v = hslot[0]
+            hslot[0] += 1
+            hentries.append((lb + sum(d), (v | 1 << 16 | i << 17, s0 | s1 << 16)))
+        ket_memo[key] = v
+        return v
+
+    out = [ket(a, b, c, d) for a in cart(la) for b in cart(lb) for c in cart(lc) for d in cart(ld)]
+    assert hslot[0] < 65536
+    hentries.sort(key=lambda t: t[0])
+    nhl = lb + ld + 1
+    hlev = [0] * (nhl + 1)
+    for lev, _ in hentries:
+        hlev[lev] += 1  # levels 1..lb+ld -> hlev[lev] counts; prefix below
+    hl = [0]
+    for lev in range(1, nhl):
+        hl.append(hl[-1] + hlev[lev])
+    hrr = [e for _, e in hentries]
+    return dict(L=L, NACC=len(layout), NOUT=len(out), NT=nslot[0], NH=hslot[0], vrr=vrr, vlev=vlev,
+                acc=acc, hrr=hrr, hlev=hl, out=out)
+
+
+def emit_prog(la, lb, lc, ld):
+    g = coop_program(la, lb, lc, ld)
+    tag = '%d%d%d%d' % (la, lb, lc, ld)
+    na, nb, nc, nd = ncart(la), ncart(lb), ncart(lc), ncart(ld)
+
+    def arr(name, typ, vals, per_line=8):
+        items = ['{%s}' % ', '.join(str(x) for x in v) if isinstance(v, tuple) else str(v) for v in vals]
+        if not items:
+            items = ['{}' if typ != 'int' else '0']
+        lines = [', '.join(items[k:k + per_line]) for k in range(0, len(items), per_line)]
+        return '__device__ const %s %s_%s[] = {\n  %s};' % (typ, name, tag, ',\n  '.join(lines))
+
+    s = [arr('gx_vrr', 'int4', g['vrr'], 4), arr('gx_vlev', 'int', g['vlev']), arr('gx_acc', 'unsigned short', g['acc'], 16),
+         arr('gx_hrr', 'int2', g['hrr'], 6), arr('gx_hlev', 'int', g['hlev']),
+         arr('gx_out', 'unsigned short', g['out'], 16)]
+    s.append('template <> struct EriProg<%d, %d, %d, %d> {' % (la, lb, lc, ld))
+    s.append('  static constexpr int L = %d, NACC = %d, NOUT = %d, NT = %d, NH = %d;'
+             % (g['L'], g['NACC'], g['NOUT'], g['NT'], g['NH']))
+    s.append('  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;' % (na, nb, nc, nd))
+    s.append('  static constexpr int NVLEV = %d, NHLEV = %d;  // VRR levels 1..L, HRR levels 1..LB+LD'
+             % (g['L'], lb + ld))
+    s.append('  static __device__ __forceinline__ const int4* vrr() { return gx_vrr_%s; }' % tag)
+    s.append('  static __device__ __forceinline__ const int* vlev() { return gx_vlev_%s; }' % tag)
+    s.append('  static __device__ __forceinline__ const unsigned short* acc() { return gx_acc_%s; }' % tag)
+    s.append('  static __device__ __forceinline__ const int2* hrr() { return gx_hrr_%s; }' % tag)
+    s.append('  static __device__ __forceinline__ const int* hlev() { return gx_hlev_%s; }' % tag)
+    s.append('  static __device__ __forceinline__ const unsigned short* out() { return gx_out_%s; }' % tag)
+    s.append('};')
+    return '\n'.join(s)
+
+
+def is_coop(c):
+    la, lb, lc, ld = c
+    return (sum(ncart(e) for e in range(la, la + lb + 1)) * sum(ncart(f) for f in range(lc, lc + ld + 1))
+            > COOP_NACC)
+
+
 def main():
     out_path = sys.argv[1]
     lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
     parts = ['// Generated by gen_eri.py -- do not edit.',
              '#pragma once',
-             'template <int LA, int LB, int LC, int LD> struct EriClass;']
-    for c in classes(lmax):
+             'template <int LA, int LB, int LC, int LD> struct EriClass;',
+             'template <int LA, int LB, int LC, int LD> struct EriProg;']
+    lane = [c for c in classes(lmax) if not is_coop(c)]
+    coop = [c for c in classes(lmax) if is_coop(c)]
+    for c in lane:
         parts.append(emit_class(*c))
+    for c in classes(lmax):
+        parts.append(emit_prog(*c))
     parts.append('#define GX_ERI_LMAX %d' % lmax)
     parts.append('#define GX_FOR_EACH_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in classes(lmax)))
-    parts.append('#define GX_FOR_EACH_PAIRCLASS(X) ' + ' '.join(
-        'X(%d, %d)' % (a, b) for a in range(lmax + 1) for b in range(a + 1)))
+    parts.append('#define GX_FOR_EACH_LANE_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in lane))
+    parts.append('#define GX_FOR_EACH_COOP_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in coop))
+    pcs = [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+    parts.append('#define GX_FOR_EACH_PAIRCLASS(X) ' + ' '.join('X(%d, %d)' % c for c in pcs))
+    parts.append('#define GX_FOR_EACH_LANE_PAIRCLASS(X) ' + ' '.join(
+        'X(%d, %d)' % c for c in pcs if not is_coop(c + c)))
+    parts.append('#define GX_FOR_EACH_COOP_PAIRCLASS(X) ' + ' '.join(
+        'X(%d, %d)' % c for c in pcs if is_coop(c + c)))
+    parts.append('constexpr bool gx_class_is_coop(int la, int lb, int lc, int ld) {\n  return false' + ''.join(
+        ' ||\n         (la == %d && lb == %d && lc == %d && ld == %d)' % c for c in coop) + ';\n}')
     text = '\n\n'.join(parts) + '\n'
     with open(out_path, 'w') as f:
         f.write(text)

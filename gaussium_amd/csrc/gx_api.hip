@@ -15,6 +15,7 @@
 #include "_gen/eri_gen.h"
 // Single translation unit: the kernels are templates instantiated below.
 #include "gx_eri.hip"
+#include "gx_coop.hip"
 #include "gx_one.hip"
 
 namespace {
@@ -210,43 +211,87 @@ namespace {
 int combo_count() { return NCLS * (NCLS + 1) / 2; }
 
 // ---------------------------------------------------------------------------
-// Kernel dispatch by class.
-hipError_t launch_schwarz(int la, int lb, dim3 g, hipStream_t s, gx::ShellView sh, gx::PairView pr,
-                          int p0, int np, const double* boys, const double* scale, double* Q) {
-#define GX_CASE(A, B)                                                                   \
-  if (la == A && lb == B) {                                                             \
-    hipLaunchKernelGGL((gx::k_schwarz<A, B>), g, dim3(64), 0, s, sh, pr, p0, np, boys, scale, Q); \
-    return hipGetLastError();                                                           \
+// Kernel dispatch by class.  Per-lane classes (EriClass) run one quartet per
+// lane; cooperative classes (EriProg, gx_coop.hip) one quartet per wave with
+// the recursion tables in dynamic LDS.
+template <class G>
+size_t coop_bytes() {
+  return (size_t)gx::coop_lds<G>() * sizeof(double);
+}
+
+hipError_t launch_schwarz(int la, int lb, hipStream_t s, gx::ShellView sh, gx::PairView pr, int p0, int np,
+                          const double* boys, const double* tab64, const double* scale, double* Q) {
+#define GX_CASE(A, B)                                                                           \
+  if (la == A && lb == B) {                                                                     \
+    hipLaunchKernelGGL((gx::k_schwarz<A, B>), dim3((np + 63) / 64), dim3(64), 0, s, sh, pr, p0, np, \
+                       boys, scale, Q);                                                         \
+    return hipGetLastError();                                                                   \
   }
-  GX_FOR_EACH_PAIRCLASS(GX_CASE)
+  GX_FOR_EACH_LANE_PAIRCLASS(GX_CASE)
+#undef GX_CASE
+#define GX_CASE(A, B)                                                                          \
+  if (la == A && lb == B) {                                                                    \
+    const size_t lds = coop_bytes<EriProg<A, B, A, B>>();                                      \
+    hipError_t e = hipFuncSetAttribute((const void*)gx::k_schwarz_coop<A, B>,                   \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);  \
+    if (e != hipSuccess) return e;                                                             \
+    hipLaunchKernelGGL((gx::k_schwarz_coop<A, B>), dim3(np), dim3(GX_WAVE), lds, s, sh, pr, p0, \
+                       tab64, scale, Q);                                                       \
+    return hipGetLastError();                                                                  \
+  }
+  GX_FOR_EACH_COOP_PAIRCLASS(GX_CASE)
 #undef GX_CASE
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_dense(int la, int lb, int lc, int ld, dim3 g, hipStream_t s, gx::ShellView sh,
-                        gx::PairView pr, int b0, int nb, int k0, int nk, int same, const double* boys,
+hipError_t launch_dense(int la, int lb, int lc, int ld, hipStream_t s, gx::ShellView sh, gx::PairView pr,
+                        int b0, int nb, int k0, int nk, int same, const double* boys, const double* tab64,
                         const double* scale, double* out, long N) {
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
-    hipLaunchKernelGGL((gx::k_dense<A, B, C, D>), g, dim3(64), 0, s, sh, pr, b0, nb, k0, nk, same, \
-                       boys, scale, out, N);                                                  \
+    hipLaunchKernelGGL((gx::k_dense<A, B, C, D>), dim3((nk + 63) / 64, nb), dim3(64), 0, s, sh, pr, \
+                       b0, nb, k0, nk, same, boys, scale, out, N);                           \
     return hipGetLastError();                                                                 \
   }
-  GX_FOR_EACH_CLASS(GX_CASE)
+  GX_FOR_EACH_LANE_CLASS(GX_CASE)
+#undef GX_CASE
+#define GX_CASE(A, B, C, D)                                                                   \
+  if (la == A && lb == B && lc == C && ld == D) {                                             \
+    const size_t lds = coop_bytes<EriProg<A, B, C, D>>();                                     \
+    hipError_t e = hipFuncSetAttribute((const void*)gx::k_dense_coop<A, B, C, D>,              \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                            \
+    hipLaunchKernelGGL((gx::k_dense_coop<A, B, C, D>), dim3(nk, nb), dim3(GX_WAVE), lds, s, sh, pr, \
+                       b0, k0, same, tab64, scale, out, N);                                   \
+    return hipGetLastError();                                                                 \
+  }
+  GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_quartets(int la, int lb, int lc, int ld, dim3 g, hipStream_t s, gx::ShellView sh,
-                           gx::PairView pr, int n, const int* bra, const int* ket, const int* comp,
-                           const double* sc, const long* pos, const double* boys, double* out) {
+hipError_t launch_quartets(int la, int lb, int lc, int ld, hipStream_t s, gx::ShellView sh, gx::PairView pr,
+                           int n, const int* bra, const int* ket, const int* comp, const double* sc,
+                           const long* pos, const double* boys, const double* tab64, double* out) {
 #define GX_CASE(A, B, C, D)                                                                       \
   if (la == A && lb == B && lc == C && ld == D) {                                                 \
-    hipLaunchKernelGGL((gx::k_quartets<A, B, C, D>), g, dim3(64), 0, s, sh, pr, n, bra, ket, comp, \
-                       sc, pos, boys, out);                                                       \
+    hipLaunchKernelGGL((gx::k_quartets<A, B, C, D>), dim3((n + 63) / 64), dim3(64), 0, s, sh, pr, n, \
+                       bra, ket, comp, sc, pos, boys, out);                                       \
     return hipGetLastError();                                                                     \
   }
-  GX_FOR_EACH_CLASS(GX_CASE)
+  GX_FOR_EACH_LANE_CLASS(GX_CASE)
+#undef GX_CASE
+#define GX_CASE(A, B, C, D)                                                                       \
+  if (la == A && lb == B && lc == C && ld == D) {                                                 \
+    const size_t lds = coop_bytes<EriProg<A, B, C, D>>();                                         \
+    hipError_t e = hipFuncSetAttribute((const void*)gx::k_quartets_coop<A, B, C, D>,               \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
+    if (e != hipSuccess) return e;                                                                \
+    hipLaunchKernelGGL((gx::k_quartets_coop<A, B, C, D>), dim3(n), dim3(GX_WAVE), lds, s, sh, pr, bra, \
+                       ket, comp, sc, pos, tab64, out);                                           \
+    return hipGetLastError();                                                                     \
+  }
+  GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   return hipErrorInvalidValue;
 }
@@ -276,13 +321,42 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
                          a.pr.pp64, a.pr.pp32);                                               \
     return hipGetLastError();                                                                 \
   }
-  GX_FOR_EACH_CLASS(GX_CASE)
+  GX_FOR_EACH_LANE_CLASS(GX_CASE)
+#undef GX_CASE
+#define GX_CASE(A, B, C, D)                                                                   \
+  if (la == A && lb == B && lc == C && ld == D) {                                             \
+    const void* fn = (const void*)gx::k_jk_coop<A, B, C, D>;                                  \
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                            \
+    if (*per_cu <= 0) {                                                                       \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, GX_WAVE, lds);             \
+      if (e != hipSuccess) return e;                                                          \
+      if (*per_cu < 1) *per_cu = 1;                                                           \
+    }                                                                                         \
+    const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
+    hipLaunchKernelGGL((gx::k_jk_coop<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a);   \
+    return hipGetLastError();                                                                 \
+  }
+  GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   return hipErrorInvalidValue;
 }
 
-// Must match the layout at the top of gx::k_jk.
-size_t jk_lds_bytes(int la, int lb, int N, int mixed) {
+// Whether a class combination runs on the cooperative kernels (one quartet
+// per wave; always FP64).
+bool combo_is_coop(int X, int Y) { return gx_class_is_coop(class_la(X), class_lb(X), class_la(Y), class_lb(Y)); }
+
+// Must match the layout at the top of gx::k_jk (per-lane classes) or
+// gx::k_jk_coop (cooperative classes).
+size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed) {
+#define GX_CASE(A, B, C, D)                                                                     \
+  if (la == A && lb == B && lc == C && ld == D) {                                               \
+    using G = EriProg<A, B, C, D>;                                                              \
+    const int nblk = (G::NA + G::NB) * (G::NC + G::ND) + G::NA * G::NB + G::NC * G::ND;          \
+    return coop_bytes<G>() + 2 * (size_t)nblk * sizeof(double);                                 \
+  }
+  GX_FOR_EACH_COOP_CLASS(GX_CASE)
+#undef GX_CASE
   const int na = ncart(la), nb = ncart(lb);
   size_t b = mixed ? 0 : (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
   return b + 2 * (size_t)(na + nb) * N * sizeof(double);
@@ -342,8 +416,8 @@ int compute_schwarz(gx_ctx* c) {
   for (int k = 0; k < NCLS; ++k) {
     const int p0 = c->cls_start[k], n = c->cls_start[k + 1] - p0;
     if (n == 0) continue;
-    GX_HIP(launch_schwarz(class_la(k), class_lb(k), dim3((n + 63) / 64), c->stream, c->shells(),
-                          c->pairs(), p0, n, c->d_boys, c->d_scale, c->d_Q));
+    GX_HIP(launch_schwarz(class_la(k), class_lb(k), c->stream, c->shells(), c->pairs(), p0, n, c->d_boys,
+                          c->d_tab64, c->d_scale, c->d_Q));
   }
   c->Q.assign(c->npair, 0.0);
   GX_HIP(hipMemcpyAsync(c->Q.data(), c->d_Q, c->npair * sizeof(double), hipMemcpyDeviceToHost,
@@ -368,6 +442,7 @@ int host_count(const gx_ctx* c, int lo, int hi, double QA, double thr) {
 // For every class combo and bra: [lo, hi, n64, n32] per ket segment.
 template <class F>
 void for_each_bra_segment(const gx_ctx* c, int X, int Y, double cutoff, double fp32, int mixed, F&& f) {
+  if (combo_is_coop(X, Y)) mixed = 0;  // cooperative classes are always FP64
   const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
   for (int i = 0; i < nb; ++i) {
     const int ip = b0 + i;
@@ -426,7 +501,12 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
         rng[(size_t)i * 2 * nseg + sg] = n64;
         rng[(size_t)i * 2 * nseg + nseg + sg] = n32;
       });
-      // per bra: virtual layout and per-chunk cost (widest ket of the chunk)
+      // per bra: virtual layout and per-chunk cost (widest ket of the chunk).
+      // Per-lane classes: chunks of 64 kets, four waves per item; cooperative
+      // classes: one ket per chunk, one wave per item.
+      const bool coop = combo_is_coop(X, Y);
+      const int csz = coop ? 1 : GX_WAVE;
+      const double waves = coop ? 1.0 : 4.0;
       std::vector<std::vector<double>> chunk(nb);
       double total = 0.0;
       std::vector<int> vk;  // virtual index -> ket pair id (-1 = padding)
@@ -442,18 +522,18 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
         }
         while (!vk.empty() && vk.back() < 0) vk.pop_back();
         const double nbp = c->pnp[b0 + i];
-        for (size_t v0 = 0; v0 < vk.size(); v0 += GX_WAVE) {
+        for (size_t v0 = 0; v0 < vk.size(); v0 += csz) {
           int widest = 0;
-          for (size_t v = v0; v < std::min(vk.size(), v0 + GX_WAVE); ++v)
+          for (size_t v = v0; v < std::min(vk.size(), v0 + csz); ++v)
             if (vk[v] >= 0) widest = std::max(widest, c->pnp[vk[v]]);
           chunk[i].push_back(nbp * widest + kQuartetCost);
           total += chunk[i].back();
         }
       }
       // item size: about GX_ITEMS_PER_WG items per resident workgroup (four
-      // per CU assumed), each spread over the four waves of its workgroup
+      // per CU assumed), each spread over the waves of its workgroup
       const double slots = (double)c->num_cu * 4.0;
-      const double target = std::max(total / (slots * 4.0 * GX_ITEMS_PER_WG), kItemOverhead);
+      const double target = std::max(total / (slots * waves * GX_ITEMS_PER_WG), kItemOverhead);
       std::vector<JKItem> items;
       std::vector<double> icost;
       for (int i = 0; i < nb; ++i) {
@@ -467,20 +547,20 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
         for (int sp = 0; sp < nsplit; ++sp) {
           const int p0 = (int)((long)nbp * sp / nsplit), p1 = (int)((long)nbp * (sp + 1) / nsplit);
           const double frac = (double)(p1 - p0) / nbp;
-          // runs of chunks of about target x 4 waves
+          // runs of chunks of about target x waves
           size_t k0 = 0;
           double acc = 0.0;
           for (size_t k = 0; k < ch.size(); ++k) {
             acc += ch[k] * frac;
             const bool last = k + 1 == ch.size();
-            if (last || (acc >= 4.0 * target && k + 1 - k0 >= 4)) {
+            if (last || (acc >= waves * target && k + 1 - k0 >= (size_t)waves)) {
               JKItem it;
               it.bpos = i;
-              it.v0 = (int)(k0 * GX_WAVE);
-              it.v1 = (int)((k + 1) * GX_WAVE);
+              it.v0 = (int)(k0 * csz);
+              it.v1 = (int)((k + 1) * csz);
               it.prims = p0 | (p1 << 16);
               items.push_back(it);
-              icost.push_back(acc / 4.0 + kItemOverhead);
+              icost.push_back(acc / waves + kItemOverhead);
               k0 = k + 1;
               acc = 0.0;
             }
@@ -784,9 +864,8 @@ int gx_eri_dense(gx_ctx* c, double* out) {
       const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
       const int k0 = c->cls_start[Y], nk = c->cls_start[Y + 1] - k0;
       if (nb == 0 || nk == 0) continue;
-      GX_HIP(launch_dense(class_la(X), class_lb(X), class_la(Y), class_lb(Y), dim3((nk + 63) / 64, nb),
-                          c->stream, c->shells(), c->pairs(), b0, nb, k0, nk, X == Y, c->d_boys,
-                          c->d_scale, out, N));
+      GX_HIP(launch_dense(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->stream, c->shells(),
+                          c->pairs(), b0, nb, k0, nk, X == Y, c->d_boys, c->d_tab64, c->d_scale, out, N));
     }
   GX_HIP(hipStreamSynchronize(c->stream));
   return GX_OK;
@@ -860,9 +939,9 @@ int gx_eri_quartets(gx_ctx* c, int64_t n, const int32_t* ijkl, double* out) {
       keep_d.push_back(ds);
       keep_l.push_back(dp);
       if (rc != GX_OK) break;
-      hipError_t e = launch_quartets(class_la(X), class_lb(X), class_la(Y), class_lb(Y),
-                                     dim3((m + 63) / 64), c->stream, c->shells(), c->pairs(), m, db,
-                                     dk, dc, ds, dp, c->d_boys, d_out);
+      hipError_t e = launch_quartets(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->stream,
+                                     c->shells(), c->pairs(), m, db, dk, dc, ds, dp, c->d_boys, c->d_tab64,
+                                     d_out);
       if (e != hipSuccess) rc = fail(GX_ERR_HIP, std::string("k_quartets: ") + hipGetErrorString(e));
     }
   if (rc == GX_OK) {
@@ -888,12 +967,14 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   const int N = c->nbf;
   const long nn = (long)N * N;
   const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
-  for (int X = 0; X < NCLS; ++X) {
-    const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), N, mixed);
-    if (c->cls_start[X + 1] > c->cls_start[X] && lds > (size_t)c->max_lds)
-      return fail(GX_ERR_ARG, "gx_jk: basis too large for the LDS row cache (nbf=" + std::to_string(N) +
-                                  ", needs " + std::to_string(lds) + " bytes)");
-  }
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      if (c->cls_start[X + 1] == c->cls_start[X] || c->cls_start[Y + 1] == c->cls_start[Y]) continue;
+      const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), N, mixed);
+      if (lds > (size_t)c->max_lds)
+        return fail(GX_ERR_ARG, "gx_jk: basis too large for the LDS row cache (nbf=" + std::to_string(N) +
+                                    ", needs " + std::to_string(lds) + " bytes)");
+    }
   const unsigned g = (unsigned)((nn + 255) / 256);
   hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, P, c->d_Ps);
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Jt, 0.0);
@@ -946,7 +1027,7 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       }
       if (c->wg_per_cu.empty()) c->wg_per_cu.assign(2 * combo_count(), 0);
       GX_HIP(launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
-                       jk_lds_bytes(class_la(X), class_lb(X), N, mixed), c->stream, a,
+                       jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), N, mixed), c->stream, a,
                        &c->wg_per_cu[2 * combo + mixed]));
       if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
     }

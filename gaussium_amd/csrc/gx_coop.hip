@@ -1,0 +1,292 @@
+// Wave-cooperative ERI classes for gfx950.
+//
+// Classes whose contracted [e0|f0] intermediates do not fit in one lane's
+// registers -- (dp|pp), (dp|ds), (dp|dp) and the dd classes -- are evaluated
+// by a whole wave on one shell quartet.  The VRR table [e0|f0]^(m), the
+// contracted accumulators and the HRR table sit in LDS, and the 64 lanes
+// share the entries of each recursion level.  The recursions are the same as
+// the per-lane class code (obara_saika_scheme.py:131-146 of the reference, in
+// the VRR + HRR arrangement of head_gordon_pople.py:111-181); gen_eri.py
+// writes them out as level-ordered programs (EriProg<...>).
+//
+// One workgroup is one wave, so __syncthreads() is the level barrier.
+// LDS per wave, in doubles: max(NACC + NT, NH) for the quartet; the J/K
+// kernel adds the density blocks and the partial J/K blocks of one quartet.
+
+namespace gx {
+
+template <class G>
+constexpr int coop_lds() {
+  return G::NACC + G::NT > G::NH ? G::NACC + G::NT : G::NH;
+}
+
+// Contracted shell quartet into H: on return H[G::out()[k]] is component k
+// (reference order), unscaled.  Every lane of the wave calls it.
+template <class G>
+__device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra, int nb,
+                                             const PrimPair64* __restrict__ ket, int nk, double ABx,
+                                             double ABy, double ABz, double CDx, double CDy, double CDz,
+                                             const double* __restrict__ tab, double* __restrict__ H) {
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  double* __restrict__ T = H + G::NACC;
+  const int4* __restrict__ vp = G::vrr();
+  const int* __restrict__ vl = G::vlev();
+  const unsigned short* __restrict__ ap = G::acc();
+  __syncthreads();  // the previous quartet's readers of H are done
+  for (int t = lane; t < G::NACC; t += GX_WAVE) H[t] = 0.0;
+  if (lane == 0) T[0] = 0.0;
+  for (int ik = 0; ik < nk; ++ik) {
+    const PrimPair64 k = ket[ik];
+    for (int ib = 0; ib < nb; ++ib) {
+      const PrimPair64 b = bra[ib];
+      const double rs = gx_rsqrt(b.p + k.p);
+      const double is = rs * rs;
+      const double rho = b.p * k.p * is;
+      const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+      const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = b.Kip * k.Kip * rs;
+      double F[G::L + 1];
+      boys64_nb<G::L>(Tb, pref, tab, F);
+      const double rho_p = rho * b.ip, rho_q = rho * k.ip;
+      const double WPx = -rho_p * PQx, WPy = -rho_p * PQy, WPz = -rho_p * PQz;
+      const double WQx = rho_q * PQx, WQy = rho_q * PQy, WQz = rho_q * PQz;
+      const double o2p = 0.5 * b.ip, o2q = 0.5 * k.ip, o2pq = 0.5 * is;
+#pragma unroll
+      for (int m = 0; m <= G::L; ++m)
+        if (lane == m) T[1 + m] = F[m];
+      __syncthreads();
+      for (int lev = 1; lev <= G::L; ++lev) {
+        const int e1 = vl[lev + 1];
+        for (int t = vl[lev] + lane; t < e1; t += GX_WAVE) {
+          const int4 en = vp[t];
+          const unsigned x = (unsigned)en.x, y = (unsigned)en.y, z = (unsigned)en.z;
+          const bool ket_step = (x >> 16) & 1u;
+          const unsigned ax = (x >> 17) & 3u;
+          const double k1 = (double)((x >> 20) & 15u), k2 = (double)((x >> 24) & 15u);
+          const double pa = ax == 0 ? b.PAx : (ax == 1 ? b.PAy : b.PAz);
+          const double qc = ax == 0 ? k.PAx : (ax == 1 ? k.PAy : k.PAz);
+          const double wp = ax == 0 ? WPx : (ax == 1 ? WPy : WPz);
+          const double wq = ax == 0 ? WQx : (ax == 1 ? WQy : WQz);
+          const double cA = ket_step ? qc : pa, cB = ket_step ? wq : wp;
+          const double o = ket_step ? o2q : o2p, r = ket_step ? rho_q : rho_p;
+          double v = cA * T[y & 0xffffu];
+          v = fma(cB, T[y >> 16], v);
+          v = fma(k1 * o, T[z & 0xffffu] - r * T[z >> 16], v);
+          v = fma(k2 * o2pq, T[(unsigned)en.w], v);
+          T[x & 0xffffu] = v;
+        }
+        __syncthreads();
+      }
+      for (int t = lane; t < G::NACC; t += GX_WAVE) H[t] += T[ap[t]];
+      __syncthreads();  // T is rewritten by the next primitive quartet
+    }
+  }
+  const int2* __restrict__ hp = G::hrr();
+  const int* __restrict__ hl = G::hlev();
+  for (int lev = 0; lev < G::NHLEV; ++lev) {
+    const int e1 = hl[lev + 1];
+    for (int t = hl[lev] + lane; t < e1; t += GX_WAVE) {
+      const int2 en = hp[t];
+      const unsigned x = (unsigned)en.x, y = (unsigned)en.y;
+      const bool ket_step = (x >> 16) & 1u;
+      const unsigned ax = (x >> 17) & 3u;
+      const double ab = ax == 0 ? ABx : (ax == 1 ? ABy : ABz);
+      const double cd = ax == 0 ? CDx : (ax == 1 ? CDy : CDz);
+      H[x & 0xffffu] = fma(ket_step ? cd : ab, H[y >> 16], H[y & 0xffffu]);
+    }
+    __syncthreads();
+  }
+}
+
+template <class G>
+__device__ __forceinline__ void coop_quartet_pairs(const ShellView& sh, const PairView& pr, int ip, int jp,
+                                                   const double* __restrict__ tab, double* __restrict__ H) {
+  const int A = pr.A[ip], B = pr.B[ip], Cs = pr.A[jp], Ds = pr.B[jp];
+  coop_quartet<G>(pr.pp64 + pr.poff[ip], pr.np[ip], pr.pp64 + pr.poff[jp], pr.np[jp],
+                  sh.ctr[3 * A] - sh.ctr[3 * B], sh.ctr[3 * A + 1] - sh.ctr[3 * B + 1],
+                  sh.ctr[3 * A + 2] - sh.ctr[3 * B + 2], sh.ctr[3 * Cs] - sh.ctr[3 * Ds],
+                  sh.ctr[3 * Cs + 1] - sh.ctr[3 * Ds + 1], sh.ctr[3 * Cs + 2] - sh.ctr[3 * Ds + 2], tab, H);
+}
+
+// Schwarz bounds of one pair class, one wave per pair (as k_schwarz).
+template <int LA, int LB>
+__global__ __launch_bounds__(GX_WAVE) void k_schwarz_coop(ShellView sh, PairView pr, int p0,
+                                                          const double* __restrict__ tab,
+                                                          const double* __restrict__ scale,
+                                                          double* __restrict__ Q) {
+  using G = EriProg<LA, LB, LA, LB>;
+  extern __shared__ double H[];
+  const int ip = p0 + blockIdx.x;
+  coop_quartet_pairs<G>(sh, pr, ip, ip, tab + G::L * BOYS_NT * BOYS_ROW64, H);
+  if (threadIdx.x == 0) {
+    const int ao = sh.bf[pr.A[ip]], bo = sh.bf[pr.B[ip]];
+    const unsigned short* __restrict__ out = G::out();
+    double m = 0.0;
+    for (int ia = 0; ia < G::NA; ++ia)
+      for (int ib = 0; ib < G::NB; ++ib) {
+        const double s = scale[ao + ia] * scale[bo + ib];
+        const double d = fabs(H[out[((ia * G::NB + ib) * G::NA + ia) * G::NB + ib]]) * s * s;
+        m = d > m ? d : m;
+      }
+    Q[ip] = sqrt(m);
+  }
+}
+
+// Dense ERI tensor, one wave per shell quartet (as k_dense).
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(GX_WAVE) void k_dense_coop(ShellView sh, PairView pr, int bra0, int ket0,
+                                                        int same, const double* __restrict__ tab,
+                                                        const double* __restrict__ scale,
+                                                        double* __restrict__ out, long N) {
+  using G = EriProg<LA, LB, LC, LD>;
+  extern __shared__ double H[];
+  const int bi = blockIdx.y, kj = blockIdx.x;
+  if (same && kj > bi) return;
+  const int ip = bra0 + bi, jp = ket0 + kj;
+  coop_quartet_pairs<G>(sh, pr, ip, jp, tab + G::L * BOYS_NT * BOYS_ROW64, H);
+  const long ao = sh.bf[pr.A[ip]], bo = sh.bf[pr.B[ip]];
+  const long co = sh.bf[pr.A[jp]], dof = sh.bf[pr.B[jp]];
+  const unsigned short* __restrict__ os = G::out();
+  for (int k = threadIdx.x; k < G::NOUT; k += GX_WAVE) {
+    const int id = k % G::ND, ic = (k / G::ND) % G::NC, ib = (k / (G::ND * G::NC)) % G::NB;
+    const int ia = k / (G::ND * G::NC * G::NB);
+    const long i = ao + ia, j = bo + ib, kk = co + ic, l = dof + id;
+    const double x = H[os[k]] * scale[i] * scale[j] * scale[kk] * scale[l];
+    out[((i * N + j) * N + kk) * N + l] = x;
+    out[((j * N + i) * N + kk) * N + l] = x;
+    out[((i * N + j) * N + l) * N + kk] = x;
+    out[((j * N + i) * N + l) * N + kk] = x;
+    out[((kk * N + l) * N + i) * N + j] = x;
+    out[((l * N + kk) * N + i) * N + j] = x;
+    out[((kk * N + l) * N + j) * N + i] = x;
+    out[((l * N + kk) * N + j) * N + i] = x;
+  }
+}
+
+// Quartet list, one wave per task (as k_quartets).
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(GX_WAVE) void k_quartets_coop(ShellView sh, PairView pr,
+                                                           const int* __restrict__ bra,
+                                                           const int* __restrict__ ket,
+                                                           const int* __restrict__ comp,
+                                                           const double* __restrict__ sc,
+                                                           const long* __restrict__ pos,
+                                                           const double* __restrict__ tab,
+                                                           double* __restrict__ out) {
+  using G = EriProg<LA, LB, LC, LD>;
+  extern __shared__ double H[];
+  const int t = blockIdx.x;
+  coop_quartet_pairs<G>(sh, pr, bra[t], ket[t], tab + G::L * BOYS_NT * BOYS_ROW64, H);
+  if (threadIdx.x == 0) out[pos[t]] = H[G::out()[comp[t]]] * sc[t];
+}
+
+// Fused direct J/K for a cooperative class: persistent one-wave workgroups
+// over work items (bra, range of surviving kets); each ket is one quartet.
+// Digestion as in k_jk (same Jt/Kt accumulators and degeneracy factors), via
+// per-quartet LDS blocks flushed with device atomics.  Always FP64.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
+  using G = EriProg<LA, LB, LC, LD>;
+  constexpr int NA = G::NA, NB = G::NB, NC = G::NC, ND = G::ND;
+  constexpr int SAB = NA * NB, SCD = NC * ND, SAC = NA * NC, SAD = NA * ND, SBC = NB * NC, SBD = NB * ND;
+  constexpr int NBLK = SAB + SCD + SAC + SAD + SBC + SBD;
+  extern __shared__ double smem[];
+  double* __restrict__ H = smem;
+  double* __restrict__ pblk = smem + coop_lds<G>();  // P_ab P_cd P_ac P_ad P_bc P_bd
+  double* __restrict__ jblk = pblk + NBLK;           // J_ab J_cd K_ac K_ad K_bc K_bd partials
+  __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
+  const int lane = threadIdx.x;
+  const int N = a.N;
+  const int nseg = a.nseg;
+  const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
+  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
+    const int4 it = a.items[w];
+    const int ip = a.bra0 + it.x;
+    const int A = a.pr.A[ip], B = a.pr.B[ip];
+    const int ao = a.sh.bf[A], bo = a.sh.bf[B];
+    const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
+    const PrimPair64* __restrict__ bra = a.pr.pp64 + a.pr.poff[ip] + p0;
+    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
+                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
+    const double fA = (A == B) ? 0.5 : 1.0;
+    __syncthreads();  // the previous item's readers of r_* are done
+    if (lane < nseg) {
+      r_start[lane] = a.seg_lo[lane];
+      r_pre[lane + 1] = a.rng[(long)it.x * 2 * nseg + lane];  // FP64 counts; cooperative classes have no FP32 part
+    }
+    __syncthreads();
+    if (lane == 0) {
+      r_pre[0] = 0;
+      for (int r = 0; r < nseg; ++r) r_pre[r + 1] += r_pre[r];
+    }
+    __syncthreads();
+    const int total = min(r_pre[nseg], it.z);
+    for (int v = it.y; v < total; ++v) {
+      int r = 0;
+      while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
+      const int jp = r_start[r] + (v - r_pre[r]);
+      const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
+      const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
+      coop_quartet<G>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
+                      a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
+                      a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], tab, H);
+      double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
+      if (jp == ip) f *= 0.5;
+      // density blocks of this quartet; partials cleared
+      for (int t = lane; t < NBLK; t += GX_WAVE) {
+        int r0, c0, s;
+        int u = t;
+        if (u < SAB) { r0 = ao; c0 = bo; s = NB; }
+        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; }
+        else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; s = NC; }
+        else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
+        else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
+        else { u -= SBC; r0 = bo; c0 = dof; s = ND; }
+        pblk[t] = a.P[(long)(r0 + u / s) * N + c0 + u % s];
+        jblk[t] = 0.0;
+      }
+      __syncthreads();
+      const double* pab = pblk;
+      const double* pcd = pab + SAB;
+      const double* pac = pcd + SCD;
+      const double* pad = pac + SAC;
+      const double* pbc = pad + SAD;
+      const double* pbd = pbc + SBC;
+      double* jab = jblk;
+      double* jcd = jab + SAB;
+      double* kac = jcd + SCD;
+      double* kad = kac + SAC;
+      double* kbc = kad + SAD;
+      double* kbd = kbc + SBC;
+      const unsigned short* __restrict__ os = G::out();
+      for (int k = lane; k < G::NOUT; k += GX_WAVE) {
+        const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
+        const double x = f * H[os[k]];
+        atomicAdd(&jab[ia * NB + ib], x * pcd[ic * ND + id]);
+        atomicAdd(&jcd[ic * ND + id], x * pab[ia * NB + ib]);
+        atomicAdd(&kac[ia * NC + ic], x * pbd[ib * ND + id]);
+        atomicAdd(&kad[ia * ND + id], x * pbc[ib * NC + ic]);
+        atomicAdd(&kbc[ib * NC + ic], x * pad[ia * ND + id]);
+        atomicAdd(&kbd[ib * ND + id], x * pac[ia * NC + ic]);
+      }
+      __syncthreads();
+      for (int t = lane; t < NBLK; t += GX_WAVE) {
+        const double x = jblk[t];
+        if (x == 0.0) continue;
+        int r0, c0, s;
+        int u = t;
+        double* dst = a.Kt;
+        double m = 1.0;
+        if (u < SAB) { r0 = ao; c0 = bo; s = NB; dst = a.Jt; m = 2.0; }
+        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; dst = a.Jt; m = 2.0; }
+        else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; s = NC; }
+        else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
+        else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
+        else { u -= SBC; r0 = bo; c0 = dof; s = ND; }
+        atomicAdd(&dst[(long)(r0 + u / s) * N + c0 + u % s], m * x);
+      }
+    }
+  }
+}
+
+}  // namespace gx

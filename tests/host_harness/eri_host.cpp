@@ -8,6 +8,12 @@
 
 #define __device__
 #define __forceinline__ inline
+struct int2 {
+  int x, y;
+};
+struct int4 {
+  int x, y, z, w;
+};
 #include "eri_gen.h"
 
 static void boys_host(int L, double T, double scale, double* F) {
@@ -76,9 +82,112 @@ static void quartet(const Shell& A, const Shell& B, const Shell& C, const Shell&
   for (int t = 0; t < K::NOUT; ++t) out[t] = o[t];
 }
 
-extern "C" int host_quartet(const int* l, const double* ctr, const int* n, const double* a,
-                            const double* w, double* out) {
-  Shell s[4];
+// The cooperative program of the class (EriProg), run entry by entry in level
+// order as the lanes of a wave run it.  Every source slot must have been
+// written before it is read (checked).
+template <int LA, int LB, int LC, int LD>
+static int quartet_prog(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
+  using G = EriProg<LA, LB, LC, LD>;
+  static double T[G::NT], H[G::NH + G::NACC + G::NT];
+  static bool ok[G::NT], hok[G::NH];
+  for (int t = 0; t < G::NACC; ++t) H[t] = 0.0;
+  const double two_pi_52 = 34.98683665524972;
+  for (int i = 0; i < A.n; ++i)
+    for (int j = 0; j < B.n; ++j) {
+      const double a = A.a[i], b = B.a[j], p = a + b;
+      double P[3], ab2 = 0;
+      for (int x = 0; x < 3; ++x) {
+        P[x] = (a * A.c[x] + b * B.c[x]) / p;
+        ab2 += (A.c[x] - B.c[x]) * (A.c[x] - B.c[x]);
+      }
+      const double Kab = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2);
+      for (int k = 0; k < C.n; ++k)
+        for (int l = 0; l < D.n; ++l) {
+          const double c = C.a[k], d = D.a[l], q = c + d;
+          double Q[3], cd2 = 0, pq2 = 0;
+          for (int x = 0; x < 3; ++x) {
+            Q[x] = (c * C.c[x] + d * D.c[x]) / q;
+            cd2 += (C.c[x] - D.c[x]) * (C.c[x] - D.c[x]);
+            pq2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+          }
+          const double Kcd = C.w[k] * D.w[l] * std::exp(-c * d / q * cd2);
+          const double rho = p * q / (p + q);
+          const double pref = two_pi_52 / (p * q * std::sqrt(p + q)) * Kab * Kcd;
+          double F[G::L + 1];
+          boys_host(G::L, rho * pq2, pref, F);
+          const double rp = rho / p, rq = rho / q;
+          const double PA[3] = {P[0] - A.c[0], P[1] - A.c[1], P[2] - A.c[2]};
+          const double QC[3] = {Q[0] - C.c[0], Q[1] - C.c[1], Q[2] - C.c[2]};
+          double WP[3], WQ[3];
+          for (int x = 0; x < 3; ++x) {
+            WP[x] = -rp * (P[x] - Q[x]);
+            WQ[x] = rq * (P[x] - Q[x]);
+          }
+          for (int t = 0; t < G::NT; ++t) ok[t] = false;
+          T[0] = 0.0;
+          ok[0] = true;
+          for (int m = 0; m <= G::L; ++m) {
+            T[1 + m] = F[m];
+            ok[1 + m] = true;
+          }
+          const int4* vp = G::vrr();
+          const int* vl = G::vlev();
+          for (int lev = 1; lev <= G::L; ++lev) {
+            // entries of one level read only earlier levels
+            for (int t = vl[lev]; t < vl[lev + 1]; ++t) {
+              const unsigned x = (unsigned)vp[t].x, y = (unsigned)vp[t].y, z = (unsigned)vp[t].z;
+              const unsigned s[5] = {y & 0xffffu, y >> 16, z & 0xffffu, z >> 16, (unsigned)vp[t].w};
+              for (unsigned v : s)
+                if (v >= (unsigned)G::NT || !ok[v]) return -2;
+            }
+            for (int t = vl[lev]; t < vl[lev + 1]; ++t) {
+              const unsigned x = (unsigned)vp[t].x, y = (unsigned)vp[t].y, z = (unsigned)vp[t].z;
+              const bool ket = (x >> 16) & 1u;
+              const unsigned ax = (x >> 17) & 3u;
+              const double k1 = (double)((x >> 20) & 15u), k2 = (double)((x >> 24) & 15u);
+              const double cA = ket ? QC[ax] : PA[ax], cB = ket ? WQ[ax] : WP[ax];
+              const double o = ket ? 0.5 / q : 0.5 / p, r = ket ? rq : rp;
+              const unsigned dst = x & 0xffffu;
+              if (dst >= (unsigned)G::NT || ok[dst]) return -3;
+              T[dst] = cA * T[y & 0xffffu] + cB * T[y >> 16] + k1 * o * (T[z & 0xffffu] - r * T[z >> 16]) +
+                       k2 * (0.5 / (p + q)) * T[vp[t].w];
+            }
+            for (int t = vl[lev]; t < vl[lev + 1]; ++t) ok[(unsigned)vp[t].x & 0xffffu] = true;
+          }
+          for (int t = 0; t < G::NACC; ++t) {
+            if (!ok[G::acc()[t]]) return -4;
+            H[t] += T[G::acc()[t]];
+          }
+        }
+    }
+  for (int t = 0; t < G::NH; ++t) hok[t] = t < G::NACC;
+  const double AB[3] = {A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2]};
+  const double CD[3] = {C.c[0] - D.c[0], C.c[1] - D.c[1], C.c[2] - D.c[2]};
+  const int2* hp = G::hrr();
+  const int* hl = G::hlev();
+  for (int lev = 0; lev < G::NHLEV; ++lev) {
+    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
+      const unsigned y = (unsigned)hp[t].y;
+      if ((y & 0xffffu) >= (unsigned)G::NH || (y >> 16) >= (unsigned)G::NH || !hok[y & 0xffffu] || !hok[y >> 16])
+        return -5;
+    }
+    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
+      const unsigned x = (unsigned)hp[t].x, y = (unsigned)hp[t].y;
+      const unsigned ax = (x >> 17) & 3u;
+      const double X = ((x >> 16) & 1u) ? CD[ax] : AB[ax];
+      if ((x & 0xffffu) >= (unsigned)G::NH || hok[x & 0xffffu]) return -6;
+      H[x & 0xffffu] = H[y & 0xffffu] + X * H[y >> 16];
+    }
+    for (int t = hl[lev]; t < hl[lev + 1]; ++t) hok[(unsigned)hp[t].x & 0xffffu] = true;
+  }
+  for (int t = 0; t < G::NOUT; ++t) {
+    if (!hok[G::out()[t]]) return -7;
+    out[t] = H[G::out()[t]];
+  }
+  return 0;
+}
+
+static void unpack(const int* n, const double* ctr, const double* a, const double* w, Shell* s) {
   int off = 0;
   for (int k = 0; k < 4; ++k) {
     s[k].c[0] = ctr[3 * k];
@@ -89,12 +198,36 @@ extern "C" int host_quartet(const int* l, const double* ctr, const int* n, const
     s[k].w = w + off;
     off += n[k];
   }
+}
+
+// Every class through its cooperative program (returns < 0 on a malformed program).
+extern "C" int host_quartet_prog(const int* l, const double* ctr, const int* n, const double* a,
+                                 const double* w, double* out) {
+  Shell s[4];
+  unpack(n, ctr, a, w, s);
+#define GX_CASE(A, B, C, D) \
+  if (l[0] == A && l[1] == B && l[2] == C && l[3] == D) return quartet_prog<A, B, C, D>(s[0], s[1], s[2], s[3], out);
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return -1;
+}
+
+// Every class through the path the device takes: per-lane class code or the
+// cooperative program.
+extern "C" int host_quartet(const int* l, const double* ctr, const int* n, const double* a,
+                            const double* w, double* out) {
+  Shell s[4];
+  unpack(n, ctr, a, w, s);
 #define GX_CASE(A, B, C, D)                                                         \
   if (l[0] == A && l[1] == B && l[2] == C && l[3] == D) {                           \
     quartet<A, B, C, D>(s[0], s[1], s[2], s[3], out);                               \
     return 0;                                                                       \
   }
-  GX_FOR_EACH_CLASS(GX_CASE)
+  GX_FOR_EACH_LANE_CLASS(GX_CASE)
+#undef GX_CASE
+#define GX_CASE(A, B, C, D) \
+  if (l[0] == A && l[1] == B && l[2] == C && l[3] == D) return quartet_prog<A, B, C, D>(s[0], s[1], s[2], s[3], out);
+  GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   return -1;
 }

@@ -33,6 +33,7 @@ def host_lib():
                     os.path.join(HARNESS, 'eri_host.cpp'), '-o', so], check=True)
     lib = ctypes.CDLL(so)
     lib.host_quartet.restype = ctypes.c_int
+    lib.host_quartet_prog.restype = ctypes.c_int
     return lib
 
 
@@ -81,8 +82,13 @@ CLASSES = [(la, lb, lc, ld) for la in range(3) for lb in range(la + 1)
            if (la * (la + 1) // 2 + lb) >= (lc * (lc + 1) // 2 + ld)]
 
 
+@pytest.mark.parametrize('entry', ['host_quartet', 'host_quartet_prog'])
 @pytest.mark.parametrize('cls', CLASSES, ids=lambda c: '%d%d%d%d' % c)
-def test_generated_class_matches_oracle(host_lib, cls):
+def test_generated_class_matches_oracle(host_lib, cls, entry):
+    """``host_quartet``: the code path the device takes for the class (per-lane
+    class code, or the cooperative program for the large classes);
+    ``host_quartet_prog``: the cooperative program of every class, run level by
+    level with a check that each source slot is written before it is read."""
     rng = np.random.default_rng(sum(v * 10 ** i for i, v in enumerate(cls)) + 7)
     for trial in range(2):
         centers = rng.uniform(-1.5, 1.5, (4, 3))
@@ -96,7 +102,7 @@ def test_generated_class_matches_oracle(host_lib, cls):
         w = np.ascontiguousarray(np.concatenate([s[3] for s in shells]))
         nout = int(np.prod([len(COMPS[x]) for x in cls]))
         out = np.zeros(nout)
-        rc = host_lib.host_quartet(l.ctypes.data_as(ctypes.c_void_p), ctr.ctypes.data_as(ctypes.c_void_p),
+        rc = getattr(host_lib, entry)(l.ctypes.data_as(ctypes.c_void_p), ctr.ctypes.data_as(ctypes.c_void_p),
                                    n.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
                                    w.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
         assert rc == 0
