@@ -185,6 +185,16 @@ class ShellSet:
     def nshell(self):
         return len(self.l)
 
+    def moved(self, old_xyz, new_xyz):
+        """The same shells with every shell moved as rigidly as its atom
+        (``atom`` indices into ``old_xyz`` / ``new_xyz``, arrays [natom, 3])."""
+        if np.any(self.atom < 0):
+            raise ValueError('every shell needs an atom to move with the geometry')
+        d = np.asarray(new_xyz, dtype=np.float64) - np.asarray(old_xyz, dtype=np.float64)
+        out = ShellSet(self.l, self.nprim, self.prim_off, self.bf_off, self.ctr + d[self.atom],
+                       self.ctr1 + d[self.atom], self.atom, self.exps, self.coefs, self.bf_scale)
+        return out
+
     @property
     def nbf(self):
         return len(self.bf_scale)

@@ -251,6 +251,51 @@ def emit_class(la, lb, lc, ld):
 COOP_NACC = 90  # classes with more contracted intermediates than this are cooperative
 
 
+def compact_hrr(hentries, nacc, fixed, keep):
+    """Physical H slots for the HRR entries (level, (x, y)) written with
+    logical slots: a slot is reused once the value it holds has been read for
+    the last time at an earlier level (entries of one level run in parallel,
+    so a level never writes a slot that the same level reads).  Slots
+    0..nacc-1 (the accumulators) and `fixed` keep their positions; values in
+    `keep` stay live to the end.  Returns (entries, logical->physical, NH)."""
+    import heapq
+    hentries = sorted(hentries, key=lambda t: t[0])
+    ents = [(lev, x & 0xffff, x >> 16, y & 0xffff, y >> 16) for lev, (x, y) in hentries]
+    inf = 1 << 30
+    last = {}
+    for lev, d, fl, s0, s1 in ents:
+        last[s0] = max(last.get(s0, 0), lev)
+        last[s1] = max(last.get(s1, 0), lev)
+    for k in keep:
+        last[k] = inf
+    for f in fixed:
+        last[f] = inf
+    phys = {i: i for i in range(nacc)}
+    for f in fixed:
+        phys[f] = f
+    live = dict(phys)  # logical -> physical
+    top = max([nacc - 1] + list(fixed)) + 1
+    free = []
+    out = []
+    levels = sorted(set(e[0] for e in ents))
+    k = 0
+    for lev in levels:
+        for lg in [lg for lg in live if last.get(lg, -1) < lev]:
+            heapq.heappush(free, live.pop(lg))
+        while k < len(ents) and ents[k][0] == lev:
+            _, d, fl, s0, s1 = ents[k]
+            if free:
+                pd = heapq.heappop(free)
+            else:
+                pd = top
+                top += 1
+            phys[d] = pd
+            live[d] = pd
+            out.append((lev, (pd | fl << 16, phys[s0] | phys[s1] << 16)))
+            k += 1
+    return out, phys, top
+
+
 def coop_program(la, lb, lc, ld):
     L = la + lb + lc + ld
     zero = (0, 0, 0)
@@ -348,7 +393,9 @@ def coop_program(la, lb, lc, ld):
 
     out = [ket(a, b, c, d) for a in cart(la) for b in cart(lb) for c in cart(lc) for d in cart(ld)]
     assert hslot[0] < 65536
-    hentries.sort(key=lambda t: t[0])
+    hentries, phys, nh = compact_hrr(hentries, len(layout), [], set(out))
+    out = [phys[o] for o in out]
+    hslot[0] = nh
     nhl = lb + ld + 1
     hlev = [0] * (nhl + 1)
     for lev, _ in hentries:
@@ -359,6 +406,198 @@ def coop_program(la, lb, lc, ld):
     hrr = [e for _, e in hentries]
     return dict(L=L, NACC=len(layout), NOUT=len(out), NT=nslot[0], NH=hslot[0], vrr=vrr, vlev=vlev,
                 acc=acc, hrr=hrr, hlev=hl, out=out)
+
+
+# ---------------------------------------------------------------------------
+# Gradient programs.  The nuclear derivative of a contracted quartet on
+# centre A is  d/dA_i (ab|cd) = sum_prim [2 alpha_a (a+1i b|cd) - a_i (a-1i b|cd)]
+# (likewise B and C; D follows from translational invariance).  One VRR per
+# primitive quartet covers every shifted class; its [e0|f0] values are
+# accumulated into four weighted sets:
+#   set 0: weight 1        (lowered A, B, C terms)
+#   set 1: weight 2 alpha_a (raised A),  set 2: 2 alpha_b (raised B),
+#   set 3: 2 alpha_c (raised C)
+# and HRR runs per set.  For every component k of (la lb|lc ld) and every
+# (centre X in A, B, C; axis i) the program lists the H slots of the raised
+# and lowered values and the integer factor:  g = H[plus] - n H[minus].
+#   accumulator entry (unsigned int): T slot | set << 16
+#   gradient term (int2): x = plus, y = minus | n << 16   (minus = the zero slot if n = 0)
+# ---------------------------------------------------------------------------
+def grad_program(la, lb, lc, ld):
+    L = la + lb + lc + ld + 1
+    zero = (0, 0, 0)
+    memo = {}
+    entries = []
+    nslot = [L + 2]
+
+    def node(e, f, m):
+        if e == zero and f == zero:
+            return 1 + m
+        key = (e, f, m)
+        if key in memo:
+            return memo[key]
+        s2 = s3 = s4 = 0
+        k1 = k2 = 0
+        if f == zero:
+            kind = 0
+            i = pick_axis(e)
+            em1 = sub(e, i)
+            s0, s1 = node(em1, f, m), node(em1, f, m + 1)
+            if e[i] >= 2:
+                k1 = e[i] - 1
+                s2, s3 = node(sub(e, i, 2), f, m), node(sub(e, i, 2), f, m + 1)
+        else:
+            kind = 1
+            i = pick_axis(f)
+            fm1 = sub(f, i)
+            s0, s1 = node(e, fm1, m), node(e, fm1, m + 1)
+            if f[i] >= 2:
+                k1 = f[i] - 1
+                s2, s3 = node(e, sub(f, i, 2), m), node(e, sub(f, i, 2), m + 1)
+            if e[i] >= 1:
+                k2 = e[i]
+                s4 = node(sub(e, i), fm1, m + 1)
+        dest = nslot[0]
+        nslot[0] += 1
+        assert dest < 65536 and k1 < 16 and k2 < 16
+        entries.append((sum(e) + sum(f), (dest | kind << 16 | i << 17 | k1 << 20 | k2 << 24, s0 | s1 << 16,
+                                          s2 | s3 << 16, s4)))
+        memo[key] = dest
+        return dest
+
+    def rect(e0, e1, f0, f1):
+        return [(le, lf) for le in range(max(e0, 0), e1 + 1) for lf in range(max(f0, 0), f1 + 1)]
+
+    sets = {1: rect(la + 1, la + lb + 1, lc, lc + ld), 2: rect(la, la + lb + 1, lc, lc + ld),
+            3: rect(la, la + lb, lc + 1, lc + ld + 1)}
+    low = set()
+    if la >= 1:
+        low |= set(rect(la - 1, la + lb - 1, lc, lc + ld))
+    if lb >= 1:
+        low |= set(rect(la, la + lb - 1, lc, lc + ld))
+    if lc >= 1:
+        low |= set(rect(la, la + lb, lc - 1, lc + ld - 1))
+    sets[0] = sorted(low)
+    acc = []
+    index = {}
+    for st in (0, 1, 2, 3):
+        for (le, lf) in sets[st]:
+            for e in cart(le):
+                for f in cart(lf):
+                    index[(st, e, f)] = len(acc)
+                    acc.append(node(e, f, 0) | st << 16)
+    nacc = len(acc)
+    entries.sort(key=lambda t: t[0])
+    vlev = [0] * (L + 2)
+    for lev, _ in entries:
+        vlev[lev + 1] += 1
+    for k in range(1, L + 2):
+        vlev[k] += vlev[k - 1]
+    vrr = [e for _, e in entries]
+
+    hslot = [nacc + 1]  # slot nacc holds 0
+    zslot = nacc
+    hentries = []
+    bra_memo = {}
+
+    def bra(st, a, b, f):
+        key = (st, a, b, f)
+        if key in bra_memo:
+            return bra_memo[key]
+        if b == zero:
+            v = index[(st, a, f)]
+        else:
+            i = [k for k in range(3) if b[k] > 0][0]
+            s0, s1 = bra(st, add(a, i), sub(b, i), f), bra(st, a, sub(b, i), f)
+            v = hslot[0]
+            hslot[0] += 1
+            hentries.append((sum(b), (v | 0 << 16 | i << 17, s0 | s1 << 16)))
+        bra_memo[key] = v
+        return v
+
+    ket_memo = {}
+    lbmax = lb + 1
+
+    def ket(st, a, b, c, d):
+        key = (st, a, b, c, d)
+        if key in ket_memo:
+            return ket_memo[key]
+        if d == zero:
+            v = bra(st, a, b, c)
+        else:
+            i = [k for k in range(3) if d[k] > 0][0]
+            s0, s1 = ket(st, a, b, add(c, i), sub(d, i)), ket(st, a, b, c, sub(d, i))
+            v = hslot[0]
+            hslot[0] += 1
+            hentries.append((lbmax + sum(d), (v | 1 << 16 | i << 17, s0 | s1 << 16)))
+        ket_memo[key] = v
+        return v
+
+    def unit(i):
+        return tuple(1 if k == i else 0 for k in range(3))
+
+    terms = []
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    for X in range(3):
+                        for i in range(3):
+                            if X == 0:
+                                plus = ket(1, add(a, i), b, c, d)
+                                n = a[i]
+                                minus = ket(0, sub(a, i), b, c, d) if n else zslot
+                            elif X == 1:
+                                plus = ket(2, a, add(b, i), c, d)
+                                n = b[i]
+                                minus = ket(0, a, sub(b, i), c, d) if n else zslot
+                            else:
+                                plus = ket(3, a, b, add(c, i), d)
+                                n = c[i]
+                                minus = ket(0, a, b, sub(c, i), d) if n else zslot
+                            terms.append((plus, minus, n))
+    assert hslot[0] < 65536
+    keep = set(t[0] for t in terms) | set(t[1] for t in terms)
+    hentries, phys, nh = compact_hrr(hentries, nacc, [zslot], keep)
+    terms = [(phys[pl], phys[mi] | n << 16) for pl, mi, n in terms]
+    hslot[0] = nh
+    nhl = lbmax + ld + 1
+    hcount = [0] * (nhl + 1)
+    for lev, _ in hentries:
+        hcount[lev] += 1
+    hl = [0]
+    for lev in range(1, nhl):
+        hl.append(hl[-1] + hcount[lev])
+    hrr = [e for _, e in hentries]
+    return dict(L=L, NACC=nacc, NOUT=len(terms) // 9, NT=nslot[0], NH=hslot[0], ZSLOT=zslot, vrr=vrr,
+                vlev=vlev, acc=acc, hrr=hrr, hlev=hl, NHLEV=nhl - 1, terms=terms)
+
+
+def emit_grad_prog(la, lb, lc, ld):
+    g = grad_program(la, lb, lc, ld)
+    tag = 'g%d%d%d%d' % (la, lb, lc, ld)
+    na, nb, nc, nd = ncart(la), ncart(lb), ncart(lc), ncart(ld)
+
+    def arr(name, typ, vals, per_line=8):
+        items = ['{%s}' % ', '.join(str(x) for x in v) if isinstance(v, tuple) else str(v) for v in vals]
+        if not items:
+            items = ['{}']
+        lines = [', '.join(items[k:k + per_line]) for k in range(0, len(items), per_line)]
+        return '__device__ const %s %s_%s[] = {\n  %s};' % (typ, name, tag, ',\n  '.join(lines))
+
+    s = [arr('gx_vrr', 'int4', g['vrr'], 4), arr('gx_vlev', 'int', g['vlev']),
+         arr('gx_acc', 'unsigned int', g['acc'], 12), arr('gx_hrr', 'int2', g['hrr'], 6),
+         arr('gx_hlev', 'int', g['hlev']), arr('gx_gterm', 'int2', g['terms'], 6)]
+    s.append('template <> struct EriGradProg<%d, %d, %d, %d> {' % (la, lb, lc, ld))
+    s.append('  static constexpr int L = %d, NACC = %d, NOUT = %d, NT = %d, NH = %d, ZSLOT = %d;'
+             % (g['L'], g['NACC'], g['NOUT'], g['NT'], g['NH'], g['ZSLOT']))
+    s.append('  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;' % (na, nb, nc, nd))
+    s.append('  static constexpr int NVLEV = %d, NHLEV = %d;' % (g['L'], g['NHLEV']))
+    for fn, typ in (('vrr', 'int4'), ('vlev', 'int'), ('acc', 'unsigned int'), ('hrr', 'int2'),
+                    ('hlev', 'int'), ('gterm', 'int2')):
+        s.append('  static __device__ __forceinline__ const %s* %s() { return gx_%s_%s; }' % (typ, fn, fn, tag))
+    s.append('};')
+    return '\n'.join(s)
 
 
 def emit_prog(la, lb, lc, ld):
@@ -404,13 +643,16 @@ def main():
     parts = ['// Generated by gen_eri.py -- do not edit.',
              '#pragma once',
              'template <int LA, int LB, int LC, int LD> struct EriClass;',
-             'template <int LA, int LB, int LC, int LD> struct EriProg;']
+             'template <int LA, int LB, int LC, int LD> struct EriProg;',
+             'template <int LA, int LB, int LC, int LD> struct EriGradProg;']
     lane = [c for c in classes(lmax) if not is_coop(c)]
     coop = [c for c in classes(lmax) if is_coop(c)]
     for c in lane:
         parts.append(emit_class(*c))
     for c in classes(lmax):
         parts.append(emit_prog(*c))
+    for c in classes(lmax):
+        parts.append(emit_grad_prog(*c))
     parts.append('#define GX_ERI_LMAX %d' % lmax)
     parts.append('#define GX_FOR_EACH_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in classes(lmax)))
     parts.append('#define GX_FOR_EACH_LANE_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in lane))

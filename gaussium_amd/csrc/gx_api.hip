@@ -132,21 +132,34 @@ struct gx_ctx {
   double* d_Q = nullptr;
   gx::PrimPair64* d_pp64 = nullptr;
   gx::PrimPair32* d_pp32 = nullptr;
-  // per class combo: work items (4 ints: bra position, virtual ket range
-  // [v0, v1), bra primitive-pair range p0 | p1 << 16), largest first; and per
-  // bra position the surviving ket counts of each ket segment (FP64 parts,
-  // then FP32 parts)
-  std::vector<int*> d_items, d_rng;
-  std::vector<int> n_items;
+  // Work sets per class combo: work items (4 ints: bra position, virtual ket
+  // range [v0, v1), bra primitive-pair range p0 | p1 << 16), largest first;
+  // and per bra position the surviving ket counts of each ket segment (FP64
+  // parts, then FP32 parts).  `jk` for gx_jk, `gr` for gx_grad (every combo
+  // on the cooperative kernels).
+  struct WorkSet {
+    std::vector<int*> d_items, d_rng;
+    std::vector<int> n_items;
+    double cutoff = -1.0, fp32 = -1.0;
+    int mixed = -1;
+    void release() {
+      for (auto& p : d_items) dfree(&p);
+      for (auto& p : d_rng) dfree(&p);
+      d_items.clear();
+      d_rng.clear();
+      n_items.clear();
+      cutoff = fp32 = -1.0;
+      mixed = -1;
+    }
+  } jk, gr;
   std::vector<int> wg_per_cu;  // [combo][mixed] resident workgroups per CU, 0 = not queried
+  std::vector<int> wg_per_cu_grad;
+  int* d_atom = nullptr;
   // ket segments: per class, runs of pairs with the same primitive-count key
   std::vector<int> seg_lo, seg_hi, cls_seg;  // cls_seg: NCLS + 1 offsets into seg_*
   int *d_seg_lo = nullptr, *d_seg_hi = nullptr;
   std::vector<int64_t> pre_pnp, pre_nab;     // prefix sums over pair ids
   std::vector<int> nab;                       // basis-function pairs per shell pair
-  // cached work orders for (cutoff, fp32_bound, mixed)
-  double ord_cutoff = -1.0, ord_fp32 = -1.0;
-  int ord_mixed = -1;
   int num_cu = 256;
   // scratch
   double *d_Ps = nullptr, *d_Jt = nullptr, *d_Kt = nullptr;
@@ -170,19 +183,16 @@ struct gx_ctx {
     dfree(&d_Q);
     dfree(&d_pp64);
     dfree(&d_pp32);
-    for (auto& p : d_items) dfree(&p);
-    for (auto& p : d_rng) dfree(&p);
-    d_items.clear();
-    d_rng.clear();
-    n_items.clear();
+    jk.release();
+    gr.release();
     wg_per_cu.clear();
+    wg_per_cu_grad.clear();
     dfree(&d_seg_lo);
     dfree(&d_seg_hi);
-    ord_cutoff = ord_fp32 = -1.0;
-    ord_mixed = -1;
   }
   void free_all() {
     free_pairs();
+    dfree(&d_atom);
     dfree(&d_l);
     dfree(&d_np);
     dfree(&d_poff);
@@ -342,6 +352,41 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
   return hipErrorInvalidValue;
 }
 
+// Two-electron gradient of one class combination (gx_coop.hip, k_grad_coop).
+size_t grad_lds_bytes(int la, int lb, int lc, int ld) {
+#define GX_CASE(A, B, C, D)                                                                     \
+  if (la == A && lb == B && lc == C && ld == D) {                                               \
+    using G = EriGradProg<A, B, C, D>;                                                          \
+    const int nblk = (G::NA + G::NB) * (G::NC + G::ND) + G::NA * G::NB + G::NC * G::ND;          \
+    return ((size_t)gx::grad_lds<G>() + nblk) * sizeof(double);                                 \
+  }
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return 0;
+}
+
+hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s, const gx::GradArgs& a,
+                       int* per_cu) {
+  const size_t lds = grad_lds_bytes(la, lb, lc, ld);
+#define GX_CASE(A, B, C, D)                                                                   \
+  if (la == A && lb == B && lc == C && ld == D) {                                             \
+    const void* fn = (const void*)gx::k_grad_coop<A, B, C, D>;                               \
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                            \
+    if (*per_cu <= 0) {                                                                       \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, GX_WAVE, lds);             \
+      if (e != hipSuccess) return e;                                                          \
+      if (*per_cu < 1) *per_cu = 1;                                                           \
+    }                                                                                         \
+    const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
+    hipLaunchKernelGGL((gx::k_grad_coop<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a); \
+    return hipGetLastError();                                                                 \
+  }
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return hipErrorInvalidValue;
+}
+
 // Whether a class combination runs on the cooperative kernels (one quartet
 // per wave; always FP64).
 bool combo_is_coop(int X, int Y) { return gx_class_is_coop(class_la(X), class_lb(X), class_la(Y), class_lb(Y)); }
@@ -471,14 +516,14 @@ struct JKItem {
   int bpos, v0, v1, prims;
 };
 
-int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
-  if (cutoff == c->ord_cutoff && fp32 == c->ord_fp32 && mixed == c->ord_mixed && !c->d_items.empty())
-    return GX_OK;
-  for (auto& p : c->d_items) dfree(&p);
-  for (auto& p : c->d_rng) dfree(&p);
-  c->d_items.assign(combo_count(), nullptr);
-  c->d_rng.assign(combo_count(), nullptr);
-  c->n_items.assign(combo_count(), 0);
+int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = false) {
+  gx_ctx::WorkSet& ws = grad ? c->gr : c->jk;
+  if (grad) fp32 = 0.0, mixed = 0;
+  if (cutoff == ws.cutoff && fp32 == ws.fp32 && mixed == ws.mixed && !ws.d_items.empty()) return GX_OK;
+  ws.release();
+  ws.d_items.assign(combo_count(), nullptr);
+  ws.d_rng.assign(combo_count(), nullptr);
+  ws.n_items.assign(combo_count(), 0);
 #ifndef GX_ITEM_OVERHEAD
 #define GX_ITEM_OVERHEAD 256.0
 #endif
@@ -504,7 +549,7 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
       // per bra: virtual layout and per-chunk cost (widest ket of the chunk).
       // Per-lane classes: chunks of 64 kets, four waves per item; cooperative
       // classes: one ket per chunk, one wave per item.
-      const bool coop = combo_is_coop(X, Y);
+      const bool coop = grad || combo_is_coop(X, Y);
       const int csz = coop ? 1 : GX_WAVE;
       const double waves = coop ? 1.0 : 4.0;
       std::vector<std::vector<double>> chunk(nb);
@@ -578,14 +623,14 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed) {
         flat[4 * k + 2] = it.v1;
         flat[4 * k + 3] = it.prims;
       }
-      c->n_items[combo] = (int)items.size();
-      GX_TRY(upload(&c->d_items[combo], flat, c->stream));
-      GX_TRY(upload(&c->d_rng[combo], rng, c->stream));
+      ws.n_items[combo] = (int)items.size();
+      GX_TRY(upload(&ws.d_items[combo], flat, c->stream));
+      GX_TRY(upload(&ws.d_rng[combo], rng, c->stream));
       GX_HIP(hipStreamSynchronize(c->stream));  // before the host vectors go out of scope
     }
-  c->ord_cutoff = cutoff;
-  c->ord_fp32 = fp32;
-  c->ord_mixed = mixed;
+  ws.cutoff = cutoff;
+  ws.fp32 = fp32;
+  ws.mixed = mixed;
   return GX_OK;
 }
 
@@ -701,6 +746,8 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
   GX_TRY(upload(&c->d_exps, c->exps, c->stream));
   GX_TRY(upload(&c->d_coefs, c->coefs, c->stream));
   GX_TRY(upload(&c->d_scale, c->scale, c->stream));
+  dfree(&c->d_atom);
+  GX_TRY(upload(&c->d_atom, c->atom, c->stream));
   const size_t nn = (size_t)nbf * nbf;
   GX_TRY(dalloc(&c->d_Ps, nn));
   GX_TRY(dalloc(&c->d_Jt, nn));
@@ -816,8 +863,6 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
     c->pairmap[(size_t)c->pA[p] * nshell + c->pB[p]] = p;
     c->pairmap[(size_t)c->pB[p] * nshell + c->pA[p]] = p;
   }
-  c->ord_cutoff = c->ord_fp32 = -1.0;
-  c->ord_mixed = -1;
   GX_HIP(hipStreamSynchronize(c->stream));
   return GX_OK;
 }
@@ -1001,7 +1046,7 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
       const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
       if (nb == 0 || nk == 0) continue;
-      if (c->n_items[combo] == 0) continue;
+      if (c->jk.n_items[combo] == 0) continue;
       gx::JKArgs a;
       a.sh = c->shells();
       a.pr = c->pairs();
@@ -1010,9 +1055,9 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
       a.same = (X == Y);
       a.seg_lo = c->d_seg_lo + c->cls_seg[Y];
       a.nseg = c->cls_seg[Y + 1] - c->cls_seg[Y];
-      a.items = reinterpret_cast<const int4*>(c->d_items[combo]);
-      a.nitem = c->n_items[combo];
-      a.rng = c->d_rng[combo];
+      a.items = reinterpret_cast<const int4*>(c->jk.d_items[combo]);
+      a.nitem = c->jk.n_items[combo];
+      a.rng = c->jk.d_rng[combo];
       a.master = c->d_boys;
       a.tab64 = c->d_tab64;
       a.tab32 = c->d_tab32;
@@ -1034,6 +1079,65 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, c->d_Jt,
                      c->d_Kt, J, K);
   GX_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cutoff) {
+  if (!c || !P || !W || !grad) return fail(GX_ERR_ARG, "gx_grad: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_grad: no basis set");
+  if (c->natom == 0) return fail(GX_ERR_STATE, "gx_grad: no nuclei set");
+  if (!(cutoff >= 0.0)) return fail(GX_ERR_ARG, "gx_grad: negative threshold");
+  for (int sh = 0; sh < c->nshell; ++sh)
+    if (c->atom[sh] < 0 || c->atom[sh] >= c->natom)
+      return fail(GX_ERR_STATE, "gx_grad: shell " + std::to_string(sh) + " has no nucleus (atom index " +
+                                    std::to_string(c->atom[sh]) + ")");
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      if (c->cls_start[X + 1] == c->cls_start[X] || c->cls_start[Y + 1] == c->cls_start[Y]) continue;
+      const size_t lds = grad_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y));
+      if (lds > (size_t)c->max_lds)
+        return fail(GX_ERR_ARG, "gx_grad: class (" + std::to_string(class_la(X)) + std::to_string(class_lb(X)) +
+                                    "|" + std::to_string(class_la(Y)) + std::to_string(class_lb(Y)) +
+                                    ") needs " + std::to_string(lds) + " bytes of LDS, the device has " +
+                                    std::to_string(c->max_lds));
+    }
+  GX_HIP(hipSetDevice(c->device));
+  const int N = c->nbf;
+  const long nn = (long)N * N;
+  const unsigned g = (unsigned)((nn + 255) / 256);
+  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, P, c->d_Ps);
+  hipLaunchKernelGGL(gx::k_fill, dim3((3 * c->natom + 255) / 256), dim3(256), 0, c->stream, (long)3 * c->natom,
+                     grad, 0.0);
+  const int npair = c->nshell * (c->nshell + 1) / 2;
+  hipLaunchKernelGGL(gx::k_one_grad, dim3((npair + 63) / 64), dim3(64), 0, c->stream, c->nshell, c->d_l,
+                     c->d_np, c->d_poff, c->d_bf, c->d_atom, c->d_ctr1, c->d_exps, c->d_coefs, c->d_scale,
+                     c->natom, c->d_nxyz, c->d_nZ, c->d_boys, N, P, W, grad);
+  hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_nxyz,
+                     c->d_nZ, grad);
+  GX_HIP(hipGetLastError());
+  GX_TRY(build_orders(c, cutoff, 0.0, 0, true));
+  if (c->wg_per_cu_grad.empty()) c->wg_per_cu_grad.assign(combo_count(), 0);
+  int combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      if (c->gr.n_items[combo] == 0) continue;
+      gx::GradArgs a;
+      a.sh = c->shells();
+      a.pr = c->pairs();
+      a.bra0 = c->cls_start[X];
+      a.seg_lo = c->d_seg_lo + c->cls_seg[Y];
+      a.nseg = c->cls_seg[Y + 1] - c->cls_seg[Y];
+      a.items = reinterpret_cast<const int4*>(c->gr.d_items[combo]);
+      a.nitem = c->gr.n_items[combo];
+      a.rng = c->gr.d_rng[combo];
+      a.tab64 = c->d_tab64;
+      a.P = c->d_Ps;
+      a.atom = c->d_atom;
+      a.grad = grad;
+      a.N = N;
+      GX_HIP(launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, c->stream, a,
+                         &c->wg_per_cu_grad[combo]));
+    }
   return GX_OK;
 }
 

@@ -33,10 +33,11 @@ namespace gx {
 
 // One primitive pair (a on A, b on B) of an oriented shell pair (A, B):
 //   p = a + b, ip = 1/p, P the Gaussian product centre, PA = P - A,
-//   Kip = sqrt(2 pi^(5/2)) c_a c_b exp(-a b / p |AB|^2) / p
-// so that a primitive quartet's prefactor is Kip_bra Kip_ket / sqrt(p + q).
+//   Kip = sqrt(2 pi^(5/2)) c_a c_b exp(-a b / p |AB|^2) / p,  ea = a
+// so that a primitive quartet's prefactor is Kip_bra Kip_ket / sqrt(p + q);
+// ea (and p - ea) weight the raised classes of the nuclear gradient.
 struct __align__(16) PrimPair64 {
-  double p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad;
+  double p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, ea;
 };
 
 struct __align__(16) PrimPair32 {

@@ -21,8 +21,10 @@ constexpr int coop_lds() {
 }
 
 // Contracted shell quartet into H: on return H[G::out()[k]] is component k
-// (reference order), unscaled.  Every lane of the wave calls it.
-template <class G>
+// (reference order), unscaled.  Every lane of the wave calls it.  With
+// GRAD, G is a gradient program (EriGradProg): accumulator entries carry a
+// weight set (1, 2 alpha_a, 2 alpha_b, 2 alpha_c) and slot NACC stays zero.
+template <class G, bool GRAD = false>
 __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra, int nb,
                                              const PrimPair64* __restrict__ ket, int nk, double ABx,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
@@ -31,7 +33,7 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
   double* __restrict__ T = H + G::NACC;
   const int4* __restrict__ vp = G::vrr();
   const int* __restrict__ vl = G::vlev();
-  const unsigned short* __restrict__ ap = G::acc();
+  const auto* __restrict__ ap = G::acc();
   __syncthreads();  // the previous quartet's readers of H are done
   for (int t = lane; t < G::NACC; t += GX_WAVE) H[t] = 0.0;
   if (lane == 0) T[0] = 0.0;
@@ -77,7 +79,16 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
         }
         __syncthreads();
       }
-      for (int t = lane; t < G::NACC; t += GX_WAVE) H[t] += T[ap[t]];
+      if constexpr (GRAD) {
+        const double w1 = 2.0 * b.ea, w2 = 2.0 * (b.p - b.ea), w3 = 2.0 * k.ea;
+        for (int t = lane; t < G::NACC; t += GX_WAVE) {
+          const unsigned e = ap[t], st = e >> 16;
+          const double wt = st == 0 ? 1.0 : (st == 1 ? w1 : (st == 2 ? w2 : w3));
+          H[t] = fma(wt, T[e & 0xffffu], H[t]);
+        }
+      } else {
+        for (int t = lane; t < G::NACC; t += GX_WAVE) H[t] += T[ap[t]];
+      }
       __syncthreads();  // T is rewritten by the next primitive quartet
     }
   }
@@ -284,6 +295,138 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
         else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
         else { u -= SBC; r0 = bo; c0 = dof; s = ND; }
         atomicAdd(&dst[(long)(r0 + u / s) * N + c0 + u % s], m * x);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-electron part of the RHF nuclear gradient over the canonical shell
+// quartets (the same work items and screening as the J/K build):
+//   g_X += sum_k D_k d(ab|cd)_k / dX,  D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc)
+// with f the 8-fold degeneracy weight (the energy of one canonical quartet is
+// f v D_k / f = f v (4 P_ab P_cd - P_ac P_bd - P_ad P_bc), from
+// E_2e = 1/2 tr(P J) - 1/4 tr(P K)).  Derivatives on A, B, C come from the
+// gradient program; D from translational invariance.
+struct GradArgs {
+  ShellView sh;
+  PairView pr;
+  int bra0;
+  const int* seg_lo;
+  int nseg;
+  const int4* items;
+  int nitem;
+  const int* rng;
+  const double* tab64;
+  const double* P;     // density, pre-scaled by s_k s_l
+  const int* atom;     // shell -> atom
+  double* grad;        // 3 natom
+  int N;
+};
+
+template <class G>
+constexpr int grad_lds() {
+  return G::NACC + G::NT > G::NH ? G::NACC + G::NT : G::NH;
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
+  using G = EriGradProg<LA, LB, LC, LD>;
+  constexpr int NA = G::NA, NB = G::NB, NC = G::NC, ND = G::ND;
+  constexpr int SAB = NA * NB, SCD = NC * ND, SAC = NA * NC, SAD = NA * ND, SBC = NB * NC, SBD = NB * ND;
+  constexpr int NBLK = SAB + SCD + SAC + SAD + SBC + SBD;
+  extern __shared__ double smem[];
+  double* __restrict__ H = smem;
+  double* __restrict__ pblk = smem + grad_lds<G>();
+  __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
+  const int lane = threadIdx.x;
+  const int N = a.N;
+  const int nseg = a.nseg;
+  const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
+  const int2* __restrict__ gt = G::gterm();
+  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
+    const int4 it = a.items[w];
+    const int ip = a.bra0 + it.x;
+    const int A = a.pr.A[ip], B = a.pr.B[ip];
+    const int ao = a.sh.bf[A], bo = a.sh.bf[B];
+    const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
+    const PrimPair64* __restrict__ bra = a.pr.pp64 + a.pr.poff[ip] + p0;
+    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
+                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
+    const double fA = (A == B) ? 0.5 : 1.0;
+    __syncthreads();
+    if (lane < nseg) {
+      r_start[lane] = a.seg_lo[lane];
+      r_pre[lane + 1] = a.rng[(long)it.x * 2 * nseg + lane];
+    }
+    __syncthreads();
+    if (lane == 0) {
+      r_pre[0] = 0;
+      for (int r = 0; r < nseg; ++r) r_pre[r + 1] += r_pre[r];
+    }
+    __syncthreads();
+    const int total = min(r_pre[nseg], it.z);
+    for (int v = it.y; v < total; ++v) {
+      int r = 0;
+      while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
+      const int jp = r_start[r] + (v - r_pre[r]);
+      const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
+      const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
+      coop_quartet<G, true>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
+                            a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
+                            a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], tab, H);
+      double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
+      if (jp == ip) f *= 0.5;
+      for (int t = lane; t < NBLK; t += GX_WAVE) {
+        int r0, c0, st;
+        int u = t;
+        if (u < SAB) { r0 = ao; c0 = bo; st = NB; }
+        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; st = ND; }
+        else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; st = NC; }
+        else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; st = ND; }
+        else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; st = NC; }
+        else { u -= SBC; r0 = bo; c0 = dof; st = ND; }
+        pblk[t] = a.P[(long)(r0 + u / st) * N + c0 + u % st];
+      }
+      __syncthreads();
+      const double* pab = pblk;
+      const double* pcd = pab + SAB;
+      const double* pac = pcd + SCD;
+      const double* pad = pac + SAC;
+      const double* pbc = pad + SAD;
+      const double* pbd = pbc + SBC;
+      double g[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) g[j] = 0.0;
+      for (int k = lane; k < G::NOUT; k += GX_WAVE) {
+        const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
+        const double D = f * (4.0 * pab[ia * NB + ib] * pcd[ic * ND + id] - pac[ia * NC + ic] * pbd[ib * ND + id] -
+                              pad[ia * ND + id] * pbc[ib * NC + ic]);
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+          const int2 e = gt[9 * k + j];
+          const unsigned y = (unsigned)e.y;
+          g[j] = fma(D, H[e.x] - (double)(y >> 16) * H[y & 0xffffu], g[j]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        double x = g[j];
+#pragma unroll
+        for (int off = GX_WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, GX_WAVE);
+        g[j] = x;
+      }
+      if (lane == 0) {
+        const int at[3] = {a.atom[A], a.atom[B], a.atom[Cs]};
+        const int atD = a.atom[Ds];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          const double gd = -(g[x] + g[3 + x] + g[6 + x]);
+          atomicAdd(&a.grad[3 * at[0] + x], g[x]);
+          atomicAdd(&a.grad[3 * at[1] + x], g[3 + x]);
+          atomicAdd(&a.grad[3 * at[2] + x], g[6 + x]);
+          atomicAdd(&a.grad[3 * atD + x], gd);
+        }
       }
     }
   }

@@ -45,7 +45,7 @@ __global__ void k_primpairs(int npp, const int* __restrict__ pair_of, const int*
   r.PAx = r.Px - Ax;
   r.PAy = r.Py - Ay;
   r.PAz = r.Pz - Az;
-  r.pad = 0.0;
+  r.ea = a;
   out64[t] = r;
   PrimPair32 s;
   s.p = (float)p;

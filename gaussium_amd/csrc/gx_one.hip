@@ -224,3 +224,240 @@ __global__ __launch_bounds__(64) void k_one_electron(
 }
 
 }  // namespace gx
+
+namespace gx {
+
+// ---------------------------------------------------------------------------
+// One-electron part of the RHF nuclear gradient (the reference has no
+// gradient; its optimiser differentiates energies numerically,
+// geometryoptimization/nelder_mead.py:126-144):
+//   g_X += sum_ab P_ab d(T + V)_ab / dX  -  W_ab dS_ab / dX
+// with d/dA_i <a|O|b> = 2 alpha <a+1i|O|b> - a_i <a-1i|O|b> (likewise B);
+// S and T depend on A - B only (d/dB = -d/dA); the nuclear-attraction term of
+// nucleus C moves with C:  d/dC = -(d/dA + d/dB).
+// One thread per shell pair (i >= j); atomics into grad[3 * atom + axis].
+#define ONE_GRAD_E (ONE_LMAX + 3)      // bra raised by 1, ket by 2 (kinetic) or 1
+#define ONE_GRAD_R (2 * ONE_LMAX + 1)  // Hermite order for raised V
+
+__device__ inline void hermite_e_g(int imax, int jmax, double PA, double PB, double o2p,
+                                   double (*E)[ONE_GRAD_E + 1][2 * ONE_GRAD_E + 1]) {
+  for (int i = 0; i <= imax; ++i)
+    for (int j = 0; j <= jmax; ++j)
+      for (int t = 0; t <= i + j; ++t) E[i][j][t] = 0.0;
+  E[0][0][0] = 1.0;
+  for (int i = 0; i <= imax; ++i) {
+    if (i > 0) {
+      for (int t = 0; t <= i; ++t) {
+        double v = (t <= i - 1) ? PA * E[i - 1][0][t] : 0.0;
+        if (t > 0) v += o2p * E[i - 1][0][t - 1];
+        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
+        E[i][0][t] = v;
+      }
+    }
+    for (int j = 1; j <= jmax; ++j) {
+      for (int t = 0; t <= i + j; ++t) {
+        double v = (t <= i + j - 1) ? PB * E[i][j - 1][t] : 0.0;
+        if (t > 0) v += o2p * E[i][j - 1][t - 1];
+        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+        E[i][j][t] = v;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_one_grad(
+    int nshell, const int* __restrict__ sl, const int* __restrict__ snp, const int* __restrict__ spoff,
+    const int* __restrict__ sbf, const int* __restrict__ satom, const double* __restrict__ ctr1,
+    const double* __restrict__ exps, const double* __restrict__ coefs, const double* __restrict__ scale,
+    int natom, const double* __restrict__ nxyz, const double* __restrict__ nZ,
+    const double* __restrict__ master, int N, const double* __restrict__ P, const double* __restrict__ W,
+    double* __restrict__ grad) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npair = nshell * (nshell + 1) / 2;
+  if (t >= npair) return;
+  int si = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (si * (si + 1) / 2 > t) --si;
+  while ((si + 1) * (si + 2) / 2 <= t) ++si;
+  const int sj = t - si * (si + 1) / 2;
+  const int la = sl[si], lb = sl[sj];
+  const int na = ncart_d(la), nb = ncart_d(lb);
+  const double A[3] = {ctr1[3 * si], ctr1[3 * si + 1], ctr1[3 * si + 2]};
+  const double B[3] = {ctr1[3 * sj], ctr1[3 * sj + 1], ctr1[3 * sj + 2]};
+  const int ao = sbf[si], bo = sbf[sj];
+  const double wpair = (si == sj) ? 1.0 : 2.0;  // both triangles of an off-diagonal block
+  double pab[36], wab[36];
+  for (int ka = 0; ka < na; ++ka)
+    for (int kb = 0; kb < nb; ++kb) {
+      const long i = ao + ka, j = bo + kb;
+      const double s = wpair * scale[i] * scale[j];
+      pab[ka * nb + kb] = s * P[i * N + j];
+      wab[ka * nb + kb] = s * W[i * N + j];
+    }
+  double E[3][ONE_GRAD_E + 1][ONE_GRAD_E + 1][2 * ONE_GRAD_E + 1];
+  double R[ONE_GRAD_R + 1][ONE_GRAD_R + 1][ONE_GRAD_R + 1][ONE_GRAD_R + 1];
+  double F[ONE_GRAD_R + 1];
+  double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+  // overlap and kinetic: depend on A - B only
+  for (int ia = 0; ia < snp[si]; ++ia) {
+    const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
+    for (int ib = 0; ib < snp[sj]; ++ib) {
+      const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
+      const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
+      double Pc[3], r2 = 0.0;
+      for (int x = 0; x < 3; ++x) {
+        Pc[x] = (a * A[x] + b * B[x]) / p;
+        r2 += (A[x] - B[x]) * (A[x] - B[x]);
+        hermite_e_g(la + 1, lb + 2, Pc[x] - A[x], Pc[x] - B[x], o2p, E[x]);
+      }
+      const double s00 = exp(-mu * r2) * ca * cb * pow(M_PI / p, 1.5);
+      auto S1 = [&](int x, int i, int j) { return E[x][i][j][0]; };
+      // <a|T|b> for exponent triples ea, eb (the reference's del-operator expansion on b)
+      auto Tval = [&](const int* ea, const int* eb) {
+        const double sx = S1(0, ea[0], eb[0]), sy = S1(1, ea[1], eb[1]), sz = S1(2, ea[2], eb[2]);
+        const int lbt = eb[0] + eb[1] + eb[2];
+        double tv = b * (2 * lbt + 3) * sx * sy * sz;
+        tv -= 2.0 * b * b * (S1(0, ea[0], eb[0] + 2) * sy * sz + sx * S1(1, ea[1], eb[1] + 2) * sz +
+                             sx * sy * S1(2, ea[2], eb[2] + 2));
+        if (eb[0] >= 2) tv -= 0.5 * eb[0] * (eb[0] - 1) * S1(0, ea[0], eb[0] - 2) * sy * sz;
+        if (eb[1] >= 2) tv -= 0.5 * eb[1] * (eb[1] - 1) * sx * S1(1, ea[1], eb[1] - 2) * sz;
+        if (eb[2] >= 2) tv -= 0.5 * eb[2] * (eb[2] - 1) * sx * sy * S1(2, ea[2], eb[2] - 2);
+        return tv;
+      };
+      auto Sval = [&](const int* ea, const int* eb) {
+        return S1(0, ea[0], eb[0]) * S1(1, ea[1], eb[1]) * S1(2, ea[2], eb[2]);
+      };
+      for (int ka = 0; ka < na; ++ka) {
+        int ea[3];
+        cart_d(la, ka, ea);
+        for (int kb = 0; kb < nb; ++kb) {
+          int eb[3];
+          cart_d(lb, kb, eb);
+          const double pw = pab[ka * nb + kb], ww = wab[ka * nb + kb];
+          for (int x = 0; x < 3; ++x) {
+            int up[3] = {ea[0], ea[1], ea[2]};
+            up[x] += 1;
+            double dS = 2.0 * a * Sval(up, eb), dT = 2.0 * a * Tval(up, eb);
+            if (ea[x] > 0) {
+              int dn[3] = {ea[0], ea[1], ea[2]};
+              dn[x] -= 1;
+              dS -= ea[x] * Sval(dn, eb);
+              dT -= ea[x] * Tval(dn, eb);
+            }
+            const double g = s00 * (pw * dT - ww * dS);
+            gA[x] += g;
+            gB[x] -= g;
+          }
+        }
+      }
+    }
+  }
+  // nuclear attraction, one nucleus at a time
+  const int L = la + lb + 1;
+  for (int c = 0; c < natom; ++c) {
+    double gC[3] = {0.0, 0.0, 0.0};
+    for (int ia = 0; ia < snp[si]; ++ia) {
+      const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
+      for (int ib = 0; ib < snp[sj]; ++ib) {
+        const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
+        const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
+        double Pc[3], r2 = 0.0;
+        for (int x = 0; x < 3; ++x) {
+          Pc[x] = (a * A[x] + b * B[x]) / p;
+          r2 += (A[x] - B[x]) * (A[x] - B[x]);
+          hermite_e_g(la + 1, lb + 1, Pc[x] - A[x], Pc[x] - B[x], o2p, E[x]);
+        }
+        const double PC[3] = {Pc[0] - nxyz[3 * c], Pc[1] - nxyz[3 * c + 1], Pc[2] - nxyz[3 * c + 2]};
+        const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+        boys_master(L, Tb, master, F);
+        double m2p = 1.0;
+        for (int n = 0; n <= L; ++n) {
+          R[n][0][0][0] = m2p * F[n];
+          m2p *= -2.0 * p;
+        }
+        for (int n = L - 1; n >= 0; --n) {
+          const int top = L - n;
+          for (int tt = 0; tt <= top; ++tt)
+            for (int u = 0; u <= top - tt; ++u)
+              for (int v = 0; v <= top - tt - u; ++v) {
+                if (tt + u + v == 0) continue;
+                double r;
+                if (tt > 0) {
+                  r = PC[0] * R[n + 1][tt - 1][u][v];
+                  if (tt > 1) r += (tt - 1) * R[n + 1][tt - 2][u][v];
+                } else if (u > 0) {
+                  r = PC[1] * R[n + 1][tt][u - 1][v];
+                  if (u > 1) r += (u - 1) * R[n + 1][tt][u - 2][v];
+                } else {
+                  r = PC[2] * R[n + 1][tt][u][v - 1];
+                  if (v > 1) r += (v - 1) * R[n + 1][tt][u][v - 2];
+                }
+                R[n][tt][u][v] = r;
+              }
+        }
+        const double pref = -nZ[c] * 2.0 * M_PI / p * exp(-mu * r2) * ca * cb;
+        auto Vval = [&](const int* ea, const int* eb) {
+          double sum = 0.0;
+          for (int tt = 0; tt <= ea[0] + eb[0]; ++tt)
+            for (int u = 0; u <= ea[1] + eb[1]; ++u)
+              for (int v = 0; v <= ea[2] + eb[2]; ++v)
+                sum += E[0][ea[0]][eb[0]][tt] * E[1][ea[1]][eb[1]][u] * E[2][ea[2]][eb[2]][v] * R[0][tt][u][v];
+          return sum;
+        };
+        for (int ka = 0; ka < na; ++ka) {
+          int ea[3];
+          cart_d(la, ka, ea);
+          for (int kb = 0; kb < nb; ++kb) {
+            int eb[3];
+            cart_d(lb, kb, eb);
+            const double pw = pref * pab[ka * nb + kb];
+            for (int x = 0; x < 3; ++x) {
+              int ua[3] = {ea[0], ea[1], ea[2]}, ub[3] = {eb[0], eb[1], eb[2]};
+              ua[x] += 1;
+              ub[x] += 1;
+              double dA = 2.0 * a * Vval(ua, eb), dB = 2.0 * b * Vval(ea, ub);
+              if (ea[x] > 0) {
+                int da[3] = {ea[0], ea[1], ea[2]};
+                da[x] -= 1;
+                dA -= ea[x] * Vval(da, eb);
+              }
+              if (eb[x] > 0) {
+                int db[3] = {eb[0], eb[1], eb[2]};
+                db[x] -= 1;
+                dB -= eb[x] * Vval(ea, db);
+              }
+              gA[x] += pw * dA;
+              gB[x] += pw * dB;
+              gC[x] -= pw * (dA + dB);
+            }
+          }
+        }
+      }
+    }
+    for (int x = 0; x < 3; ++x)
+      if (gC[x] != 0.0) atomicAdd(&grad[3 * c + x], gC[x]);
+  }
+  const int atA = satom[si], atB = satom[sj];
+  for (int x = 0; x < 3; ++x) {
+    atomicAdd(&grad[3 * atA + x], gA[x]);
+    atomicAdd(&grad[3 * atB + x], gB[x]);
+  }
+}
+
+// Nuclear repulsion gradient  dE_nuc/dR_A = -sum_B Z_A Z_B (R_A - R_B) / |R_A - R_B|^3
+// (common/nuclear_repulsion.py:5-24 of the reference gives E_nuc), added to grad.
+__global__ void k_nuc_grad(int natom, const double* __restrict__ nxyz, const double* __restrict__ nZ,
+                           double* __restrict__ grad) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= natom) return;
+  double g[3] = {0.0, 0.0, 0.0};
+  for (int b = 0; b < natom; ++b) {
+    if (b == a) continue;
+    const double d[3] = {nxyz[3 * a] - nxyz[3 * b], nxyz[3 * a + 1] - nxyz[3 * b + 1], nxyz[3 * a + 2] - nxyz[3 * b + 2]};
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double f = nZ[a] * nZ[b] / (r2 * sqrt(r2));
+    for (int x = 0; x < 3; ++x) g[x] -= f * d[x];
+  }
+  for (int x = 0; x < 3; ++x) grad[3 * a + x] += g[x];
+}
+
+}  // namespace gx

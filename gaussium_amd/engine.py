@@ -136,6 +136,20 @@ class Engine:
             _lib.check(n, 'gx_jk_launch_times')
         return {tuple(int(x) for x in r[:4]): (float(r[4]), int(r[5])) for r in out[:n]}
 
+    def grad(self, P, W, cutoff=0.0):
+        """Nuclear gradient dE/dR [natom, 3] of the closed-shell RHF energy from
+        the density ``P`` and the energy-weighted density ``W`` (device tensors)."""
+        if self.nuclei is None:
+            raise _lib.GxError('grad needs nuclei (set_nuclei)')
+        n = self.nbf
+        P, W = self._dev(P), self._dev(W)
+        if P.shape != (n, n) or W.shape != (n, n):
+            raise ValueError('P and W must be (%d, %d)' % (n, n))
+        g = self._empty(len(self.nuclei), 3)
+        self._bind_stream()
+        _lib.check(self.lib.gx_grad(self.ctx, _p(P), _p(W), _p(g), float(cutoff)), 'gx_grad')
+        return g
+
     def jk_stats(self, cutoff=0.0, fp32_bound=0.0, mixed=False):
         st = _lib.GxStats()
         _lib.check(self.lib.gx_jk_stats(self.ctx, float(cutoff), float(fp32_bound),

@@ -68,7 +68,7 @@ class RestrictedHF:
 
     def __init__(self, nuclei_array, basis_set_array, electrons, symmetry=None, processes=1,
                  schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64', max_iter=None, engine=None,
-                 verbose=False, threshold=1e-12):
+                 verbose=False, threshold=1e-12, density_threshold=None):
         self.nuclei_array = nuclei_array
         self.basis_set_array = basis_set_array
         self.electrons = electrons
@@ -76,7 +76,7 @@ class RestrictedHF:
         self.verbose = verbose
         t0 = time.perf_counter()
         self.engine = engine if engine is not None else Engine(basis_set_array, nuclei_array)
-        if engine is not None:
+        if engine is not None and basis_set_array is not None:
             self.engine.set_basis(basis_set_array, nuclei_array)
             self.engine.set_nuclei(nuclei_array)
         S, T, V = self.engine.one_electron()
@@ -87,8 +87,10 @@ class RestrictedHF:
         self.linear_algebra = LinearAlgebra(S)
         self.fock_builder = FockMatrixRestricted(self.core_hamiltonian, self.engine, schwarz_cutoff,
                                                  fp32_bound, precision)
+        self.schwarz_cutoff = schwarz_cutoff
         self.scf_method = RestrictedSCF(self.linear_algebra, electrons, S, self.fock_builder,
-                                        threshold=threshold, max_iter=max_iter)
+                                        threshold=threshold, max_iter=max_iter,
+                                        density_threshold=density_threshold)
         self.setup_time = time.perf_counter() - t0
         self._repulsion = None
 
@@ -113,3 +115,15 @@ class RestrictedHF:
 
     def energies(self):
         return self.begin_scf()[0], 0.0
+
+    def gradient(self):
+        """Analytic nuclear gradient dE_total/dR [natom, 3] (device tensor) at the
+        converged SCF: the density and the energy-weighted density
+        W = 2 sum_occ e_i C_i C_i^T of the last iteration (runs the SCF first if
+        it has not run)."""
+        if not hasattr(self.scf_method, 'density'):
+            self.begin_scf()
+        eps = self.scf_method.orbital_energies
+        c = self.scf_method.orbital_coefficients[:, :self.electrons // 2]
+        W = 2.0 * (c * eps[:self.electrons // 2]) @ c.T
+        return self.engine.grad(self.scf_method.density, W, cutoff=self.schwarz_cutoff)

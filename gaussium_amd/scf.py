@@ -119,7 +119,7 @@ class DIIS:
 
 class RestrictedSCF:
     def __init__(self, linear_algebra, electrons, overlap, hamiltonian_matrix_factory, threshold=1e-12,
-                 max_iter=None, use_diis=True):
+                 max_iter=None, use_diis=True, density_threshold=None):
         self.linear_algebra = linear_algebra
         self.electrons = electrons
         self.hamiltonian_matrix_factory = hamiltonian_matrix_factory
@@ -129,9 +129,13 @@ class RestrictedSCF:
         self.diis = DIIS(overlap, linear_algebra) if use_diis else None
         self.iterations = 0
         self.energy_history = []
+        # optional second stop test, max |P_n - P_{n-1}| below this (gradients
+        # are first order in the density error, energies second order)
+        self.density_threshold = density_threshold
 
     def begin_iterations(self, orbital_energies, orbital_coefficients):
         previous = 0.0
+        previous_density = None
         while True:
             density = density_matrix_restricted(orbital_coefficients, self.electrons)
             fock = self.hamiltonian_matrix_factory.create(density)
@@ -140,7 +144,13 @@ class RestrictedSCF:
             previous = total
             self.iterations += 1
             self.energy_history.append(total)
-            if abs(delta) < self.threshold:
+            converged = abs(delta) < self.threshold
+            if converged and self.density_threshold is not None:
+                converged = previous_density is not None and \
+                    float((density - previous_density).abs().max()) < self.density_threshold
+            previous_density = density
+            self.converged = converged
+            if converged:
                 break
             if self.max_iter is not None and self.iterations >= self.max_iter:
                 break
@@ -149,4 +159,6 @@ class RestrictedSCF:
             orbital_energies, orbital_coefficients = self.linear_algebra.diagonalize(fock)
         self.density = density
         self.fock = fock
+        self.orbital_energies = orbital_energies
+        self.orbital_coefficients = orbital_coefficients
         return total, orbital_energies, orbital_coefficients

@@ -118,6 +118,19 @@ int gx_eri_dense(gx_ctx* ctx, double* out_dev);
 int gx_jk(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double cutoff,
           double fp32_bound, int flags);
 
+/* Nuclear gradient of the closed-shell RHF energy, dE/dR (3 natom doubles,
+ * device buffer, atom-major), from the converged density P and the
+ * energy-weighted density W = 2 sum_occ e_i C_i C_i^T (nbf x nbf device
+ * buffers):
+ *   dE/dX = sum P d(T + V)/dX - sum W dS/dX
+ *         + sum over canonical quartets f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc) d(ab|cd)/dX
+ *         + dE_nuc/dX
+ * Shell quartets with Q_ab Q_cd < cutoff are skipped.  Needs gx_set_nuclei;
+ * every shell's atom index must name a nucleus.  The reference has no
+ * analytic gradient: it replaces the numerical differentiation of energies
+ * by its optimiser (geometryoptimization/nelder_mead.py:126-144). */
+int gx_grad(gx_ctx* ctx, const double* P_dev, const double* W_dev, double* grad_dev, double cutoff);
+
 /* Shell-quartet and flop counts that gx_jk would evaluate (host-side count). */
 int gx_jk_stats(gx_ctx* ctx, double cutoff, double fp32_bound, int flags, gx_stats* out);
 

@@ -82,6 +82,121 @@ static void quartet(const Shell& A, const Shell& B, const Shell& C, const Shell&
   for (int t = 0; t < K::NOUT; ++t) out[t] = o[t];
 }
 
+template <class G>
+static bool run_hrr(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* H, bool* hok);
+
+// gradient programs keep one zero slot right after the accumulators
+template <class G, class = void>
+struct has_zslot {
+  static constexpr int value = 0;
+};
+template <class G>
+struct has_zslot<G, decltype((void)G::ZSLOT)> {
+  static constexpr int value = 1;
+};
+template <class G>
+constexpr int zero_slots() {
+  return has_zslot<G>::value;
+}
+
+// VRR of one primitive quartet by the program's levels into T (every source
+// written at an earlier level, every slot written once).
+template <class G>
+static bool run_vrr(const double* F, const double* PA, const double* QC, const double* WP, const double* WQ,
+                    double p, double q, double rp, double rq, double* T, bool* ok) {
+  for (int t = 0; t < G::NT; ++t) ok[t] = false;
+  T[0] = 0.0;
+  ok[0] = true;
+  for (int m = 0; m <= G::L; ++m) {
+    T[1 + m] = F[m];
+    ok[1 + m] = true;
+  }
+  const int4* vp = G::vrr();
+  const int* vl = G::vlev();
+  for (int lev = 1; lev <= G::L; ++lev) {
+    for (int t = vl[lev]; t < vl[lev + 1]; ++t) {
+      const unsigned y = (unsigned)vp[t].y, z = (unsigned)vp[t].z;
+      const unsigned src[5] = {y & 0xffffu, y >> 16, z & 0xffffu, z >> 16, (unsigned)vp[t].w};
+      for (unsigned v : src)
+        if (v >= (unsigned)G::NT || !ok[v]) return false;
+    }
+    for (int t = vl[lev]; t < vl[lev + 1]; ++t) {
+      const unsigned x = (unsigned)vp[t].x, y = (unsigned)vp[t].y, z = (unsigned)vp[t].z;
+      const bool ket = (x >> 16) & 1u;
+      const unsigned ax = (x >> 17) & 3u;
+      const double k1 = (double)((x >> 20) & 15u), k2 = (double)((x >> 24) & 15u);
+      const double cA = ket ? QC[ax] : PA[ax], cB = ket ? WQ[ax] : WP[ax];
+      const double o = ket ? 0.5 / q : 0.5 / p, r = ket ? rq : rp;
+      const unsigned dst = x & 0xffffu;
+      if (dst >= (unsigned)G::NT || ok[dst]) return false;
+      T[dst] = cA * T[y & 0xffffu] + cB * T[y >> 16] + k1 * o * (T[z & 0xffffu] - r * T[z >> 16]) +
+               k2 * (0.5 / (p + q)) * T[vp[t].w];
+    }
+    for (int t = vl[lev]; t < vl[lev + 1]; ++t) ok[(unsigned)vp[t].x & 0xffffu] = true;
+  }
+  return true;
+}
+
+// The gradient program of the class (EriGradProg): for every component k and
+// (centre X in A, B, C; axis i), d/dX_i of the contracted quartet, k-major.
+template <int LA, int LB, int LC, int LD>
+static int quartet_grad(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
+  using G = EriGradProg<LA, LB, LC, LD>;
+  static double T[G::NT], H[G::NH + G::NACC + G::NT];
+  static bool ok[G::NT], hok[G::NH + 1];
+  for (int t = 0; t < G::NACC + 1; ++t) H[t] = 0.0;  // slot NACC is the zero slot
+  const double two_pi_52 = 34.98683665524972;
+  for (int i = 0; i < A.n; ++i)
+    for (int j = 0; j < B.n; ++j) {
+      const double a = A.a[i], b = B.a[j], p = a + b;
+      double P[3], ab2 = 0;
+      for (int x = 0; x < 3; ++x) {
+        P[x] = (a * A.c[x] + b * B.c[x]) / p;
+        ab2 += (A.c[x] - B.c[x]) * (A.c[x] - B.c[x]);
+      }
+      const double Kab = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2);
+      for (int k = 0; k < C.n; ++k)
+        for (int l = 0; l < D.n; ++l) {
+          const double c = C.a[k], d = D.a[l], q = c + d;
+          double Q[3], cd2 = 0, pq2 = 0;
+          for (int x = 0; x < 3; ++x) {
+            Q[x] = (c * C.c[x] + d * D.c[x]) / q;
+            cd2 += (C.c[x] - D.c[x]) * (C.c[x] - D.c[x]);
+            pq2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+          }
+          const double Kcd = C.w[k] * D.w[l] * std::exp(-c * d / q * cd2);
+          const double rho = p * q / (p + q);
+          const double pref = two_pi_52 / (p * q * std::sqrt(p + q)) * Kab * Kcd;
+          double F[G::L + 1];
+          boys_host(G::L, rho * pq2, pref, F);
+          const double rp = rho / p, rq = rho / q;
+          const double PA[3] = {P[0] - A.c[0], P[1] - A.c[1], P[2] - A.c[2]};
+          const double QC[3] = {Q[0] - C.c[0], Q[1] - C.c[1], Q[2] - C.c[2]};
+          double WP[3], WQ[3];
+          for (int x = 0; x < 3; ++x) {
+            WP[x] = -rp * (P[x] - Q[x]);
+            WQ[x] = rq * (P[x] - Q[x]);
+          }
+          if (!run_vrr<G>(F, PA, QC, WP, WQ, p, q, rp, rq, T, ok)) return -2;
+          const double wt[4] = {1.0, 2.0 * a, 2.0 * b, 2.0 * c};
+          for (int t = 0; t < G::NACC; ++t) {
+            const unsigned e = G::acc()[t];
+            if (!ok[e & 0xffffu]) return -4;
+            H[t] += wt[e >> 16] * T[e & 0xffffu];
+          }
+        }
+    }
+  if (!run_hrr<G>(A, B, C, D, H, hok)) return -5;
+  if (H[G::ZSLOT] != 0.0) return -6;
+  for (int t = 0; t < 9 * G::NOUT; ++t) {
+    const int2 g = G::gterm()[t];
+    const unsigned plus = (unsigned)g.x, minus = (unsigned)g.y & 0xffffu, n = (unsigned)g.y >> 16;
+    if (!hok[plus] || !hok[minus]) return -7;
+    out[t] = H[plus] - n * H[minus];
+  }
+  return 0;
+}
+
 // The cooperative program of the class (EriProg), run entry by entry in level
 // order as the lanes of a wave run it.  Every source slot must have been
 // written before it is read (checked).
@@ -160,31 +275,52 @@ static int quartet_prog(const Shell& A, const Shell& B, const Shell& C, const Sh
           }
         }
     }
-  for (int t = 0; t < G::NH; ++t) hok[t] = t < G::NACC;
-  const double AB[3] = {A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2]};
-  const double CD[3] = {C.c[0] - D.c[0], C.c[1] - D.c[1], C.c[2] - D.c[2]};
-  const int2* hp = G::hrr();
-  const int* hl = G::hlev();
-  for (int lev = 0; lev < G::NHLEV; ++lev) {
-    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
-      const unsigned y = (unsigned)hp[t].y;
-      if ((y & 0xffffu) >= (unsigned)G::NH || (y >> 16) >= (unsigned)G::NH || !hok[y & 0xffffu] || !hok[y >> 16])
-        return -5;
-    }
-    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
-      const unsigned x = (unsigned)hp[t].x, y = (unsigned)hp[t].y;
-      const unsigned ax = (x >> 17) & 3u;
-      const double X = ((x >> 16) & 1u) ? CD[ax] : AB[ax];
-      if ((x & 0xffffu) >= (unsigned)G::NH || hok[x & 0xffffu]) return -6;
-      H[x & 0xffffu] = H[y & 0xffffu] + X * H[y >> 16];
-    }
-    for (int t = hl[lev]; t < hl[lev + 1]; ++t) hok[(unsigned)hp[t].x & 0xffffu] = true;
-  }
+  if (!run_hrr<G>(A, B, C, D, H, hok)) return -5;
   for (int t = 0; t < G::NOUT; ++t) {
     if (!hok[G::out()[t]]) return -7;
     out[t] = H[G::out()[t]];
   }
   return 0;
+}
+
+// HRR levels as the lanes run them: every source written before the level,
+// no slot both read and written within a level, distinct destinations.
+template <class G>
+static bool run_hrr(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* H, bool* hok) {
+  for (int t = 0; t < G::NH; ++t) hok[t] = t < G::NACC + zero_slots<G>();
+  const double AB[3] = {A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2]};
+  const double CD[3] = {C.c[0] - D.c[0], C.c[1] - D.c[1], C.c[2] - D.c[2]};
+  const int2* hp = G::hrr();
+  const int* hl = G::hlev();
+  static double tmp[1 << 16];
+  static int mark[1 << 16];
+  static int stamp = 0;
+  for (int lev = 0; lev < G::NHLEV; ++lev) {
+    ++stamp;
+    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
+      const unsigned y = (unsigned)hp[t].y;
+      const unsigned s0 = y & 0xffffu, s1 = y >> 16;
+      if (s0 >= (unsigned)G::NH || s1 >= (unsigned)G::NH || !hok[s0] || !hok[s1]) return false;
+      mark[s0] = mark[s1] = stamp;
+    }
+    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
+      const unsigned d = (unsigned)hp[t].x & 0xffffu;
+      if (d >= (unsigned)G::NH || mark[d] == stamp || mark[d] == -stamp) return false;
+      mark[d] = -stamp;
+    }
+    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
+      const unsigned x = (unsigned)hp[t].x, y = (unsigned)hp[t].y;
+      const unsigned ax = (x >> 17) & 3u;
+      const double X = ((x >> 16) & 1u) ? CD[ax] : AB[ax];
+      tmp[t - hl[lev]] = H[y & 0xffffu] + X * H[y >> 16];
+    }
+    for (int t = hl[lev]; t < hl[lev + 1]; ++t) {
+      const unsigned d = (unsigned)hp[t].x & 0xffffu;
+      H[d] = tmp[t - hl[lev]];
+      hok[d] = true;
+    }
+  }
+  return true;
 }
 
 static void unpack(const int* n, const double* ctr, const double* a, const double* w, Shell* s) {
@@ -198,6 +334,17 @@ static void unpack(const int* n, const double* ctr, const double* a, const doubl
     s[k].w = w + off;
     off += n[k];
   }
+}
+
+extern "C" int host_quartet_grad(const int* l, const double* ctr, const int* n, const double* a,
+                                 const double* w, double* out) {
+  Shell s[4];
+  unpack(n, ctr, a, w, s);
+#define GX_CASE(A, B, C, D) \
+  if (l[0] == A && l[1] == B && l[2] == C && l[3] == D) return quartet_grad<A, B, C, D>(s[0], s[1], s[2], s[3], out);
+  GX_FOR_EACH_CLASS(GX_CASE)
+#undef GX_CASE
+  return -1;
 }
 
 // Every class through its cooperative program (returns < 0 on a malformed program).

@@ -34,6 +34,7 @@ def host_lib():
     lib = ctypes.CDLL(so)
     lib.host_quartet.restype = ctypes.c_int
     lib.host_quartet_prog.restype = ctypes.c_int
+    lib.host_quartet_grad.restype = ctypes.c_int
     return lib
 
 
@@ -109,3 +110,59 @@ def test_generated_class_matches_oracle(host_lib, cls, entry):
         ref = _oracle(shells)
         scale = max(1.0, np.abs(ref).max())
         assert np.abs(out - ref).max() <= 1e-10 * scale, (cls, trial, np.abs(out - ref).max())
+
+
+GRAD_CLASSES = [c for c in CLASSES if sum(c) <= 5] + [(2, 1, 2, 1), (2, 2, 2, 2)]
+
+
+@pytest.mark.parametrize('cls', GRAD_CLASSES, ids=lambda c: '%d%d%d%d' % c)
+def test_gradient_program_matches_finite_differences(host_lib, cls):
+    """d/dX_i (ab|cd) for X = A, B, C from the gradient program (raised and
+    lowered classes with 2 alpha weights) against a 4-point central difference
+    of the oracle's integrals with the centre displaced (h = 1e-3, error
+    O(h^4))."""
+    host_lib.host_quartet_grad.restype = ctypes.c_int
+    rng = np.random.default_rng(sum(v * 10 ** i for i, v in enumerate(cls)) + 31)
+    centers = rng.uniform(-1.2, 1.2, (4, 3))
+    shells = [_shell(rng, l, c) for l, c in zip(cls, centers)]
+    l = np.array([s[0] for s in shells], dtype=np.int32)
+    ctr = np.ascontiguousarray(np.concatenate([s[1] for s in shells]))
+    n = np.array([len(s[2]) for s in shells], dtype=np.int32)
+    a = np.ascontiguousarray(np.concatenate([s[2] for s in shells]))
+    w = np.ascontiguousarray(np.concatenate([s[3] for s in shells]))
+    nout = int(np.prod([len(COMPS[x]) for x in cls]))
+    out = np.zeros(9 * nout)
+    rc = host_lib.host_quartet_grad(l.ctypes.data_as(ctypes.c_void_p), ctr.ctypes.data_as(ctypes.c_void_p),
+                                    n.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
+                                    w.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    got = out.reshape(nout, 3, 3)
+    h = 1e-3
+    scale = max(1.0, np.abs(_oracle(shells)).max())
+    for X in range(3):
+        for i in range(3):
+            vals = []
+            for step in (2, 1, -1, -2):
+                sh = [list(s) for s in shells]
+                c = sh[X][1].copy()
+                c[i] += step * h
+                sh[X][1] = c
+                vals.append(_oracle([tuple(s) for s in sh]))
+            fd = (8 * (vals[1] - vals[2]) - (vals[0] - vals[3])) / (12 * h)
+            err = np.abs(got[:, X, i] - fd).max()
+            assert err <= 1e-8 * scale, (cls, X, i, err)
+    # translational invariance fixes the D centre: the three derivatives must
+    # not sum to zero unless D is on one of the other centres, so only check
+    # that the D derivative from a finite difference equals -(A + B + C)
+    fdD = []
+    for i in range(3):
+        vals = []
+        for step in (2, 1, -1, -2):
+            sh = [list(s) for s in shells]
+            c = sh[3][1].copy()
+            c[i] += step * h
+            sh[3][1] = c
+            vals.append(_oracle([tuple(s) for s in sh]))
+        fdD.append((8 * (vals[1] - vals[2]) - (vals[0] - vals[3])) / (12 * h))
+    for i in range(3):
+        assert np.abs(-got[:, :, i].sum(axis=1) - fdD[i]).max() <= 1e-8 * scale

@@ -1,0 +1,174 @@
+"""Analytic RHF nuclear gradient (gx_grad) on the GPU.
+
+* Integral derivatives, exactly: for any symmetric P and W the kernel
+  returns the derivative at fixed P, W of
+      f(R) = tr(P (T + V)) - tr(W S) + 1/2 tr(P J(P)) - 1/4 tr(P K(P)) + E_nuc,
+  so it is compared with a 4-point central difference (h = 1e-3, error
+  O(h^4)) of f built from the oracle's S, T, V and dense ERIs (exact Boys
+  function) at displaced geometries.  Bar 1e-8.
+* The 66-function cluster (all 36 coordinates) against the same difference
+  built from the GPU energy kernels; bar 1e-7.
+* The converged gradient against central differences of GPU SCF energies
+  (h = 1e-4) on C2H4/3-21G; bar 1e-6 Eh/bohr.  (The seeded random clusters
+  are not used here: the reference's DIIS does not converge on them.)
+* Translational and rotational invariance; torch.autograd through RHFEnergy.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _displaced(basis, nuclei, atom, axis, h):
+    """Basis and nuclei with nucleus `atom` (and the functions on it) moved by h along `axis`."""
+    from gaussium_amd.basis import Basis, Nuclei, PrimitiveBasis
+    c0 = np.array(nuclei[atom].coordinates)
+    d = np.zeros(3)
+    d[axis] = h
+
+    def mv(c):
+        c = np.array(c)
+        return tuple(c + d) if np.abs(c - c0).sum() < 1e-12 else tuple(c)
+
+    out = []
+    for b in basis:
+        prims = [PrimitiveBasis(g.contraction, g.exponent, mv(g.coordinates), g.integral_exponents)
+                 for g in b.primitive_gaussian_array]
+        out.append(Basis(prims, mv(b.coordinates), b.integral_exponents))
+    nuc = [Nuclei(n.element, n.charge, n.mass, mv(n.coordinates) if k == atom else n.coordinates)
+           for k, n in enumerate(nuclei)]
+    return out, nuc
+
+
+def _f_fixed(basis, nuclei, P, W):
+    from oracle import eri_c
+    from oracle.basis import nuclear_repulsion
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from oracle.scf import jk_from_dense
+    S = overlap_matrix(basis)
+    T = kinetic_matrix(basis)
+    V = nuclear_matrix(basis, nuclei)
+    with eri_c.exact_boys():
+        eri = eri_c.eri_dense(basis)
+    J, K = jk_from_dense(eri, P)
+    return (np.sum(P * (T + V)) - np.sum(W * S) + 0.5 * np.sum(P * J) - 0.25 * np.sum(P * K)
+            + nuclear_repulsion(nuclei))
+
+
+def _check_fixed_density(basis, nuclei, coords, seed):
+    from gaussium_amd.engine import Engine
+    n = len(basis)
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    P = (a + a.T) / n
+    b = rng.standard_normal((n, n))
+    W = (b + b.T) / n
+    eng = Engine(basis, nuclei)
+    g = eng.grad(torch.from_numpy(P).cuda(), torch.from_numpy(W).cuda()).cpu().numpy()
+    h = 1e-3
+    for atom, axis in coords:
+        vals = [_f_fixed(*_displaced(basis, nuclei, atom, axis, s * h), P, W) for s in (2, 1, -1, -2)]
+        fd = (8 * (vals[1] - vals[2]) - (vals[0] - vals[3])) / (12 * h)
+        assert abs(g[atom, axis] - fd) < 1e-8, (atom, axis, g[atom, axis], fd)
+
+
+def test_gradient_fixed_density_triatomic():
+    from gaussium_amd import workloads
+    nuc, basis, ne, info = workloads.triatomic_7bf()
+    _check_fixed_density(basis, nuc, [(a, x) for a in range(3) for x in range(3)], 1)
+
+
+def test_gradient_fixed_density_with_d_shells():
+    """Contracted S/L/P/D shells on three nuclei: every class up to (dd|dd)."""
+    from gaussium_amd.basis import Nuclei
+    from tests.test_gpu_dshell import _d_basis
+    basis = _d_basis()
+    nuc = [Nuclei('A', 3.0, 6.0, (0.0, 0.0, 0.0)), Nuclei('B', 2.0, 4.0, (1.3, -0.4, 0.6)),
+           Nuclei('C', 1.0, 1.0, (-0.5, 1.1, -0.9))]
+    _check_fixed_density(basis, nuc, [(0, 0), (1, 2), (2, 1)], 2)
+
+
+def _f_fixed_gpu(eng, P, W):
+    """f(R) of the fixed-density check from the GPU energy kernels at the engine's geometry."""
+    from gaussium_amd.basis import nuclear_repulsion
+    S, T, V = eng.one_electron()
+    J, K = eng.jk(P)
+    return float(torch.sum(P * (T + V)) - torch.sum(W * S) + 0.5 * torch.sum(P * J) - 0.25 * torch.sum(P * K)) + \
+        nuclear_repulsion(eng.nuclei)
+
+
+def test_gradient_fixed_density_cluster_66bf_all_coordinates():
+    """The 12-atom, 66-function cluster: all 36 coordinates against a 4-point
+    difference of f(R) from the GPU energy kernels (S, T, V, J, K), which share
+    no code with the derivative kernels (the oracle comparison is done on the
+    smaller systems above: its pure-Python one-electron integrals take minutes
+    at 66 functions)."""
+    from gaussium_amd import workloads
+    from gaussium_amd.basis import Nuclei, pack_shells
+    from gaussium_amd.engine import Engine
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    n = len(basis)
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((n, n))
+    P = torch.from_numpy((a + a.T) / n).cuda()
+    b = rng.standard_normal((n, n))
+    W = torch.from_numpy((b + b.T) / n).cuda()
+    eng = Engine(basis, nuc)
+    g = eng.grad(P, W).cpu().numpy()
+    sh0 = pack_shells(basis, nuc)
+    xyz0 = np.array([x.coordinates for x in nuc])
+    h = 1e-3
+    for atom in range(len(nuc)):
+        for axis in range(3):
+            vals = []
+            for st in (2, 1, -1, -2):
+                xyz = xyz0.copy()
+                xyz[atom, axis] += st * h
+                eng.set_basis(sh0.moved(xyz0, xyz))
+                eng.set_nuclei([Nuclei(q.element, q.charge, q.mass, tuple(x)) for q, x in zip(nuc, xyz)])
+                vals.append(_f_fixed_gpu(eng, P, W))
+            fd = (8 * (vals[1] - vals[2]) - (vals[0] - vals[3])) / (12 * h)
+            assert abs(g[atom, axis] - fd) < 1e-7, (atom, axis, g[atom, axis], fd)
+
+
+@pytest.fixture(scope='module')
+def surface_c2h4():
+    """C2H4 / 3-21G (the reference's own RHF test system), which converges."""
+    from gaussium_amd.gradient import RHFSurface
+    from tests.helpers import load_system
+    nuc, basis, ne = load_system('C2H4.mol', '3-21G.gbs')
+    return RHFSurface(nuc, basis, ne)
+
+
+def test_gradient_matches_scf_finite_differences(surface_c2h4):
+    """Converged-SCF gradient against central differences of GPU SCF energies
+    (h = 1e-4 bohr) on every coordinate of C2H4."""
+    s = surface_c2h4
+    e0, g = s.energy_and_gradient(s.xyz0)
+    h = 1e-4
+    for atom in range(len(s.nuclei)):
+        for axis in range(3):
+            xp, xm = s.xyz0.copy(), s.xyz0.copy()
+            xp[atom, axis] += h
+            xm[atom, axis] -= h
+            fd = (s.energy(xp) - s.energy(xm)) / (2 * h)
+            assert abs(g[atom, axis] - fd) < 1e-6, (atom, axis, g[atom, axis], fd)
+
+
+def test_gradient_invariances(surface_c2h4):
+    s = surface_c2h4
+    _, g = s.energy_and_gradient(s.xyz0)
+    assert np.abs(g.sum(axis=0)).max() < 1e-8
+    torque = np.cross(s.xyz0, g).sum(axis=0)
+    assert np.abs(torque).max() < 1e-7
+
+
+def test_autograd_function(surface_c2h4):
+    from gaussium_amd.gradient import RHFEnergy
+    s = surface_c2h4
+    xyz = torch.tensor(s.xyz0, dtype=torch.float64, requires_grad=True)
+    e = RHFEnergy.apply(xyz, s)
+    (gx,) = torch.autograd.grad(2.0 * e, xyz)
+    _, g = s.energy_and_gradient(s.xyz0)
+    assert torch.allclose(gx, 2.0 * torch.from_numpy(g), atol=1e-9, rtol=0)
