@@ -2,10 +2,13 @@
 """Benchmark: direct J/K build of a 40-atom, 220-function cluster on one MI355X.
 
 One step = one direct J/K build (all ERIs screened by Schwarz bounds,
-evaluated and digested into J and K) from a converged SCF density; inputs
-are resident in HBM.  The mixed-precision build (quartets with Schwarz bound
-< 1e-4 in FP32) is the headline; the plain FP64 build is timed in the same
-run, and the SCF energies of both modes are compared (|dE| < 1e-6 Eh).
+evaluated and digested into J and K) from an SCF density; inputs are
+resident in HBM.  The mixed-precision build (quartets with Schwarz bound
+< 1e-4 in FP32) is the headline when its energy error is below 1e-6 Eh; the
+plain FP64 build is timed in the same run.  The energy error is the RHF
+energy functional at one density with the mixed and the FP64 J/K (the
+seeded random cluster does not converge under the reference's DIIS, so
+SCF end points are not comparable; they are reported as they are).
 
 Prints one JSON line.  ``--gpus`` above 1 is not supported (one GPU per run).
 """
@@ -34,6 +37,7 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=12.0,
                     help='wall seconds of the sampled CPU baseline (0 to skip)')
     ap.add_argument('--seed', type=int, default=2)
+    ap.add_argument('--scf-iter', type=int, default=30, help='SCF iterations of the untimed set-up')
     ap.add_argument('--kernels-only', action='store_true',
                     help='counter runs: no SCF; time the J/K kernels on the core-guess density and run '
                          'one %d MiB device copy as the HBM-counter calibration' % (COPY_BYTES >> 20))
@@ -138,18 +142,27 @@ def main():
     eng = Engine(basis, nuc)
     e_nuc = nuclear_repulsion(nuc)
 
-    # Converged SCF energies in both modes (untimed set-up).
+    # SCF in both modes (untimed set-up).  The reference's DIIS does not
+    # converge on the seeded random clusters (the energy enters a cycle), so the
+    # precision criterion is taken at a fixed density: the RHF energy
+    # functional E[P] = 1/2 tr(P (H + F)) with F = H + J - K/2 from the FP64 and
+    # from the mixed build, at the density the FP64 SCF ends with.
     scf = {}
     for mode, kw in (('fp64', dict(precision='fp64')), ('mixed', dict(precision='mixed', fp32_bound=FP32_BOUND))):
         t0 = time.perf_counter()
-        hf = RestrictedHF(nuc, basis, ne, engine=eng, schwarz_cutoff=CUTOFF, max_iter=100, threshold=1e-9,
-                          **kw)
+        hf = RestrictedHF(nuc, basis, ne, engine=eng, schwarz_cutoff=CUTOFF, max_iter=args.scf_iter,
+                          threshold=1e-9, **kw)
         e = hf.energies()[0] + e_nuc
         scf[mode] = {'energy': e, 'iterations': hf.scf_method.iterations, 'wall_s': time.perf_counter() - t0,
-                     'density': hf.scf_method.density}
-    dE = abs(scf['mixed']['energy'] - scf['fp64']['energy'])
+                     'density': hf.scf_method.density, 'converged': bool(hf.scf_method.converged),
+                     'core_hamiltonian': hf.core_hamiltonian}
     P = scf['fp64']['density']
-
+    H = scf['fp64']['core_hamiltonian']
+    energy_at = {}
+    for mode, kw in (('fp64', {}), ('mixed', dict(fp32_bound=FP32_BOUND, mixed=True))):
+        J, K = eng.jk(P, cutoff=CUTOFF, **kw)
+        energy_at[mode] = 0.5 * float(torch.sum(P * (2.0 * H + J - 0.5 * K))) + e_nuc
+    dE = abs(energy_at['mixed'] - energy_at['fp64'])
     fp64 = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF)
     mixed = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
     st64 = eng.jk_stats(cutoff=CUTOFF)
@@ -198,11 +211,11 @@ def main():
         'jk_ms_fp64': fp64['median_ms'],
         'jk_ms_mixed': mixed['median_ms'],
         'mixed_speedup': fp64['median_ms'] / mixed['median_ms'],
-        'scf_energy_fp64': scf['fp64']['energy'],
-        'scf_energy_mixed': scf['mixed']['energy'],
         'dE_mixed_Eh': dE,
-        'scf_iterations': [scf['fp64']['iterations'], scf['mixed']['iterations']],
-        'scf_wall_s': [scf['fp64']['wall_s'], scf['mixed']['wall_s']],
+        'dE_definition': 'fixed density: |E[P] mixed J/K - E[P] FP64 J/K|, P = final FP64 SCF density',
+        'energy_at_density': energy_at,
+        'scf': {m: {'energy': scf[m]['energy'], 'iterations': scf[m]['iterations'],
+                    'converged': scf[m]['converged'], 'wall_s': scf[m]['wall_s']} for m in scf},
         'quartets': {'canonical_bf': st64['bf_quartets'], 'canonical_shell': st64['shell_quartets'],
                      'evaluated_bf_fp64_build': st64['bf_quartets_fp64'],
                      'evaluated_bf_mixed_build': [stmx['bf_quartets_fp64'], stmx['bf_quartets_fp32']],

@@ -140,6 +140,7 @@ struct gx_ctx {
   struct WorkSet {
     std::vector<int*> d_items, d_rng;
     std::vector<int> n_items;
+    std::vector<double> ipw;  // items per resident workgroup, per combo (autotuned)
     double cutoff = -1.0, fp32 = -1.0;
     int mixed = -1;
     void release() {
@@ -156,8 +157,21 @@ struct gx_ctx {
   std::vector<int> wg_per_cu_grad;
   int* d_atom = nullptr;
   // ket segments: per class, runs of pairs with the same primitive-count key
-  std::vector<int> seg_lo, seg_hi, cls_seg;  // cls_seg: NCLS + 1 offsets into seg_*
-  int *d_seg_lo = nullptr, *d_seg_hi = nullptr;
+  std::vector<int> seg_lo, seg_hi;
+  std::vector<int> cseg;  // [class][system] first segment, NCLS x (nsys + 1)
+  int *d_seg_lo = nullptr, *d_seg_hi = nullptr, *d_cseg = nullptr;
+  // systems of a batch (one system unless gx_set_basis_batch): shells
+  // [sys_shell[s], sys_shell[s + 1]), nuclei [sys_atom[s], sys_atom[s + 1]),
+  // nbf sn[s], first function sbfb[s], packed-matrix offset smb[s]
+  int nsys = 1;
+  std::vector<int> sys_shell, sys_atom, ssys, lbf, sn, sbfb;
+  std::vector<long> smb;
+  long mtot = 0;
+  int nmax = 0;
+  int *d_ssys = nullptr, *d_lbf = nullptr, *d_sn = nullptr, *d_sbfb = nullptr, *d_sys_atom = nullptr,
+      *d_atom_sys = nullptr;
+  long* d_smb = nullptr;
+  bool sys_atom_ok = false;  // d_sys_atom / d_atom_sys match the current basis and nuclei
   std::vector<int64_t> pre_pnp, pre_nab;     // prefix sums over pair ids
   std::vector<int> nab;                       // basis-function pairs per shell pair
   int num_cu = 256;
@@ -172,7 +186,9 @@ struct gx_ctx {
   std::vector<hipEvent_t> ev;
   std::vector<int> ev_combo;
 
-  gx::ShellView shells() const { return gx::ShellView{d_l, d_bf, d_ctr}; }
+  gx::ShellView shells() const { return gx::ShellView{d_l, d_bf, d_ctr, d_lbf, d_ssys}; }
+  gx::SysView systems() const { return gx::SysView{d_sn, d_smb}; }
+  int cseg_at(int cls, int sys) const { return cseg[(size_t)cls * (nsys + 1) + sys]; }
   gx::PairView pairs() const { return gx::PairView{d_pA, d_pB, d_ppoff, d_pnp, d_Q, d_pp64, d_pp32}; }
 
   void free_pairs() {
@@ -185,14 +201,24 @@ struct gx_ctx {
     dfree(&d_pp32);
     jk.release();
     gr.release();
+    jk.ipw.clear();
+    gr.ipw.clear();
     wg_per_cu.clear();
     wg_per_cu_grad.clear();
     dfree(&d_seg_lo);
     dfree(&d_seg_hi);
+    dfree(&d_cseg);
   }
   void free_all() {
     free_pairs();
     dfree(&d_atom);
+    dfree(&d_ssys);
+    dfree(&d_lbf);
+    dfree(&d_sn);
+    dfree(&d_sbfb);
+    dfree(&d_smb);
+    dfree(&d_sys_atom);
+    dfree(&d_atom_sys);
     dfree(&d_l);
     dfree(&d_np);
     dfree(&d_poff);
@@ -492,7 +518,8 @@ void for_each_bra_segment(const gx_ctx* c, int X, int Y, double cutoff, double f
   for (int i = 0; i < nb; ++i) {
     const int ip = b0 + i;
     const double QA = c->Q[ip];
-    for (int sg = c->cls_seg[Y]; sg < c->cls_seg[Y + 1]; ++sg) {
+    const int sy = c->ssys[c->pA[ip]];
+    for (int sg = c->cseg_at(Y, sy); sg < c->cseg_at(Y, sy + 1); ++sg) {
       const int lo = c->seg_lo[sg];
       int hi = c->seg_hi[sg];
       if (X == Y && hi > ip + 1) hi = ip + 1;
@@ -516,6 +543,118 @@ struct JKItem {
   int bpos, v0, v1, prims;
 };
 
+#ifndef GX_ITEM_OVERHEAD
+#define GX_ITEM_OVERHEAD 256.0
+#endif
+#ifndef GX_ITEMS_PER_WG
+#define GX_ITEMS_PER_WG 4.0
+#endif
+
+// Work items of one class combination (X|Y) into ws (see build_orders).
+int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double cutoff, double fp32, int mixed,
+                bool grad, double ipw) {
+  constexpr double kItemOverhead = GX_ITEM_OVERHEAD;  // setup + flush of one item
+  constexpr double kQuartetCost = 8.0;                // digestion of one shell quartet per lane
+  dfree(&ws.d_items[combo]);
+  dfree(&ws.d_rng[combo]);
+  ws.n_items[combo] = 0;
+  const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
+  const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
+  if (nb == 0 || nk == 0) return GX_OK;
+  std::vector<int> rng((size_t)nb * 2 * JK_MAXSEG, 0);
+  for_each_bra_segment(c, X, Y, cutoff, fp32, mixed, [&](int i, int ip, int lo, int, int n64, int n32) {
+    const int s0 = c->cseg_at(Y, c->ssys[c->pA[ip]]);
+    int sg = 0;
+    while (c->seg_lo[s0 + sg] != lo) ++sg;
+    rng[(size_t)i * 2 * JK_MAXSEG + sg] = n64;
+    rng[(size_t)i * 2 * JK_MAXSEG + JK_MAXSEG + sg] = n32;
+  });
+  // per bra: virtual layout and per-chunk cost (widest ket of the chunk).
+  // Per-lane classes: chunks of 64 kets, four waves per item; cooperative
+  // classes: one ket per chunk, one wave per item.
+  const bool coop = grad || combo_is_coop(X, Y);
+  const int csz = coop ? 1 : GX_WAVE;
+  const double waves = coop ? 1.0 : 4.0;
+  std::vector<std::vector<double>> chunk(nb);
+  double total = 0.0;
+  std::vector<int> vk;  // virtual index -> ket pair id (-1 = padding)
+  for (int i = 0; i < nb; ++i) {
+    const int* r = &rng[(size_t)i * 2 * JK_MAXSEG];
+    const int sy = c->ssys[c->pA[b0 + i]];
+    const int s0 = c->cseg_at(Y, sy), nseg = c->cseg_at(Y, sy + 1) - s0;
+    vk.clear();
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1) vk.resize((vk.size() + GX_WAVE - 1) / GX_WAVE * GX_WAVE, -1);
+      for (int sg = 0; sg < nseg; ++sg) {
+        const int start = c->seg_lo[s0 + sg] + (h ? r[sg] : 0);
+        for (int k = 0; k < r[h * JK_MAXSEG + sg]; ++k) vk.push_back(start + k);
+      }
+    }
+    while (!vk.empty() && vk.back() < 0) vk.pop_back();
+    const double nbp = c->pnp[b0 + i];
+    for (size_t v0 = 0; v0 < vk.size(); v0 += csz) {
+      int widest = 0;
+      for (size_t v = v0; v < std::min(vk.size(), v0 + csz); ++v)
+        if (vk[v] >= 0) widest = std::max(widest, c->pnp[vk[v]]);
+      chunk[i].push_back(nbp * widest + kQuartetCost);
+      total += chunk[i].back();
+    }
+  }
+  // item size: about `ipw` items per resident workgroup (four per CU
+  // assumed), each spread over the waves of its workgroup
+  const double slots = (double)c->num_cu * 4.0;
+  const double target = std::max(total / (slots * waves * ipw), kItemOverhead);
+  std::vector<JKItem> items;
+  std::vector<double> icost;
+  for (int i = 0; i < nb; ++i) {
+    const auto& ch = chunk[i];
+    if (ch.empty()) continue;
+    const int nbp = c->pnp[b0 + i];
+    double widest = 0.0;
+    for (double x : ch) widest = std::max(widest, x);
+    // primitive split so that one chunk of each part fits the target
+    const int nsplit = std::max(1, std::min(nbp, (int)std::ceil(widest / target)));
+    for (int sp = 0; sp < nsplit; ++sp) {
+      const int p0 = (int)((long)nbp * sp / nsplit), p1 = (int)((long)nbp * (sp + 1) / nsplit);
+      const double frac = (double)(p1 - p0) / nbp;
+      // runs of chunks of about target x waves
+      size_t k0 = 0;
+      double acc = 0.0;
+      for (size_t k = 0; k < ch.size(); ++k) {
+        acc += ch[k] * frac;
+        const bool last = k + 1 == ch.size();
+        if (last || (acc >= waves * target && k + 1 - k0 >= (size_t)waves)) {
+          JKItem it;
+          it.bpos = i;
+          it.v0 = (int)(k0 * csz);
+          it.v1 = (int)((k + 1) * csz);
+          it.prims = p0 | (p1 << 16);
+          items.push_back(it);
+          icost.push_back(acc / waves + kItemOverhead);
+          k0 = k + 1;
+          acc = 0.0;
+        }
+      }
+    }
+  }
+  std::vector<int> order(items.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return icost[a] > icost[b]; });
+  std::vector<int> flat(4 * items.size());
+  for (size_t k = 0; k < order.size(); ++k) {
+    const JKItem& it = items[order[k]];
+    flat[4 * k] = it.bpos;
+    flat[4 * k + 1] = it.v0;
+    flat[4 * k + 2] = it.v1;
+    flat[4 * k + 3] = it.prims;
+  }
+  ws.n_items[combo] = (int)items.size();
+  GX_TRY(upload(&ws.d_items[combo], flat, c->stream));
+  GX_TRY(upload(&ws.d_rng[combo], rng, c->stream));
+  GX_HIP(hipStreamSynchronize(c->stream));  // before the host vectors go out of scope
+  return GX_OK;
+}
+
 int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = false) {
   gx_ctx::WorkSet& ws = grad ? c->gr : c->jk;
   if (grad) fp32 = 0.0, mixed = 0;
@@ -524,114 +663,85 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = f
   ws.d_items.assign(combo_count(), nullptr);
   ws.d_rng.assign(combo_count(), nullptr);
   ws.n_items.assign(combo_count(), 0);
-#ifndef GX_ITEM_OVERHEAD
-#define GX_ITEM_OVERHEAD 256.0
-#endif
-#ifndef GX_ITEMS_PER_WG
-#define GX_ITEMS_PER_WG 4.0
-#endif
-  constexpr double kItemOverhead = GX_ITEM_OVERHEAD;  // setup + flush of one item
-  constexpr double kQuartetCost = 8.0;                // digestion of one shell quartet per lane
+  if ((int)ws.ipw.size() != combo_count()) ws.ipw.assign(combo_count(), GX_ITEMS_PER_WG);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo) {
-      const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
-      const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
-      if (nb == 0 || nk == 0) continue;
-      const int s0 = c->cls_seg[Y], nseg = c->cls_seg[Y + 1] - s0;
-      std::vector<int> rng((size_t)nb * 2 * nseg, 0);
-      for_each_bra_segment(c, X, Y, cutoff, fp32, mixed, [&](int i, int, int lo, int, int n64, int n32) {
-        int sg = 0;
-        while (c->seg_lo[s0 + sg] != lo) ++sg;
-        rng[(size_t)i * 2 * nseg + sg] = n64;
-        rng[(size_t)i * 2 * nseg + nseg + sg] = n32;
-      });
-      // per bra: virtual layout and per-chunk cost (widest ket of the chunk).
-      // Per-lane classes: chunks of 64 kets, four waves per item; cooperative
-      // classes: one ket per chunk, one wave per item.
-      const bool coop = grad || combo_is_coop(X, Y);
-      const int csz = coop ? 1 : GX_WAVE;
-      const double waves = coop ? 1.0 : 4.0;
-      std::vector<std::vector<double>> chunk(nb);
-      double total = 0.0;
-      std::vector<int> vk;  // virtual index -> ket pair id (-1 = padding)
-      for (int i = 0; i < nb; ++i) {
-        const int* r = &rng[(size_t)i * 2 * nseg];
-        vk.clear();
-        for (int h = 0; h < 2; ++h) {
-          if (h == 1) vk.resize((vk.size() + GX_WAVE - 1) / GX_WAVE * GX_WAVE, -1);
-          for (int sg = 0; sg < nseg; ++sg) {
-            const int start = c->seg_lo[s0 + sg] + (h ? r[sg] : 0);
-            for (int k = 0; k < r[h * nseg + sg]; ++k) vk.push_back(start + k);
-          }
-        }
-        while (!vk.empty() && vk.back() < 0) vk.pop_back();
-        const double nbp = c->pnp[b0 + i];
-        for (size_t v0 = 0; v0 < vk.size(); v0 += csz) {
-          int widest = 0;
-          for (size_t v = v0; v < std::min(vk.size(), v0 + csz); ++v)
-            if (vk[v] >= 0) widest = std::max(widest, c->pnp[vk[v]]);
-          chunk[i].push_back(nbp * widest + kQuartetCost);
-          total += chunk[i].back();
-        }
-      }
-      // item size: about GX_ITEMS_PER_WG items per resident workgroup (four
-      // per CU assumed), each spread over the waves of its workgroup
-      const double slots = (double)c->num_cu * 4.0;
-      const double target = std::max(total / (slots * waves * GX_ITEMS_PER_WG), kItemOverhead);
-      std::vector<JKItem> items;
-      std::vector<double> icost;
-      for (int i = 0; i < nb; ++i) {
-        const auto& ch = chunk[i];
-        if (ch.empty()) continue;
-        const int nbp = c->pnp[b0 + i];
-        double widest = 0.0;
-        for (double x : ch) widest = std::max(widest, x);
-        // primitive split so that one chunk of each part fits the target
-        const int nsplit = std::max(1, std::min(nbp, (int)std::ceil(widest / target)));
-        for (int sp = 0; sp < nsplit; ++sp) {
-          const int p0 = (int)((long)nbp * sp / nsplit), p1 = (int)((long)nbp * (sp + 1) / nsplit);
-          const double frac = (double)(p1 - p0) / nbp;
-          // runs of chunks of about target x waves
-          size_t k0 = 0;
-          double acc = 0.0;
-          for (size_t k = 0; k < ch.size(); ++k) {
-            acc += ch[k] * frac;
-            const bool last = k + 1 == ch.size();
-            if (last || (acc >= waves * target && k + 1 - k0 >= (size_t)waves)) {
-              JKItem it;
-              it.bpos = i;
-              it.v0 = (int)(k0 * csz);
-              it.v1 = (int)((k + 1) * csz);
-              it.prims = p0 | (p1 << 16);
-              items.push_back(it);
-              icost.push_back(acc / waves + kItemOverhead);
-              k0 = k + 1;
-              acc = 0.0;
-            }
-          }
-        }
-      }
-      std::vector<int> order(items.size());
-      std::iota(order.begin(), order.end(), 0);
-      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return icost[a] > icost[b]; });
-      std::vector<int> flat(4 * items.size());
-      for (size_t k = 0; k < order.size(); ++k) {
-        const JKItem& it = items[order[k]];
-        flat[4 * k] = it.bpos;
-        flat[4 * k + 1] = it.v0;
-        flat[4 * k + 2] = it.v1;
-        flat[4 * k + 3] = it.prims;
-      }
-      ws.n_items[combo] = (int)items.size();
-      GX_TRY(upload(&ws.d_items[combo], flat, c->stream));
-      GX_TRY(upload(&ws.d_rng[combo], rng, c->stream));
-      GX_HIP(hipStreamSynchronize(c->stream));  // before the host vectors go out of scope
-    }
+    for (int Y = 0; Y <= X; ++Y, ++combo) GX_TRY(build_combo(c, ws, X, Y, combo, cutoff, fp32, mixed, grad, ws.ipw[combo]));
   ws.cutoff = cutoff;
   ws.fp32 = fp32;
   ws.mixed = mixed;
   return GX_OK;
+}
+
+// Nucleus ranges of the systems on the device (the single-system default
+// covers every nucleus) and each nucleus's system.
+int upload_sys_atoms(gx_ctx* c) {
+  if (c->sys_atom_ok) return GX_OK;  // unchanged since the last upload (keeps graph capture legal)
+  std::vector<int> sa(c->sys_atom);
+  if (sa.size() == 2 && sa[1] < 0) sa[1] = c->natom;
+  if ((int)sa.size() != c->nsys + 1 || sa[0] != 0 || sa[c->nsys] != c->natom)
+    return fail(GX_ERR_STATE, "system nucleus ranges do not cover the " + std::to_string(c->natom) + " nuclei");
+  std::vector<int> as(c->natom, 0);
+  for (int k = 0; k < c->nsys; ++k)
+    for (int a = sa[k]; a < sa[k + 1]; ++a) as[a] = k;
+  for (int sh = 0; sh < c->nshell; ++sh) {
+    const int at = c->atom[sh], k = c->ssys[sh];
+    if (at >= 0 && (at < sa[k] || at >= sa[k + 1]))
+      return fail(GX_ERR_STATE, "shell " + std::to_string(sh) + " sits on a nucleus of another system");
+  }
+  GX_TRY(upload(&c->d_sys_atom, sa, c->stream));
+  GX_TRY(upload(&c->d_atom_sys, as, c->stream));
+  GX_HIP(hipStreamSynchronize(c->stream));
+  c->sys_atom_ok = true;
+  return GX_OK;
+}
+
+gx::OneArgs one_args(const gx_ctx* c) {
+  gx::OneArgs a;
+  a.npair = c->npair;
+  a.pA = c->d_pA;
+  a.pB = c->d_pB;
+  a.sl = c->d_l;
+  a.snp = c->d_np;
+  a.spoff = c->d_poff;
+  a.sbf = c->d_bf;
+  a.slbf = c->d_lbf;
+  a.ssys = c->d_ssys;
+  a.ctr1 = c->d_ctr1;
+  a.exps = c->d_exps;
+  a.coefs = c->d_coefs;
+  a.scale = c->d_scale;
+  a.sys_atom = c->d_sys_atom;
+  a.nxyz = c->d_nxyz;
+  a.nZ = c->d_nZ;
+  a.master = c->d_boys;
+  a.sy = c->systems();
+  return a;
+}
+
+// The J/K kernel of class combination (X|Y) over its current work items.
+hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed) {
+  gx::JKArgs a;
+  a.sh = c->shells();
+  a.pr = c->pairs();
+  a.sy = c->systems();
+  a.bra0 = c->cls_start[X];
+  a.seg_lo = c->d_seg_lo;
+  a.cseg = c->d_cseg + (size_t)Y * (c->nsys + 1);
+  a.items = reinterpret_cast<const int4*>(c->jk.d_items[combo]);
+  a.nitem = c->jk.n_items[combo];
+  a.rng = c->jk.d_rng[combo];
+  a.master = c->d_boys;
+  a.tab64 = c->d_tab64;
+  a.tab32 = c->d_tab32;
+  a.P = c->d_Ps;
+  a.Jt = c->d_Jt;
+  a.Kt = c->d_Kt;
+  a.mixed = mixed;
+  if (c->wg_per_cu.empty()) c->wg_per_cu.assign(2 * combo_count(), 0);
+  return launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
+                   jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->nmax, mixed), c->stream, a,
+                   &c->wg_per_cu[2 * combo + mixed]);
 }
 
 }  // namespace
@@ -649,12 +759,10 @@ int gx_create(gx_ctx** out, int device) {
   GX_HIP(hipSetDevice(device));
   gx_ctx* c = new gx_ctx();
   c->device = device;
-  hipError_t e = hipStreamCreate(&c->own);
-  if (e != hipSuccess) {
-    delete c;
-    return fail(GX_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-  }
-  c->stream = c->own;
+  // the null stream until gx_set_stream: a context creates no stream (and
+  // so no hardware queue with its own scratch) of its own
+  c->own = nullptr;
+  c->stream = nullptr;
   int lds = 0, ncu = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
       lds > 0)
@@ -675,7 +783,6 @@ int gx_create(gx_ctx** out, int device) {
   if (rc == GX_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GX_ERR_HIP, "boys upload");
   if (rc != GX_OK) {
     c->free_all();
-    (void)hipStreamDestroy(c->own);
     delete c;
     return rc;
   }
@@ -688,7 +795,6 @@ int gx_destroy(gx_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   c->free_all();
-  if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
   return GX_OK;
 }
@@ -699,14 +805,23 @@ int gx_set_stream(gx_ctx* c, void* stream) {
   return GX_OK;
 }
 
-int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const int* prim_off,
-                 const int* bf_off, const double* ctr, const double* ctr1, const int* atom,
-                 int nprim_total, const double* exps, const double* coefs, int nbf,
-                 const double* bf_scale) {
+int gx_set_basis_batch(gx_ctx* c, int nsys, const int* sys_shell, const int* sys_atom, int nshell,
+                       const int* l, const int* nprim, const int* prim_off, const int* bf_off, const double* ctr,
+                       const double* ctr1, const int* atom, int nprim_total, const double* exps,
+                       const double* coefs, int nbf, const double* bf_scale) {
   if (!c) return fail(GX_ERR_ARG, "gx_set_basis: null context");
   if (nshell <= 0 || nbf <= 0 || nprim_total <= 0 || !l || !nprim || !prim_off || !bf_off || !ctr ||
       !ctr1 || !atom || !exps || !coefs || !bf_scale)
     return fail(GX_ERR_ARG, "gx_set_basis: null array or non-positive size");
+  if (nsys <= 0 || !sys_shell || sys_shell[0] != 0 || sys_shell[nsys] != nshell)
+    return fail(GX_ERR_ARG, "gx_set_basis_batch: system shell ranges must start at 0 and end at nshell");
+  for (int k = 0; k < nsys; ++k)
+    if (sys_shell[k + 1] <= sys_shell[k])
+      return fail(GX_ERR_ARG, "gx_set_basis_batch: system " + std::to_string(k) + " has no shells");
+  if (sys_atom)
+    for (int k = 0; k < nsys; ++k)
+      if (sys_atom[k] < 0 || sys_atom[k + 1] < sys_atom[k])
+        return fail(GX_ERR_ARG, "gx_set_basis_batch: system nucleus ranges must be non-decreasing");
   GX_HIP(hipSetDevice(c->device));
   int covered = 0;
   for (int s = 0; s < nshell; ++s) {
@@ -715,6 +830,8 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
                                   " not supported (max " + std::to_string(GX_ERI_LMAX) + ")");
     if (nprim[s] <= 0 || prim_off[s] < 0 || prim_off[s] + nprim[s] > nprim_total)
       return fail(GX_ERR_ARG, "gx_set_basis: primitive range of shell " + std::to_string(s));
+    if (nprim[s] > 65535)
+      return fail(GX_ERR_ARG, "gx_set_basis: too many primitives in shell " + std::to_string(s));
     if (bf_off[s] < 0 || bf_off[s] + ncart(l[s]) > nbf)
       return fail(GX_ERR_ARG, "gx_set_basis: basis-function range of shell " + std::to_string(s));
     covered += ncart(l[s]);
@@ -722,11 +839,44 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
   if (covered != nbf) return fail(GX_ERR_ARG, "gx_set_basis: shells do not cover nbf functions");
   for (int p = 0; p < nprim_total; ++p)
     if (!(exps[p] > 0.0)) return fail(GX_ERR_ARG, "gx_set_basis: non-positive exponent");
+  // systems: contiguous function ranges in system order
+  std::vector<int> sn(nsys, 0), sbfb(nsys, 0), ssys(nshell), lbf(nshell);
+  std::vector<long> smb(nsys + 1, 0);
+  int next_bf = 0, nmax = 0;
+  for (int k = 0; k < nsys; ++k) {
+    int lo = nbf, hi = 0;
+    for (int sh = sys_shell[k]; sh < sys_shell[k + 1]; ++sh) {
+      lo = std::min(lo, bf_off[sh]);
+      hi = std::max(hi, bf_off[sh] + ncart(l[sh]));
+      sn[k] += ncart(l[sh]);
+      ssys[sh] = k;
+    }
+    if (lo != next_bf || hi - lo != sn[k])
+      return fail(GX_ERR_ARG, "gx_set_basis_batch: functions of system " + std::to_string(k) +
+                                  " are not one contiguous range after the previous system's");
+    sbfb[k] = lo;
+    next_bf = hi;
+    smb[k + 1] = smb[k] + (long)sn[k] * sn[k];
+    nmax = std::max(nmax, sn[k]);
+    for (int sh = sys_shell[k]; sh < sys_shell[k + 1]; ++sh) lbf[sh] = bf_off[sh] - lo;
+  }
   GX_HIP(hipStreamSynchronize(c->stream));
   c->free_pairs();
   c->nshell = nshell;
   c->nbf = nbf;
   c->nprim = nprim_total;
+  c->nsys = nsys;
+  c->sys_atom_ok = false;
+  c->sys_shell.assign(sys_shell, sys_shell + nsys + 1);
+  if (sys_atom) c->sys_atom.assign(sys_atom, sys_atom + nsys + 1);
+  else c->sys_atom = {0, -1};  // one system with every nucleus
+  c->ssys.swap(ssys);
+  c->lbf.swap(lbf);
+  c->sn.swap(sn);
+  c->sbfb.swap(sbfb);
+  c->smb.assign(smb.begin(), smb.end() - 1);
+  c->mtot = smb[nsys];
+  c->nmax = nmax;
   c->l.assign(l, l + nshell);
   c->np.assign(nprim, nprim + nshell);
   c->poff.assign(prim_off, prim_off + nshell);
@@ -746,27 +896,32 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
   GX_TRY(upload(&c->d_exps, c->exps, c->stream));
   GX_TRY(upload(&c->d_coefs, c->coefs, c->stream));
   GX_TRY(upload(&c->d_scale, c->scale, c->stream));
-  dfree(&c->d_atom);
   GX_TRY(upload(&c->d_atom, c->atom, c->stream));
-  const size_t nn = (size_t)nbf * nbf;
-  GX_TRY(dalloc(&c->d_Ps, nn));
-  GX_TRY(dalloc(&c->d_Jt, nn));
-  GX_TRY(dalloc(&c->d_Kt, nn));
+  GX_TRY(upload(&c->d_ssys, c->ssys, c->stream));
+  GX_TRY(upload(&c->d_lbf, c->lbf, c->stream));
+  GX_TRY(upload(&c->d_sn, c->sn, c->stream));
+  GX_TRY(upload(&c->d_sbfb, c->sbfb, c->stream));
+  GX_TRY(upload(&c->d_smb, c->smb, c->stream));
+  GX_TRY(dalloc(&c->d_Ps, (size_t)c->mtot));
+  GX_TRY(dalloc(&c->d_Jt, (size_t)c->mtot));
+  GX_TRY(dalloc(&c->d_Kt, (size_t)c->mtot));
 
-  // Shell pairs, oriented l_A >= l_B, sorted by class then primitive-pair count.
+  // Shell pairs inside each system, oriented l_A >= l_B, sorted by class,
+  // system, then primitive-pair count.
   struct P {
-    int A, B, cls, kp;
+    int A, B, cls, sys, kp;
   };
   std::vector<P> pl;
-  pl.reserve((size_t)nshell * (nshell + 1) / 2);
-  for (int i = 0; i < nshell; ++i)
-    for (int j = 0; j <= i; ++j) {
-      int A = i, B = j;
-      if (c->l[j] > c->l[i]) std::swap(A, B);
-      pl.push_back(P{A, B, pair_class(c->l[A], c->l[B]), c->np[A] * c->np[B]});
-    }
+  for (int k = 0; k < nsys; ++k)
+    for (int i = sys_shell[k]; i < sys_shell[k + 1]; ++i)
+      for (int j = sys_shell[k]; j <= i; ++j) {
+        int A = i, B = j;
+        if (c->l[j] > c->l[i]) std::swap(A, B);
+        pl.push_back(P{A, B, pair_class(c->l[A], c->l[B]), k, c->np[A] * c->np[B]});
+      }
   std::stable_sort(pl.begin(), pl.end(), [](const P& a, const P& b) {
     if (a.cls != b.cls) return a.cls < b.cls;
+    if (a.sys != b.sys) return a.sys < b.sys;
     return a.kp > b.kp;
   });
   c->npair = (int)pl.size();
@@ -787,16 +942,19 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
   GX_TRY(compute_schwarz(c));
 
   // Segment key per pair: its primitive-pair count, or floor(log2) of it when
-  // a class has more than JK_MAXSEG distinct counts.  Within (class, key)
-  // pairs are ordered by decreasing Schwarz bound.
+  // a (class, system) has more than JK_MAXSEG distinct counts.  Within
+  // (class, system, key) pairs are ordered by decreasing Schwarz bound.
   std::vector<int> key(c->npair);
-  for (int k = 0; k < NCLS; ++k) {
+  for (int p0 = 0; p0 < c->npair;) {
+    int p1 = p0;
+    const int cls = c->pcls[p0], sy = c->ssys[c->pA[p0]];
+    while (p1 < c->npair && c->pcls[p1] == cls && c->ssys[c->pA[p1]] == sy) ++p1;
     std::vector<int> distinct;
-    for (int p = c->cls_start[k]; p < c->cls_start[k + 1]; ++p) distinct.push_back(c->pnp[p]);
+    for (int p = p0; p < p1; ++p) distinct.push_back(c->pnp[p]);
     std::sort(distinct.begin(), distinct.end());
     distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
     const bool coarse = (int)distinct.size() > JK_MAXSEG;
-    for (int p = c->cls_start[k]; p < c->cls_start[k + 1]; ++p) {
+    for (int p = p0; p < p1; ++p) {
       int kk = c->pnp[p];
       if (coarse) {
         int lg = 0;
@@ -805,11 +963,14 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
       }
       key[p] = kk;
     }
+    p0 = p1;
   }
   std::vector<int> perm(c->npair);
   std::iota(perm.begin(), perm.end(), 0);
   std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
     if (c->pcls[a] != c->pcls[b]) return c->pcls[a] < c->pcls[b];
+    const int sa = c->ssys[c->pA[a]], sb = c->ssys[c->pA[b]];
+    if (sa != sb) return sa < sb;
     if (key[a] != key[b]) return key[a] > key[b];
     return c->Q[a] > c->Q[b];
   });
@@ -833,22 +994,28 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
   GX_TRY(upload(&c->d_Q, c->Q, c->stream));
   c->seg_lo.clear();
   c->seg_hi.clear();
-  c->cls_seg.assign(NCLS + 1, 0);
-  for (int k = 0; k < NCLS; ++k) {
-    c->cls_seg[k] = (int)c->seg_lo.size();
-    for (int p = c->cls_start[k]; p < c->cls_start[k + 1];) {
-      int q = p;
-      while (q < c->cls_start[k + 1] && key[q] == key[p]) ++q;
-      c->seg_lo.push_back(p);
-      c->seg_hi.push_back(q);
-      p = q;
-    }
-    if ((int)c->seg_lo.size() - c->cls_seg[k] > JK_MAXSEG)
-      return fail(GX_ERR_STATE, "gx_set_basis: too many primitive-count segments");
+  c->cseg.assign((size_t)NCLS * (nsys + 1), 0);
+  {
+    int p = 0;
+    for (int k = 0; k < NCLS; ++k)
+      for (int sy = 0; sy <= nsys; ++sy) {
+        c->cseg[(size_t)k * (nsys + 1) + sy] = (int)c->seg_lo.size();
+        if (sy == nsys) break;
+        const int first = (int)c->seg_lo.size();
+        while (p < c->cls_start[k + 1] && c->ssys[c->pA[p]] == sy) {
+          int q = p;
+          while (q < c->cls_start[k + 1] && c->ssys[c->pA[q]] == sy && key[q] == key[p]) ++q;
+          c->seg_lo.push_back(p);
+          c->seg_hi.push_back(q);
+          p = q;
+        }
+        if ((int)c->seg_lo.size() - first > JK_MAXSEG)
+          return fail(GX_ERR_STATE, "gx_set_basis: too many primitive-count segments");
+      }
   }
-  c->cls_seg[NCLS] = (int)c->seg_lo.size();
   GX_TRY(upload(&c->d_seg_lo, c->seg_lo, c->stream));
   GX_TRY(upload(&c->d_seg_hi, c->seg_hi, c->stream));
+  GX_TRY(upload(&c->d_cseg, c->cseg, c->stream));
   c->nab.assign(c->npair, 0);
   c->pre_pnp.assign(c->npair + 1, 0);
   c->pre_nab.assign(c->npair + 1, 0);
@@ -858,13 +1025,25 @@ int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const in
     c->pre_pnp[p + 1] = c->pre_pnp[p] + c->pnp[p];
     c->pre_nab[p + 1] = c->pre_nab[p] + c->nab[p];
   }
-  c->pairmap.assign((size_t)nshell * nshell, -1);
-  for (int p = 0; p < c->npair; ++p) {
-    c->pairmap[(size_t)c->pA[p] * nshell + c->pB[p]] = p;
-    c->pairmap[(size_t)c->pB[p] * nshell + c->pA[p]] = p;
+  c->pairmap.clear();
+  if (nsys == 1) {
+    c->pairmap.assign((size_t)nshell * nshell, -1);
+    for (int p = 0; p < c->npair; ++p) {
+      c->pairmap[(size_t)c->pA[p] * nshell + c->pB[p]] = p;
+      c->pairmap[(size_t)c->pB[p] * nshell + c->pA[p]] = p;
+    }
   }
   GX_HIP(hipStreamSynchronize(c->stream));
   return GX_OK;
+}
+
+int gx_set_basis(gx_ctx* c, int nshell, const int* l, const int* nprim, const int* prim_off,
+                 const int* bf_off, const double* ctr, const double* ctr1, const int* atom,
+                 int nprim_total, const double* exps, const double* coefs, int nbf,
+                 const double* bf_scale) {
+  const int sys_shell[2] = {0, nshell};
+  return gx_set_basis_batch(c, 1, sys_shell, nullptr, nshell, l, nprim, prim_off, bf_off, ctr, ctr1, atom,
+                            nprim_total, exps, coefs, nbf, bf_scale);
 }
 
 int gx_set_nuclei(gx_ctx* c, int natom, const double* xyz, const double* charge) {
@@ -873,6 +1052,7 @@ int gx_set_nuclei(gx_ctx* c, int natom, const double* xyz, const double* charge)
   GX_HIP(hipSetDevice(c->device));
   GX_HIP(hipStreamSynchronize(c->stream));
   c->natom = natom;
+  c->sys_atom_ok = false;
   c->nxyz.assign(xyz, xyz + 3 * natom);
   c->nZ.assign(charge, charge + natom);
   GX_TRY(upload(&c->d_nxyz, c->nxyz, c->stream));
@@ -898,6 +1078,7 @@ int gx_schwarz(gx_ctx* c, double* Q, int* shellA, int* shellB) {
 int gx_eri_dense(gx_ctx* c, double* out) {
   if (!c || !out) return fail(GX_ERR_ARG, "gx_eri_dense: null argument");
   if (c->npair == 0) return fail(GX_ERR_STATE, "gx_eri_dense: no basis set");
+  if (c->nsys != 1) return fail(GX_ERR_STATE, "gx_eri_dense: one system only");
   GX_HIP(hipSetDevice(c->device));
   const long N = c->nbf;
   const long n4 = N * N * N * N;
@@ -919,6 +1100,7 @@ int gx_eri_dense(gx_ctx* c, double* out) {
 int gx_eri_quartets(gx_ctx* c, int64_t n, const int32_t* ijkl, double* out) {
   if (!c || (n > 0 && (!ijkl || !out))) return fail(GX_ERR_ARG, "gx_eri_quartets: null argument");
   if (c->npair == 0) return fail(GX_ERR_STATE, "gx_eri_quartets: no basis set");
+  if (c->nsys != 1) return fail(GX_ERR_STATE, "gx_eri_quartets: one system only");
   if (n == 0) return GX_OK;
   GX_HIP(hipSetDevice(c->device));
   std::vector<int> shell_of(c->nbf);
@@ -1009,8 +1191,8 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   if (c->npair == 0) return fail(GX_ERR_STATE, "gx_jk: no basis set");
   if (!(cutoff >= 0.0) || !(fp32_bound >= 0.0)) return fail(GX_ERR_ARG, "gx_jk: negative threshold");
   GX_HIP(hipSetDevice(c->device));
-  const int N = c->nbf;
-  const long nn = (long)N * N;
+  const int N = c->nmax;  // LDS rows are sized for the largest system
+  const long nn = c->mtot;
   const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y) {
@@ -1021,7 +1203,8 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
                                     ", needs " + std::to_string(lds) + " bytes)");
     }
   const unsigned g = (unsigned)((nn + 255) / 256);
-  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, P, c->d_Ps);
+  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, P, c->d_Ps);
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Jt, 0.0);
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
   GX_HIP(hipGetLastError());
@@ -1043,43 +1226,79 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
-      const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
-      const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
-      if (nb == 0 || nk == 0) continue;
       if (c->jk.n_items[combo] == 0) continue;
-      gx::JKArgs a;
-      a.sh = c->shells();
-      a.pr = c->pairs();
-      a.bra0 = b0;
-      a.ket0 = c->cls_start[Y];
-      a.same = (X == Y);
-      a.seg_lo = c->d_seg_lo + c->cls_seg[Y];
-      a.nseg = c->cls_seg[Y + 1] - c->cls_seg[Y];
-      a.items = reinterpret_cast<const int4*>(c->jk.d_items[combo]);
-      a.nitem = c->jk.n_items[combo];
-      a.rng = c->jk.d_rng[combo];
-      a.master = c->d_boys;
-      a.tab64 = c->d_tab64;
-      a.tab32 = c->d_tab32;
-      a.P = c->d_Ps;
-      a.Jt = c->d_Jt;
-      a.Kt = c->d_Kt;
-      a.N = N;
-      a.mixed = mixed;
       if (timing) {
         GX_HIP(next_events(combo));
         GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
       }
-      if (c->wg_per_cu.empty()) c->wg_per_cu.assign(2 * combo_count(), 0);
-      GX_HIP(launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
-                       jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), N, mixed), c->stream, a,
-                       &c->wg_per_cu[2 * combo + mixed]));
+      GX_HIP(launch_combo(c, X, Y, combo, mixed));
       if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
     }
-  hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, c->d_Jt,
-                     c->d_Kt, J, K);
+  hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, K);
   GX_HIP(hipGetLastError());
   return GX_OK;
+}
+
+int gx_jk_autotune(gx_ctx* c, const double* P, double cutoff, double fp32_bound, int flags,
+                   const double* cand, int ncand, int reps, double* out, int cap) {
+  if (!c || !P || !cand || !out || ncand <= 0 || reps <= 0) return fail(GX_ERR_ARG, "gx_jk_autotune: bad argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_jk_autotune: no basis set");
+  for (int k = 0; k < ncand; ++k)
+    if (!(cand[k] > 0.0)) return fail(GX_ERR_ARG, "gx_jk_autotune: candidates must be positive");
+  const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
+  // one untimed build: fills Ps, builds the default items and checks sizes
+  double *dJ = nullptr, *dK = nullptr;
+  GX_TRY(dalloc(&dJ, (size_t)c->mtot));
+  int rc = dalloc(&dK, (size_t)c->mtot);
+  if (rc == GX_OK) rc = gx_jk(c, P, dJ, dK, cutoff, fp32_bound, mixed);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == GX_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(GX_ERR_HIP, "gx_jk_autotune: hipEventCreate");
+  int row = 0, combo = 0;
+  for (int X = 0; X < NCLS && rc == GX_OK; ++X)
+    for (int Y = 0; Y <= X && rc == GX_OK; ++Y, ++combo) {
+      if (c->jk.n_items[combo] == 0) continue;
+      if (row >= cap) {
+        rc = fail(GX_ERR_ARG, "gx_jk_autotune: output too small");
+        break;
+      }
+      const double before = c->jk.ipw[combo];
+      double best = before, best_ms = 1e30, before_ms = 1e30;
+      for (int k = 0; k < ncand && rc == GX_OK; ++k) {
+        rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, cand[k]);
+        if (rc != GX_OK) break;
+        double ms_min = 1e30;
+        for (int r = 0; r < reps; ++r) {
+          float ms = 0.f;
+          if (hipEventRecord(e0, c->stream) != hipSuccess || launch_combo(c, X, Y, combo, mixed) != hipSuccess ||
+              hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(GX_ERR_HIP, "gx_jk_autotune: timed launch");
+            break;
+          }
+          ms_min = std::min(ms_min, (double)ms);
+        }
+        if (ms_min < best_ms) best_ms = ms_min, best = cand[k];
+        if (cand[k] == before) before_ms = ms_min;
+      }
+      if (rc != GX_OK) break;
+      c->jk.ipw[combo] = best;
+      rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, best);
+      double* r = out + 6 * row++;
+      r[0] = class_la(X);
+      r[1] = class_lb(X);
+      r[2] = class_la(Y);
+      r[3] = class_lb(Y);
+      r[4] = best;
+      r[5] = best_ms;
+      (void)before_ms;
+    }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(&dJ);
+  dfree(&dK);
+  return rc == GX_OK ? row : rc;
 }
 
 int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cutoff) {
@@ -1101,19 +1320,19 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
                                     ") needs " + std::to_string(lds) + " bytes of LDS, the device has " +
                                     std::to_string(c->max_lds));
     }
+  GX_TRY(upload_sys_atoms(c));
   GX_HIP(hipSetDevice(c->device));
-  const int N = c->nbf;
-  const long nn = (long)N * N;
+  const long nn = c->mtot;
   const unsigned g = (unsigned)((nn + 255) / 256);
-  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, N, c->d_scale, P, c->d_Ps);
+  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, P, c->d_Ps);
   hipLaunchKernelGGL(gx::k_fill, dim3((3 * c->natom + 255) / 256), dim3(256), 0, c->stream, (long)3 * c->natom,
                      grad, 0.0);
-  const int npair = c->nshell * (c->nshell + 1) / 2;
-  hipLaunchKernelGGL(gx::k_one_grad, dim3((npair + 63) / 64), dim3(64), 0, c->stream, c->nshell, c->d_l,
-                     c->d_np, c->d_poff, c->d_bf, c->d_atom, c->d_ctr1, c->d_exps, c->d_coefs, c->d_scale,
-                     c->natom, c->d_nxyz, c->d_nZ, c->d_boys, N, P, W, grad);
-  hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_nxyz,
-                     c->d_nZ, grad);
+  const gx::OneArgs oa = one_args(c);
+  hipLaunchKernelGGL(gx::k_one_grad, dim3((c->npair + 63) / 64), dim3(64), 0, c->stream, oa, c->d_atom, P, W,
+                     grad);
+  hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_atom_sys,
+                     c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, 0.0, 0, true));
   if (c->wg_per_cu_grad.empty()) c->wg_per_cu_grad.assign(combo_count(), 0);
@@ -1124,9 +1343,10 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
       gx::GradArgs a;
       a.sh = c->shells();
       a.pr = c->pairs();
+      a.sy = c->systems();
       a.bra0 = c->cls_start[X];
-      a.seg_lo = c->d_seg_lo + c->cls_seg[Y];
-      a.nseg = c->cls_seg[Y + 1] - c->cls_seg[Y];
+      a.seg_lo = c->d_seg_lo;
+      a.cseg = c->d_cseg + (size_t)Y * (c->nsys + 1);
       a.items = reinterpret_cast<const int4*>(c->gr.d_items[combo]);
       a.nitem = c->gr.n_items[combo];
       a.rng = c->gr.d_rng[combo];
@@ -1134,7 +1354,6 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
       a.P = c->d_Ps;
       a.atom = c->d_atom;
       a.grad = grad;
-      a.N = N;
       GX_HIP(launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, c->stream, a,
                          &c->wg_per_cu_grad[combo]));
     }
@@ -1233,11 +1452,11 @@ int gx_one_electron(gx_ctx* c, double* S, double* T, double* V) {
   if (!c || !S || !T) return fail(GX_ERR_ARG, "gx_one_electron: null argument");
   if (c->npair == 0) return fail(GX_ERR_STATE, "gx_one_electron: no basis set");
   if (V && c->natom == 0) return fail(GX_ERR_STATE, "gx_one_electron: no nuclei set");
+  if (c->natom > 0) GX_TRY(upload_sys_atoms(c));
   GX_HIP(hipSetDevice(c->device));
-  const int npair = c->nshell * (c->nshell + 1) / 2;
-  hipLaunchKernelGGL(gx::k_one_electron, dim3((npair + 63) / 64), dim3(64), 0, c->stream, c->nshell,
-                     c->d_l, c->d_np, c->d_poff, c->d_bf, c->d_ctr1, c->d_exps, c->d_coefs,
-                     c->d_scale, c->natom, c->d_nxyz, c->d_nZ, c->d_boys, c->nbf, S, T, V);
+  gx::OneArgs oa = one_args(c);
+  if (c->natom == 0) oa.sys_atom = nullptr;
+  hipLaunchKernelGGL(gx::k_one_electron, dim3((c->npair + 63) / 64), dim3(64), 0, c->stream, oa, S, T, V);
   GX_HIP(hipGetLastError());
   return GX_OK;
 }

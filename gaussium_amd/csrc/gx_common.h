@@ -47,8 +47,17 @@ struct __align__(16) PrimPair32 {
 // Device-side views passed to kernels by value.
 struct ShellView {
   const int* l;
-  const int* bf;        // first basis function
+  const int* bf;        // first basis function (index over all systems)
   const double* ctr;    // ERI centre, 3 per shell
+  const int* lbf;       // first basis function inside its system
+  const int* sys;       // system of the shell
+};
+
+// Systems of a batch: each has its own nbf x nbf matrices, packed one after
+// the other (P, J, K, S, T, V ...); quartets never mix systems.
+struct SysView {
+  const int* n;         // basis functions per system
+  const long* mb;       // offset of the system's matrix in the packed arrays
 };
 
 struct PairView {

@@ -207,14 +207,17 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
   double* __restrict__ jblk = pblk + NBLK;           // J_ab J_cd K_ac K_ad K_bc K_bd partials
   __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
   const int lane = threadIdx.x;
-  const int N = a.N;
-  const int nseg = a.nseg;
   const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
   for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
     const int A = a.pr.A[ip], B = a.pr.B[ip];
-    const int ao = a.sh.bf[A], bo = a.sh.bf[B];
+    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
+    const int sys = a.sh.sys[A];
+    const int N = a.sy.n[sys];
+    const long mb = a.sy.mb[sys];
+    const int s0 = a.cseg[sys];
+    const int nseg = a.cseg[sys + 1] - s0;
     const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
     const PrimPair64* __restrict__ bra = a.pr.pp64 + a.pr.poff[ip] + p0;
     const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
@@ -222,8 +225,8 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
     const double fA = (A == B) ? 0.5 : 1.0;
     __syncthreads();  // the previous item's readers of r_* are done
     if (lane < nseg) {
-      r_start[lane] = a.seg_lo[lane];
-      r_pre[lane + 1] = a.rng[(long)it.x * 2 * nseg + lane];  // FP64 counts; cooperative classes have no FP32 part
+      r_start[lane] = a.seg_lo[s0 + lane];
+      r_pre[lane + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + lane];  // FP64 counts; cooperative classes have no FP32 part
     }
     __syncthreads();
     if (lane == 0) {
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
       while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
       const int jp = r_start[r] + (v - r_pre[r]);
       const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
-      const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
+      const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
       coop_quartet<G>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
                       a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
                       a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], tab, H);
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
         else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
         else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
         else { u -= SBC; r0 = bo; c0 = dof; s = ND; }
-        pblk[t] = a.P[(long)(r0 + u / s) * N + c0 + u % s];
+        pblk[t] = a.P[mb + (long)(r0 + u / s) * N + c0 + u % s];
         jblk[t] = 0.0;
       }
       __syncthreads();
@@ -294,7 +297,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
         else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
         else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
         else { u -= SBC; r0 = bo; c0 = dof; s = ND; }
-        atomicAdd(&dst[(long)(r0 + u / s) * N + c0 + u % s], m * x);
+        atomicAdd(&dst[mb + (long)(r0 + u / s) * N + c0 + u % s], m * x);
       }
     }
   }
@@ -311,17 +314,17 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
 struct GradArgs {
   ShellView sh;
   PairView pr;
+  SysView sy;
   int bra0;
   const int* seg_lo;
-  int nseg;
+  const int* cseg;     // per system: first ket segment of the ket class
   const int4* items;
   int nitem;
   const int* rng;
   const double* tab64;
-  const double* P;     // density, pre-scaled by s_k s_l
-  const int* atom;     // shell -> atom
+  const double* P;     // packed densities, pre-scaled by s_k s_l
+  const int* atom;     // shell -> atom (over all systems)
   double* grad;        // 3 natom
-  int N;
 };
 
 template <class G>
@@ -340,15 +343,18 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
   double* __restrict__ pblk = smem + grad_lds<G>();
   __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
   const int lane = threadIdx.x;
-  const int N = a.N;
-  const int nseg = a.nseg;
   const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
   const int2* __restrict__ gt = G::gterm();
   for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
     const int A = a.pr.A[ip], B = a.pr.B[ip];
-    const int ao = a.sh.bf[A], bo = a.sh.bf[B];
+    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
+    const int sys = a.sh.sys[A];
+    const int N = a.sy.n[sys];
+    const long mb = a.sy.mb[sys];
+    const int s0 = a.cseg[sys];
+    const int nseg = a.cseg[sys + 1] - s0;
     const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
     const PrimPair64* __restrict__ bra = a.pr.pp64 + a.pr.poff[ip] + p0;
     const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
@@ -356,8 +362,8 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
     const double fA = (A == B) ? 0.5 : 1.0;
     __syncthreads();
     if (lane < nseg) {
-      r_start[lane] = a.seg_lo[lane];
-      r_pre[lane + 1] = a.rng[(long)it.x * 2 * nseg + lane];
+      r_start[lane] = a.seg_lo[s0 + lane];
+      r_pre[lane + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + lane];  // FP64 counts; cooperative classes have no FP32 part
     }
     __syncthreads();
     if (lane == 0) {
@@ -371,7 +377,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
       while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
       const int jp = r_start[r] + (v - r_pre[r]);
       const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
-      const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
+      const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
       coop_quartet<G, true>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
                             a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
                             a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], tab, H);
@@ -386,7 +392,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
         else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; st = ND; }
         else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; st = NC; }
         else { u -= SBC; r0 = bo; c0 = dof; st = ND; }
-        pblk[t] = a.P[(long)(r0 + u / st) * N + c0 + u % st];
+        pblk[t] = a.P[mb + (long)(r0 + u / st) * N + c0 + u % st];
       }
       __syncthreads();
       const double* pab = pblk;

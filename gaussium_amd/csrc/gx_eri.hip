@@ -360,19 +360,19 @@ struct JKShape {
 struct JKArgs {
   ShellView sh;
   PairView pr;
-  int bra0, ket0, same;
-  const int* seg_lo;     // ket segments of the ket class: first global pair id of each
-  int nseg;
+  SysView sy;
+  int bra0;
+  const int* seg_lo;     // ket segments: first global pair id of each
+  const int* cseg;       // per system: first ket segment of the ket class (nsys + 1)
   const int4* items;     // work items: bra position, [v0, v1), p0 | p1 << 16; largest first
   int nitem;
-  const int* rng;        // per bra position: surviving kets per segment, FP64 then FP32
+  const int* rng;        // per bra position, 2 JK_MAXSEG ints: surviving kets per segment, FP64 then FP32
   const double* master;  // Boys master table
   const double* tab64;   // per-L FP64 Taylor tables (BOYS_NT x BOYS_ROW64 each)
   const float* tab32;    // per-L FP32 Taylor tables
-  const double* P;       // density, pre-scaled by s_k s_l
-  double* Jt;
+  const double* P;       // packed densities, pre-scaled by s_k s_l
+  double* Jt;            // packed accumulators
   double* Kt;
-  int N;
   int mixed;
 };
 
@@ -389,15 +389,12 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
   __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1], r_len[2 * JK_MAXSEG];
   __shared__ double jab_s[NA * NB];
   __shared__ int next_chunk;
-  const int N = a.N;
   const int tid = threadIdx.x;
-  // dynamic LDS: Boys FP64 (pure FP64 build) | P rows | K rows
+  // dynamic LDS: Boys FP64 (pure FP64 build) | P rows | K rows (rows of the
+  // item's system)
   const double* __restrict__ tab64 = !MIXED ? smem : a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
   const float* __restrict__ tab32 = a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
   double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : 0);
-  double* krow = prow + (NA + NB) * N;
-  const int nseg = a.nseg;
-  const int nr = 2 * nseg;  // range r < nseg: FP64 part of segment r; else FP32 part of r - nseg
 
   if (!MIXED) boys_stage64(C::L, a.master, smem);
 
@@ -410,8 +407,17 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
     const int bpos = it.x;
     const int ip = a.bra0 + bpos;
     const int A = a.pr.A[ip], B = a.pr.B[ip];
-    const int ao = a.sh.bf[A], bo = a.sh.bf[B];
+    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
     const int rowB = (A == B) ? 0 : NA;
+    const int sys = a.sh.sys[A];
+    const int N = a.sy.n[sys];
+    const double* __restrict__ Pm = a.P + a.sy.mb[sys];
+    double* __restrict__ Jm = a.Jt + a.sy.mb[sys];
+    double* __restrict__ Km = a.Kt + a.sy.mb[sys];
+    double* krow = prow + (NA + NB) * N;
+    const int s0 = a.cseg[sys];
+    const int nseg = a.cseg[sys + 1] - s0;
+    const int nr = 2 * nseg;  // range r < nseg: FP64 part of segment r; else FP32 part of r - nseg
     // wave-uniform: kept in scalar registers so that the bra loop reads go
     // through the scalar data path
     const int p0 = __builtin_amdgcn_readfirstlane(it.w & 0xffff);
@@ -425,18 +431,18 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
     for (int i = tid; i < (NA + NB) * N; i += JK_TEAM) {
       const int r = i / N, c = i - r * N;
       const int g = r < NA ? ao + r : bo + (r - NA);
-      prow[i] = a.P[(long)g * N + c];
+      prow[i] = Pm[(long)g * N + c];
       krow[i] = 0.0;
     }
     if (tid < NA * NB) jab_s[tid] = 0.0;
     if (tid == 0) next_chunk = nwave;
     if (tid < nseg) {
-      const int* rr = a.rng + (long)bpos * nr;
-      const int n64 = rr[tid], n32 = rr[nseg + tid];
+      const int* rr = a.rng + (long)bpos * 2 * JK_MAXSEG;
+      const int n64 = rr[tid], n32 = rr[JK_MAXSEG + tid];
       // ranges: all FP64 parts first (r = sg), then all FP32 parts (r = nseg + sg),
       // so that a bra has at most one precision boundary
-      r_start[tid] = a.seg_lo[tid];
-      r_start[nseg + tid] = a.seg_lo[tid] + n64;
+      r_start[tid] = a.seg_lo[s0 + tid];
+      r_start[nseg + tid] = a.seg_lo[s0 + tid] + n64;
       r_pre[tid + 1] = n64;  // lengths for now
       r_pre[nseg + tid + 1] = n32;
     }
@@ -475,8 +481,8 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
         while (r + 1 < nr && v >= r_pre[r + 1]) ++r;
       const bool act = v < total && v < r_pre[r] + r_len[r];
       const int jp = act ? r_start[r] + (v - r_pre[r]) : 0;
-      const bool lo = act && r >= a.nseg;
-      const bool hi = act && r < a.nseg;
+      const bool lo = act && r >= nseg;
+      const bool hi = act && r < nseg;
       double val[C::NOUT];
       int Cs = 0, Ds = 0;
       double CDx = 0.0, CDy = 0.0, CDz = 0.0;
@@ -502,7 +508,7 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
         }
       }
       if (act) {
-        const int co = a.sh.bf[Cs], dof = a.sh.bf[Ds];
+        const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
         double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
         if (jp == ip) f *= 0.5;
 #pragma unroll
@@ -511,7 +517,7 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
 #pragma unroll
         for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = a.P[(long)(co + ic) * N + dof + id];
+          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pm[(long)(co + ic) * N + dof + id];
         double jcd[NC * ND], kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
 #pragma unroll
         for (int i = 0; i < NC * ND; ++i) jcd[i] = 0.0;
@@ -550,7 +556,7 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
         for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
           for (int id = 0; id < ND; ++id)
-            atomicAdd(&a.Jt[(long)(co + ic) * N + dof + id], 2.0 * jcd[ic * ND + id]);
+            atomicAdd(&Jm[(long)(co + ic) * N + dof + id], 2.0 * jcd[ic * ND + id]);
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia) {
 #pragma unroll
@@ -586,7 +592,7 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
       const double x = jab_s[tid];
       if (x != 0.0) {
         const int ia = tid / NB, ib = tid - ia * NB;
-        atomicAdd(&a.Jt[(long)(ao + ia) * N + bo + ib], 2.0 * x);
+        atomicAdd(&Jm[(long)(ao + ia) * N + bo + ib], 2.0 * x);
       }
     }
     const int nrow = (A == B) ? NA : NA + NB;
@@ -595,30 +601,51 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
       if (x != 0.0) {
         const int r = i / N, c = i - r * N;
         const int g = r < NA ? ao + r : bo + (r - NA);
-        atomicAdd(&a.Kt[(long)g * N + c], x);
+        atomicAdd(&Km[(long)g * N + c], x);
       }
     }
   }
 }
 
-// J = s (Jt + Jt^T) s, K = s (Kt + Kt^T) s; optional P' = s P s.
-__global__ void k_symmetrize(int N, const double* __restrict__ scale, const double* __restrict__ Jt,
-                             const double* __restrict__ Kt, double* __restrict__ J,
-                             double* __restrict__ K) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N * N) return;
-  const int i = t / N, j = t - i * N;
-  const double s = scale[i] * scale[j];
-  J[t] = s * (Jt[t] + Jt[j * N + i]);
-  K[t] = s * (Kt[t] + Kt[j * N + i]);
+// Element t of the packed per-system matrices -> (system, row, column).
+__device__ __forceinline__ void packed_index(long t, int nsys, const long* __restrict__ mb,
+                                             const int* __restrict__ n, int& s, int& i, int& j) {
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (mb[m] <= t) lo = m;
+    else hi = m - 1;
+  }
+  s = lo;
+  const long u = t - mb[s];
+  i = (int)(u / n[s]);
+  j = (int)(u - (long)i * n[s]);
 }
 
-__global__ void k_scale_density(int N, const double* __restrict__ scale, const double* __restrict__ P,
-                                double* __restrict__ Ps) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N * N) return;
-  const int i = t / N, j = t - i * N;
-  Ps[t] = P[t] * scale[i] * scale[j];
+// J = s (Jt + Jt^T) s, K = s (Kt + Kt^T) s per system (scale by global index).
+__global__ void k_symmetrize(long total, int nsys, const long* __restrict__ mb, const int* __restrict__ n,
+                             const int* __restrict__ bfb, const double* __restrict__ scale,
+                             const double* __restrict__ Jt, const double* __restrict__ Kt,
+                             double* __restrict__ J, double* __restrict__ K) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int s, i, j;
+  packed_index(t, nsys, mb, n, s, i, j);
+  const double f = scale[bfb[s] + i] * scale[bfb[s] + j];
+  const long tt = mb[s] + (long)j * n[s] + i;
+  J[t] = f * (Jt[t] + Jt[tt]);
+  K[t] = f * (Kt[t] + Kt[tt]);
+}
+
+// P' = s P s per system.
+__global__ void k_scale_density(long total, int nsys, const long* __restrict__ mb, const int* __restrict__ n,
+                                const int* __restrict__ bfb, const double* __restrict__ scale,
+                                const double* __restrict__ P, double* __restrict__ Ps) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int s, i, j;
+  packed_index(t, nsys, mb, n, s, i, j);
+  Ps[t] = P[t] * scale[bfb[s] + i] * scale[bfb[s] + j];
 }
 
 // Per-L Taylor tables in HBM, the same layout the kernels stage into LDS.

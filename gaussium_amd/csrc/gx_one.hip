@@ -19,6 +19,8 @@ namespace gx {
 #define ONE_LMAX 2            // shell l
 #define ONE_EMAX (ONE_LMAX + 2)  // l + 2 for the kinetic expansion
 #define ONE_RMAX (2 * ONE_LMAX)  // Hermite order for V
+#define ONE_RDIM (2 * ONE_LMAX + 2)  // R_tuv index bound (the gradient raises L by one)
+#define ONE_TMAX (ONE_LMAX + ONE_EMAX)  // Hermite index bound of E
 
 // Boys F_0..F_n(T) from the master table in global memory (Taylor + downward).
 __device__ inline void boys_master(int n, double T, const double* __restrict__ master, double* F) {
@@ -56,10 +58,10 @@ __device__ inline void boys_master(int n, double T, const double* __restrict__ m
 // Hermite coefficients E[i][j][t] (without the exp(-mu X^2) factor) for
 // i <= imax, j <= jmax along one axis.
 __device__ inline void hermite_e(int imax, int jmax, double PA, double PB, double o2p,
-                                 double (*E)[ONE_EMAX + 1][2 * ONE_EMAX + 1]) {
+                                 double (*E)[ONE_EMAX + 1][ONE_TMAX + 1]) {
   for (int i = 0; i <= imax; ++i)
     for (int j = 0; j <= jmax; ++j)
-      for (int t = 0; t <= 2 * ONE_EMAX; ++t) E[i][j][t] = 0.0;
+      for (int t = 0; t <= ONE_TMAX; ++t) E[i][j][t] = 0.0;
   E[0][0][0] = 1.0;
   for (int i = 0; i <= imax; ++i) {
     if (i > 0) {
@@ -81,6 +83,45 @@ __device__ inline void hermite_e(int imax, int jmax, double PA, double PB, doubl
   }
 }
 
+// Hermite Coulomb integrals R^{(0)}_{tuv} for t + u + v <= L (McMurchie-
+// Davidson): R^{(n)}_{000} = (-2p)^n F_n(p |PC|^2), then
+// R^{(n)}_{t+1,u,v} = t R^{(n+1)}_{t-1,u,v} + X_PC R^{(n+1)}_{t,u,v} (likewise u, v),
+// from n = L down to 0 with two buffers.
+__device__ inline void hermite_r(int L, double p, const double* PC, const double* F,
+                                 double (*R)[ONE_RDIM][ONE_RDIM], double (*S)[ONE_RDIM][ONE_RDIM]) {
+  double m2p[ONE_RDIM + 1];
+  m2p[0] = 1.0;
+  for (int n = 1; n <= L; ++n) m2p[n] = m2p[n - 1] * (-2.0 * p);
+  // the buffer holding order n is R when (L - n) is even
+  for (int n = L; n >= 0; --n) {
+    double (*cur)[ONE_RDIM][ONE_RDIM] = ((L - n) & 1) ? S : R;
+    double (*prv)[ONE_RDIM][ONE_RDIM] = ((L - n) & 1) ? R : S;
+    const int top = L - n;
+    for (int tt = 0; tt <= top; ++tt)
+      for (int u = 0; u <= top - tt; ++u)
+        for (int v = 0; v <= top - tt - u; ++v) {
+          double r;
+          if (tt + u + v == 0) {
+            r = m2p[n] * F[n];
+          } else if (tt > 0) {
+            r = PC[0] * prv[tt - 1][u][v];
+            if (tt > 1) r += (tt - 1) * prv[tt - 2][u][v];
+          } else if (u > 0) {
+            r = PC[1] * prv[tt][u - 1][v];
+            if (u > 1) r += (u - 1) * prv[tt][u - 2][v];
+          } else {
+            r = PC[2] * prv[tt][u][v - 1];
+            if (v > 1) r += (v - 1) * prv[tt][u][v - 2];
+          }
+          cur[tt][u][v] = r;
+        }
+  }
+  if (L & 1)  // order 0 ended in S
+    for (int tt = 0; tt <= L; ++tt)
+      for (int u = 0; u <= L - tt; ++u)
+        for (int v = 0; v <= L - tt - u; ++v) R[tt][u][v] = S[tt][u][v];
+}
+
 __device__ inline int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
 
 __device__ inline void cart_d(int l, int k, int* e) {
@@ -96,30 +137,54 @@ __device__ inline void cart_d(int l, int k, int* e) {
   }
 }
 
-__global__ __launch_bounds__(64) void k_one_electron(
-    int nshell, const int* __restrict__ sl, const int* __restrict__ snp,
-    const int* __restrict__ spoff, const int* __restrict__ sbf, const double* __restrict__ ctr1,
-    const double* __restrict__ exps, const double* __restrict__ coefs,
-    const double* __restrict__ scale, int natom, const double* __restrict__ nxyz,
-    const double* __restrict__ nZ, const double* __restrict__ master, int N,
-    double* __restrict__ S, double* __restrict__ T, double* __restrict__ V) {
+// Shell pairs of one system each (the ERI pair list: every unordered pair of
+// shells of a system once); nuclei of the pair's system only.
+struct OneArgs {
+  int npair;
+  const int* pA;
+  const int* pB;
+  const int* sl;
+  const int* snp;
+  const int* spoff;
+  const int* sbf;     // global first function (scale index)
+  const int* slbf;    // first function inside the system
+  const int* ssys;
+  const double* ctr1;
+  const double* exps;
+  const double* coefs;
+  const double* scale;
+  const int* sys_atom;  // nsys + 1: nuclei of system s are [sys_atom[s], sys_atom[s + 1])
+  const double* nxyz;
+  const double* nZ;
+  const double* master;
+  SysView sy;
+};
+
+__global__ __launch_bounds__(64) void k_one_electron(OneArgs a, double* __restrict__ S, double* __restrict__ T,
+                                                     double* __restrict__ V) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int npair = nshell * (nshell + 1) / 2;
-  if (t >= npair) return;
-  // unrank t -> (si >= sj)
-  int si = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while (si * (si + 1) / 2 > t) --si;
-  while ((si + 1) * (si + 2) / 2 <= t) ++si;
-  const int sj = t - si * (si + 1) / 2;
+  if (t >= a.npair) return;
+  const int si = a.pA[t], sj = a.pB[t];
+  const int* __restrict__ sl = a.sl;
+  const int* __restrict__ snp = a.snp;
+  const int* __restrict__ spoff = a.spoff;
+  const double* __restrict__ ctr1 = a.ctr1;
+  const double* __restrict__ exps = a.exps;
+  const double* __restrict__ coefs = a.coefs;
+  const double* __restrict__ nxyz = a.nxyz;
+  const double* __restrict__ nZ = a.nZ;
+  const double* __restrict__ master = a.master;
+  const int sys = a.ssys[si];
+  const int c0 = a.sys_atom ? a.sys_atom[sys] : 0, c1 = a.sys_atom ? a.sys_atom[sys + 1] : 0;
   const int la = sl[si], lb = sl[sj];
   const int na = ncart_d(la), nb = ncart_d(lb);
   const double A[3] = {ctr1[3 * si], ctr1[3 * si + 1], ctr1[3 * si + 2]};
   const double B[3] = {ctr1[3 * sj], ctr1[3 * sj + 1], ctr1[3 * sj + 2]};
   double s_acc[36], t_acc[36], v_acc[36];
   for (int k = 0; k < 36; ++k) s_acc[k] = t_acc[k] = v_acc[k] = 0.0;
-  double E[3][ONE_EMAX + 1][ONE_EMAX + 1][2 * ONE_EMAX + 1];
-  double R[ONE_RMAX + 1][ONE_RMAX + 1][ONE_RMAX + 1][ONE_RMAX + 1];
-  double F[ONE_RMAX + 1];
+  double E[3][ONE_LMAX + 1][ONE_EMAX + 1][ONE_TMAX + 1];
+  double R[ONE_RDIM][ONE_RDIM][ONE_RDIM], Rs[ONE_RDIM][ONE_RDIM][ONE_RDIM];
+  double F[ONE_RDIM + 1];
   for (int ia = 0; ia < snp[si]; ++ia) {
     const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
     for (int ib = 0; ib < snp[sj]; ++ib) {
@@ -156,36 +221,11 @@ __global__ __launch_bounds__(64) void k_one_electron(
       }
       // nuclear attraction
       const int L = la + lb;
-      for (int c = 0; c < natom; ++c) {
+      for (int c = c0; c < c1; ++c) {
         const double PC[3] = {P[0] - nxyz[3 * c], P[1] - nxyz[3 * c + 1], P[2] - nxyz[3 * c + 2]};
         const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
         boys_master(L, Tb, master, F);
-        // R^{(n)}_{tuv}: R[n][t][u][v]
-        double m2p = 1.0;
-        for (int n = 0; n <= L; ++n) {
-          R[n][0][0][0] = m2p * F[n];
-          m2p *= -2.0 * p;
-        }
-        for (int n = L - 1; n >= 0; --n) {
-          const int top = L - n;
-          for (int tt = 0; tt <= top; ++tt)
-            for (int u = 0; u <= top - tt; ++u)
-              for (int v = 0; v <= top - tt - u; ++v) {
-                if (tt + u + v == 0) continue;
-                double r;
-                if (tt > 0) {
-                  r = PC[0] * R[n + 1][tt - 1][u][v];
-                  if (tt > 1) r += (tt - 1) * R[n + 1][tt - 2][u][v];
-                } else if (u > 0) {
-                  r = PC[1] * R[n + 1][tt][u - 1][v];
-                  if (u > 1) r += (u - 1) * R[n + 1][tt][u - 2][v];
-                } else {
-                  r = PC[2] * R[n + 1][tt][u][v - 1];
-                  if (v > 1) r += (v - 1) * R[n + 1][tt][u][v - 2];
-                }
-                R[n][tt][u][v] = r;
-              }
-        }
+        hermite_r(L, p, PC, F, R, Rs);
         const double pref = -nZ[c] * 2.0 * M_PI / p * Kab;
         for (int ka = 0; ka < na; ++ka) {
           int ea[3];
@@ -197,28 +237,29 @@ __global__ __launch_bounds__(64) void k_one_electron(
             for (int tt = 0; tt <= ea[0] + eb[0]; ++tt)
               for (int u = 0; u <= ea[1] + eb[1]; ++u)
                 for (int v = 0; v <= ea[2] + eb[2]; ++v)
-                  sum += E[0][ea[0]][eb[0]][tt] * E[1][ea[1]][eb[1]][u] * E[2][ea[2]][eb[2]][v] *
-                         R[0][tt][u][v];
+                  sum += E[0][ea[0]][eb[0]][tt] * E[1][ea[1]][eb[1]][u] * E[2][ea[2]][eb[2]][v] * R[tt][u][v];
             v_acc[ka * nb + kb] += pref * sum;
           }
         }
       }
     }
   }
-  const int ao = sbf[si], bo = sbf[sj];
+  const int ao = a.sbf[si], bo = a.sbf[sj];
+  const int la_ = a.slbf[si], lb_ = a.slbf[sj];
+  const long N = a.sy.n[sys], mb = a.sy.mb[sys];
   for (int ka = 0; ka < na; ++ka)
     for (int kb = 0; kb < nb; ++kb) {
-      const long i = ao + ka, j = bo + kb;
-      const double s = scale[i] * scale[j];
+      const double s = a.scale[ao + ka] * a.scale[bo + kb];
+      const long i = la_ + ka, j = lb_ + kb;
       const double sv = s * s_acc[ka * nb + kb], tv = s * t_acc[ka * nb + kb],
                    vv = s * v_acc[ka * nb + kb];
-      S[i * N + j] = sv;
-      S[j * N + i] = sv;
-      T[i * N + j] = tv;
-      T[j * N + i] = tv;
+      S[mb + i * N + j] = sv;
+      S[mb + j * N + i] = sv;
+      T[mb + i * N + j] = tv;
+      T[mb + j * N + i] = tv;
       if (V) {
-        V[i * N + j] = vv;
-        V[j * N + i] = vv;
+        V[mb + i * N + j] = vv;
+        V[mb + j * N + i] = vv;
       }
     }
 }
@@ -236,11 +277,12 @@ namespace gx {
 // S and T depend on A - B only (d/dB = -d/dA); the nuclear-attraction term of
 // nucleus C moves with C:  d/dC = -(d/dA + d/dB).
 // One thread per shell pair (i >= j); atomics into grad[3 * atom + axis].
-#define ONE_GRAD_E (ONE_LMAX + 3)      // bra raised by 1, ket by 2 (kinetic) or 1
-#define ONE_GRAD_R (2 * ONE_LMAX + 1)  // Hermite order for raised V
+#define ONE_GRAD_I (ONE_LMAX + 1)      // bra raised by 1
+#define ONE_GRAD_J (ONE_LMAX + 2)      // ket raised by 2 (kinetic) or 1
+#define ONE_GRAD_T (ONE_GRAD_I + ONE_GRAD_J)
 
 __device__ inline void hermite_e_g(int imax, int jmax, double PA, double PB, double o2p,
-                                   double (*E)[ONE_GRAD_E + 1][2 * ONE_GRAD_E + 1]) {
+                                   double (*E)[ONE_GRAD_J + 1][ONE_GRAD_T + 1]) {
   for (int i = 0; i <= imax; ++i)
     for (int j = 0; j <= jmax; ++j)
       for (int t = 0; t <= i + j; ++t) E[i][j][t] = 0.0;
@@ -265,37 +307,42 @@ __device__ inline void hermite_e_g(int imax, int jmax, double PA, double PB, dou
   }
 }
 
-__global__ __launch_bounds__(64) void k_one_grad(
-    int nshell, const int* __restrict__ sl, const int* __restrict__ snp, const int* __restrict__ spoff,
-    const int* __restrict__ sbf, const int* __restrict__ satom, const double* __restrict__ ctr1,
-    const double* __restrict__ exps, const double* __restrict__ coefs, const double* __restrict__ scale,
-    int natom, const double* __restrict__ nxyz, const double* __restrict__ nZ,
-    const double* __restrict__ master, int N, const double* __restrict__ P, const double* __restrict__ W,
-    double* __restrict__ grad) {
+__global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restrict__ satom,
+                                                 const double* __restrict__ P, const double* __restrict__ W,
+                                                 double* __restrict__ grad) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int npair = nshell * (nshell + 1) / 2;
-  if (t >= npair) return;
-  int si = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while (si * (si + 1) / 2 > t) --si;
-  while ((si + 1) * (si + 2) / 2 <= t) ++si;
-  const int sj = t - si * (si + 1) / 2;
+  if (t >= a.npair) return;
+  const int si = a.pA[t], sj = a.pB[t];
+  const int* __restrict__ sl = a.sl;
+  const int* __restrict__ snp = a.snp;
+  const int* __restrict__ spoff = a.spoff;
+  const double* __restrict__ ctr1 = a.ctr1;
+  const double* __restrict__ exps = a.exps;
+  const double* __restrict__ coefs = a.coefs;
+  const double* __restrict__ nxyz = a.nxyz;
+  const double* __restrict__ nZ = a.nZ;
+  const double* __restrict__ master = a.master;
+  const int sys = a.ssys[si];
+  const int c0 = a.sys_atom ? a.sys_atom[sys] : 0, c1 = a.sys_atom ? a.sys_atom[sys + 1] : 0;
   const int la = sl[si], lb = sl[sj];
   const int na = ncart_d(la), nb = ncart_d(lb);
   const double A[3] = {ctr1[3 * si], ctr1[3 * si + 1], ctr1[3 * si + 2]};
   const double B[3] = {ctr1[3 * sj], ctr1[3 * sj + 1], ctr1[3 * sj + 2]};
-  const int ao = sbf[si], bo = sbf[sj];
+  const int ao = a.sbf[si], bo = a.sbf[sj];
+  const int la_ = a.slbf[si], lb_ = a.slbf[sj];
+  const long N = a.sy.n[sys], mb = a.sy.mb[sys];
   const double wpair = (si == sj) ? 1.0 : 2.0;  // both triangles of an off-diagonal block
   double pab[36], wab[36];
   for (int ka = 0; ka < na; ++ka)
     for (int kb = 0; kb < nb; ++kb) {
-      const long i = ao + ka, j = bo + kb;
-      const double s = wpair * scale[i] * scale[j];
-      pab[ka * nb + kb] = s * P[i * N + j];
-      wab[ka * nb + kb] = s * W[i * N + j];
+      const long i = la_ + ka, j = lb_ + kb;
+      const double s = wpair * a.scale[ao + ka] * a.scale[bo + kb];
+      pab[ka * nb + kb] = s * P[mb + i * N + j];
+      wab[ka * nb + kb] = s * W[mb + i * N + j];
     }
-  double E[3][ONE_GRAD_E + 1][ONE_GRAD_E + 1][2 * ONE_GRAD_E + 1];
-  double R[ONE_GRAD_R + 1][ONE_GRAD_R + 1][ONE_GRAD_R + 1][ONE_GRAD_R + 1];
-  double F[ONE_GRAD_R + 1];
+  double E[3][ONE_GRAD_I + 1][ONE_GRAD_J + 1][ONE_GRAD_T + 1];
+  double R[ONE_RDIM][ONE_RDIM][ONE_RDIM], Rs[ONE_RDIM][ONE_RDIM][ONE_RDIM];
+  double F[ONE_RDIM + 1];
   double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
   // overlap and kinetic: depend on A - B only
   for (int ia = 0; ia < snp[si]; ++ia) {
@@ -353,7 +400,7 @@ __global__ __launch_bounds__(64) void k_one_grad(
   }
   // nuclear attraction, one nucleus at a time
   const int L = la + lb + 1;
-  for (int c = 0; c < natom; ++c) {
+  for (int c = c0; c < c1; ++c) {
     double gC[3] = {0.0, 0.0, 0.0};
     for (int ia = 0; ia < snp[si]; ++ia) {
       const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
@@ -369,38 +416,14 @@ __global__ __launch_bounds__(64) void k_one_grad(
         const double PC[3] = {Pc[0] - nxyz[3 * c], Pc[1] - nxyz[3 * c + 1], Pc[2] - nxyz[3 * c + 2]};
         const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
         boys_master(L, Tb, master, F);
-        double m2p = 1.0;
-        for (int n = 0; n <= L; ++n) {
-          R[n][0][0][0] = m2p * F[n];
-          m2p *= -2.0 * p;
-        }
-        for (int n = L - 1; n >= 0; --n) {
-          const int top = L - n;
-          for (int tt = 0; tt <= top; ++tt)
-            for (int u = 0; u <= top - tt; ++u)
-              for (int v = 0; v <= top - tt - u; ++v) {
-                if (tt + u + v == 0) continue;
-                double r;
-                if (tt > 0) {
-                  r = PC[0] * R[n + 1][tt - 1][u][v];
-                  if (tt > 1) r += (tt - 1) * R[n + 1][tt - 2][u][v];
-                } else if (u > 0) {
-                  r = PC[1] * R[n + 1][tt][u - 1][v];
-                  if (u > 1) r += (u - 1) * R[n + 1][tt][u - 2][v];
-                } else {
-                  r = PC[2] * R[n + 1][tt][u][v - 1];
-                  if (v > 1) r += (v - 1) * R[n + 1][tt][u][v - 2];
-                }
-                R[n][tt][u][v] = r;
-              }
-        }
+        hermite_r(L, p, PC, F, R, Rs);
         const double pref = -nZ[c] * 2.0 * M_PI / p * exp(-mu * r2) * ca * cb;
         auto Vval = [&](const int* ea, const int* eb) {
           double sum = 0.0;
           for (int tt = 0; tt <= ea[0] + eb[0]; ++tt)
             for (int u = 0; u <= ea[1] + eb[1]; ++u)
               for (int v = 0; v <= ea[2] + eb[2]; ++v)
-                sum += E[0][ea[0]][eb[0]][tt] * E[1][ea[1]][eb[1]][u] * E[2][ea[2]][eb[2]][v] * R[0][tt][u][v];
+                sum += E[0][ea[0]][eb[0]][tt] * E[1][ea[1]][eb[1]][u] * E[2][ea[2]][eb[2]][v] * R[tt][u][v];
           return sum;
         };
         for (int ka = 0; ka < na; ++ka) {
@@ -444,13 +467,16 @@ __global__ __launch_bounds__(64) void k_one_grad(
 }
 
 // Nuclear repulsion gradient  dE_nuc/dR_A = -sum_B Z_A Z_B (R_A - R_B) / |R_A - R_B|^3
-// (common/nuclear_repulsion.py:5-24 of the reference gives E_nuc), added to grad.
-__global__ void k_nuc_grad(int natom, const double* __restrict__ nxyz, const double* __restrict__ nZ,
+// over the nuclei B of A's system (common/nuclear_repulsion.py:5-24 of the
+// reference gives E_nuc), added to grad.
+__global__ void k_nuc_grad(int natom, const int* __restrict__ atom_sys, const int* __restrict__ sys_atom,
+                           const double* __restrict__ nxyz, const double* __restrict__ nZ,
                            double* __restrict__ grad) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= natom) return;
+  const int s = atom_sys[a];
   double g[3] = {0.0, 0.0, 0.0};
-  for (int b = 0; b < natom; ++b) {
+  for (int b = sys_atom[s]; b < sys_atom[s + 1]; ++b) {
     if (b == a) continue;
     const double d[3] = {nxyz[3 * a] - nxyz[3 * b], nxyz[3 * a + 1] - nxyz[3 * b + 1], nxyz[3 * a + 2] - nxyz[3 * b + 2]};
     const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];

@@ -66,7 +66,67 @@ class Engine:
             _p(sh.ctr1), _p(sh.atom), len(sh.exps), _p(sh.exps), _p(sh.coefs), sh.nbf, _p(sh.bf_scale)),
             'gx_set_basis')
         self.shells = sh
+        self.sizes = [sh.nbf]
+        self.offsets = np.array([0, sh.nbf * sh.nbf], dtype=np.int64)
         return sh
+
+    def set_systems(self, systems):
+        """A batch of independent molecules, ``[(basis, nuclei), ...]``: one
+        context, one launch per class combination for all of them.  Matrices
+        are then packed (``pack`` / ``unpack``)."""
+        shells = [pack_shells(b, n) for b, n in systems]
+        l, nprim, poff, bfo, ctr, ctr1, atom, exps, coefs, scale = ([] for _ in range(10))
+        sys_shell, sys_atom, sizes = [0], [0], []
+        nbf = npr = nat = 0
+        nuclei = []
+        for sh, (b, nuc) in zip(shells, systems):
+            if np.any(sh.atom < 0):
+                raise ValueError('every shell of a batched system must sit on one of its nuclei')
+            l.append(sh.l)
+            nprim.append(sh.nprim)
+            poff.append(sh.prim_off + npr)
+            bfo.append(sh.bf_off + nbf)
+            ctr.append(sh.ctr)
+            ctr1.append(sh.ctr1)
+            atom.append(sh.atom + nat)
+            exps.append(sh.exps)
+            coefs.append(sh.coefs)
+            scale.append(sh.bf_scale)
+            nbf += sh.nbf
+            npr += len(sh.exps)
+            nat += len(nuc)
+            nuclei += list(nuc)
+            sys_shell.append(sys_shell[-1] + sh.nshell)
+            sys_atom.append(nat)
+            sizes.append(sh.nbf)
+        cat = lambda xs, t: np.ascontiguousarray(np.concatenate(xs), dtype=t)
+        allsh = ShellSet(cat(l, np.int32), cat(nprim, np.int32), cat(poff, np.int32), cat(bfo, np.int32),
+                         cat(ctr, np.float64), cat(ctr1, np.float64), cat(atom, np.int32),
+                         cat(exps, np.float64), cat(coefs, np.float64), cat(scale, np.float64))
+        ss = np.ascontiguousarray(sys_shell, dtype=np.int32)
+        sa = np.ascontiguousarray(sys_atom, dtype=np.int32)
+        self._bind_stream()
+        _lib.check(self.lib.gx_set_basis_batch(
+            self.ctx, len(systems), _p(ss), _p(sa), allsh.nshell, _p(allsh.l), _p(allsh.nprim),
+            _p(allsh.prim_off), _p(allsh.bf_off), _p(allsh.ctr), _p(allsh.ctr1), _p(allsh.atom), len(allsh.exps),
+            _p(allsh.exps), _p(allsh.coefs), allsh.nbf, _p(allsh.bf_scale)), 'gx_set_basis_batch')
+        self.shells = allsh
+        self.sizes = sizes
+        self.offsets = np.concatenate([[0], np.cumsum([n * n for n in sizes])]).astype(np.int64)
+        self.set_nuclei(nuclei)
+        return allsh
+
+    def pack(self, mats):
+        """List of [n_k, n_k] matrices -> one packed device vector."""
+        return torch.cat([self._dev(m).reshape(-1) for m in mats])
+
+    def unpack(self, flat):
+        """Packed vector -> list of [n_k, n_k] views."""
+        return [flat[self.offsets[k]:self.offsets[k + 1]].view(n, n) for k, n in enumerate(self.sizes)]
+
+    @property
+    def packed_size(self):
+        return int(self.offsets[-1])
 
     def set_nuclei(self, nuclei):
         xyz = np.ascontiguousarray([n.coordinates for n in nuclei], dtype=np.float64)
@@ -77,6 +137,17 @@ class Engine:
     @property
     def nbf(self):
         return self.shells.nbf
+
+    def _matrix_shape(self, M, what):
+        """(n, n) for one system, the packed length for a batch."""
+        if len(self.sizes) == 1:
+            n = self.nbf
+            if M.shape != (n, n):
+                raise ValueError('%s has shape %s, expected (%d, %d)' % (what, tuple(M.shape), n, n))
+            return (n, n)
+        if M.shape != (self.packed_size,):
+            raise ValueError('%s must be the packed vector of %d elements' % (what, self.packed_size))
+        return (self.packed_size,)
 
     def _empty(self, *shape):
         return torch.empty(*shape, dtype=torch.float64, device=self.device)
@@ -116,16 +187,29 @@ class Engine:
         ``timing=True`` records HIP events around each class-combination
         kernel on the stream it runs on; read them with ``jk_launch_times``.
         """
-        n = self.nbf
         P = self._dev(P)
-        if P.shape != (n, n):
-            raise ValueError('density has shape %s, expected (%d, %d)' % (tuple(P.shape), n, n))
-        J, K = (self._empty(n, n), self._empty(n, n)) if out is None else out
+        shape = self._matrix_shape(P, 'density')
+        J, K = (self._empty(*shape), self._empty(*shape)) if out is None else out
         self._bind_stream()
         flags = (_lib.GX_JK_MIXED if mixed else 0) | (_lib.GX_JK_TIMING if timing else 0)
         _lib.check(self.lib.gx_jk(self.ctx, _p(P), _p(J), _p(K), float(cutoff), float(fp32_bound), flags),
                    'gx_jk')
         return J, K
+
+    def autotune(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False, candidates=(1.0, 2.0, 4.0, 8.0, 16.0),
+                 reps=3):
+        """Pick the work-item size of each class combination by timing (see
+        gx_jk_autotune); returns {(la, lb, lc, ld): (items per workgroup, ms)}."""
+        P = self._dev(P)
+        self._matrix_shape(P, 'density')
+        cand = np.ascontiguousarray(candidates, dtype=np.float64)
+        out = np.zeros((64, 6))
+        self._bind_stream()
+        n = self.lib.gx_jk_autotune(self.ctx, _p(P), float(cutoff), float(fp32_bound),
+                                    _lib.GX_JK_MIXED if mixed else 0, _p(cand), len(cand), int(reps), _p(out), 64)
+        if n < 0:
+            _lib.check(n, 'gx_jk_autotune')
+        return {tuple(int(x) for x in r[:4]): (float(r[4]), float(r[5])) for r in out[:n]}
 
     def jk_launch_times(self):
         """{(la, lb, lc, ld): (total ms, launches)} over the ``jk(..., timing=True)``
@@ -141,10 +225,9 @@ class Engine:
         the density ``P`` and the energy-weighted density ``W`` (device tensors)."""
         if self.nuclei is None:
             raise _lib.GxError('grad needs nuclei (set_nuclei)')
-        n = self.nbf
         P, W = self._dev(P), self._dev(W)
-        if P.shape != (n, n) or W.shape != (n, n):
-            raise ValueError('P and W must be (%d, %d)' % (n, n))
+        self._matrix_shape(P, 'P')
+        self._matrix_shape(W, 'W')
         g = self._empty(len(self.nuclei), 3)
         self._bind_stream()
         _lib.check(self.lib.gx_grad(self.ctx, _p(P), _p(W), _p(g), float(cutoff)), 'gx_grad')
@@ -158,9 +241,9 @@ class Engine:
 
     def one_electron(self):
         """(S, T, V) on the device; V needs nuclei."""
-        n = self.nbf
-        S, T = self._empty(n, n), self._empty(n, n)
-        V = self._empty(n, n) if self.nuclei is not None else None
+        shape = (self.nbf, self.nbf) if len(self.sizes) == 1 else (self.packed_size,)
+        S, T = self._empty(*shape), self._empty(*shape)
+        V = self._empty(*shape) if self.nuclei is not None else None
         self._bind_stream()
         _lib.check(self.lib.gx_one_electron(self.ctx, _p(S), _p(T), _p(V) if V is not None else None),
                    'gx_one_electron')

@@ -66,12 +66,13 @@ typedef struct gx_stats {
 /* Last error message of this host thread ("" if none). */
 const char* gx_last_error(void);
 
-/* Create a context on HIP device `device` with its own stream. */
+/* Create a context on HIP device `device`.  It launches on the device's null
+ * stream until gx_set_stream names another. */
 int gx_create(gx_ctx** ctx, int device);
 int gx_destroy(gx_ctx* ctx);
 
-/* Launch on `stream` (a hipStream_t, e.g. torch's current stream) instead
- * of the context's own stream; NULL restores the context stream. */
+/* Launch on `stream` (a hipStream_t, e.g. torch's current stream); NULL
+ * restores the null stream. */
 int gx_set_stream(gx_ctx* ctx, void* stream);
 
 /* Basis set as shells of Cartesian functions (all host arrays).
@@ -86,12 +87,29 @@ int gx_set_stream(gx_ctx* ctx, void* stream);
  *                                (contraction x primitive norm x contracted norm
  *                                of the shell's first component)
  *   bf_scale[nbf]                per-function factor relative to that component
- * Builds primitive pairs and Schwarz bounds on the device. l <= 1 for all
- * shells in this build. */
+ * Builds primitive pairs and Schwarz bounds on the device.  l <= 2 (s, p,
+ * d Cartesian shells). */
 int gx_set_basis(gx_ctx* ctx, int nshell, const int* l, const int* nprim, const int* prim_off,
                  const int* bf_off, const double* ctr, const double* ctr1, const int* atom,
                  int nprim_total, const double* exps, const double* coefs, int nbf,
                  const double* bf_scale);
+
+/* A batch of independent molecules (systems) in one context, so that one
+ * launch per class combination builds J/K (or gradients) for all of them.
+ * Shells [sys_shell[k], sys_shell[k + 1]) form system k (sys_shell[0] = 0,
+ * sys_shell[nsys] = nshell); their functions must be one contiguous range
+ * following the previous system's.  Nuclei [sys_atom[k], sys_atom[k + 1])
+ * of the gx_set_nuclei array belong to system k (NULL: one system with all
+ * nuclei).  Quartets never mix systems.  Every nbf x nbf matrix of the other
+ * calls (P, J, K, W, S, T, V) is then the systems' matrices packed one after
+ * the other, row-major, sum_k n_k^2 doubles.  gx_set_basis is the one-system
+ * case.  No counterpart in the reference, which builds one molecule at a
+ * time (geometryoptimization/nelder_mead.py:126-144 evaluates simplex
+ * vertices one after the other). */
+int gx_set_basis_batch(gx_ctx* ctx, int nsys, const int* sys_shell, const int* sys_atom, int nshell,
+                       const int* l, const int* nprim, const int* prim_off, const int* bf_off, const double* ctr,
+                       const double* ctr1, const int* atom, int nprim_total, const double* exps,
+                       const double* coefs, int nbf, const double* bf_scale);
 
 /* Nuclei (host arrays): xyz[3*natom] in bohr, charge[natom]. */
 int gx_set_nuclei(gx_ctx* ctx, int natom, const double* xyz, const double* charge);
@@ -117,6 +135,17 @@ int gx_eri_dense(gx_ctx* ctx, double* out_dev);
  * into the FP64 J and K. */
 int gx_jk(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double cutoff,
           double fp32_bound, int flags);
+
+/* Autotuner of the J/K work-item size ("tile") per class combination: for
+ * each combination, the kernel is timed (best of `reps` launches, HIP
+ * events on the context stream) with the items sized for each of the
+ * `ncand` candidate values of "items per resident workgroup", and the
+ * fastest is kept for later gx_jk calls with these thresholds (until the
+ * next gx_set_basis).  Writes one row of 6 doubles per combination: la, lb,
+ * lc, ld, chosen value, its milliseconds.  Returns the number of rows or an
+ * error code.  No counterpart in the reference. */
+int gx_jk_autotune(gx_ctx* ctx, const double* P_dev, double cutoff, double fp32_bound, int flags,
+                   const double* candidates, int ncand, int reps, double* out, int cap);
 
 /* Nuclear gradient of the closed-shell RHF energy, dE/dR (3 natom doubles,
  * device buffer, atom-major), from the converged density P and the
