@@ -38,6 +38,10 @@ def parse():
                     help='wall seconds of the sampled CPU baseline (0 to skip)')
     ap.add_argument('--seed', type=int, default=2)
     ap.add_argument('--scf-iter', type=int, default=30, help='SCF iterations of the untimed set-up')
+    ap.add_argument('--workload', default='cluster_220bf',
+                    choices=['cluster_220bf', 'gradient_66bf', 'batch_256', 'cluster_255bf_d'],
+                    help='cluster_220bf: the headline J/K line; the others print one line each for the '
+                         'gradient (66 functions), the 256-geometry batch and the d-shell cluster')
     ap.add_argument('--kernels-only', action='store_true',
                     help='counter runs: no SCF; time the J/K kernels on the core-guess density and run '
                          'one %d MiB device copy as the HBM-counter calibration' % (COPY_BYTES >> 20))
@@ -123,6 +127,128 @@ def kernels_only(args):
     print(json.dumps(out), flush=True)
 
 
+def _median_ms(fn, steps, warmup):
+    """Median of `steps` calls of fn() timed with torch events on the current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    en = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for i in range(steps):
+        st[i].record()
+        fn()
+        en[i].record()
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in zip(st, en))
+    return ms[len(ms) // 2]
+
+
+def _scf_density(eng, nuc, basis, ne, iters):
+    """P and W = 2 sum_occ e C C^T after `iters` SCF iterations (untimed)."""
+    from gaussium_amd.hartree_fock import RestrictedHF
+    hf = RestrictedHF(nuc, basis, ne, engine=eng, schwarz_cutoff=CUTOFF, max_iter=iters, threshold=1e-9)
+    hf.begin_scf()
+    c = hf.scf_method.orbital_coefficients[:, :ne // 2]
+    eps = hf.scf_method.orbital_energies[:ne // 2]
+    return hf.scf_method.density, 2.0 * (c * eps) @ c.T, hf
+
+
+def bench_gradient(args):
+    """Analytic gradient of the 12-atom, 66-function cluster: one gx_grad call
+    (one-electron + two-electron derivative kernels, FP64) at an SCF density."""
+    from gaussium_amd import workloads
+    from gaussium_amd.engine import Engine
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    P, W, hf = _scf_density(eng, nuc, basis, ne, args.scf_iter)
+    ms = _median_ms(lambda: eng.grad(P, W, cutoff=CUTOFF), args.steps, args.warmup)
+    ms_jk = _median_ms(lambda: eng.jk(P, cutoff=CUTOFF), args.steps, args.warmup)
+    g = eng.grad(P, W, cutoff=CUTOFF)
+    return {'metric': 'RHF nuclear gradient ms (66 bf, FP64)', 'value': ms, 'unit': 'ms', 'higher_is_better': False,
+            'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'dtype': 'f64',
+            'data': 'synthetic (seeded random cluster and basis)',
+            'config': {'workload': info['workload'], 'nbf': info['nbf'], 'atoms': info['n_atoms'],
+                       'schwarz_cutoff': CUTOFF},
+            'jk_ms_same_density': ms_jk, 'max_abs_gradient': float(g.abs().max()),
+            'translation_residual': float(g.sum(0).abs().max())}
+
+
+def bench_batch(args):
+    """256 geometries of 4-40 atoms (22-220 functions) in one context: J/K and
+    gradient for all of them, one launch per class combination, captured once
+    in a HIP graph and replayed; work items autotuned per class first."""
+    import torch
+    from gaussium_amd import workloads
+    from gaussium_amd.engine import Engine
+    batch = workloads.batch(256)
+    systems = [(b, n) for n, b, ne, info in batch]
+    eng = Engine()
+    eng.set_systems(systems)
+    import numpy as np
+    rng = np.random.default_rng(0)
+    mats = []
+    for b, _ in systems:
+        a = rng.standard_normal((len(b), len(b)))
+        mats.append((a + a.T) / len(b))
+    P = eng.pack(mats)
+    print('batch: %d systems set up' % len(systems), file=sys.stderr, flush=True)
+    tuned = eng.autotune(P, cutoff=CUTOFF, reps=2)
+    print('batch: autotuned', file=sys.stderr, flush=True)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        eng.jk(P, cutoff=CUTOFF)
+        eng.grad(P, P, cutoff=CUTOFF)
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        eng.jk(P, cutoff=CUTOFF)
+        eng.grad(P, P, cutoff=CUTOFF)
+    print('batch: graph captured', file=sys.stderr, flush=True)
+    ms = _median_ms(graph.replay, args.steps, args.warmup)
+    ms_jk = _median_ms(lambda: eng.jk(P, cutoff=CUTOFF), args.steps, args.warmup)
+    ms_grad = _median_ms(lambda: eng.grad(P, P, cutoff=CUTOFF), args.steps, args.warmup)
+    st = eng.jk_stats(cutoff=CUTOFF)
+    sizes = [len(b) for b, _ in systems]
+    return {'metric': 'batched J/K + gradient, 256 geometries, ms per graph replay', 'value': ms, 'unit': 'ms',
+            'higher_is_better': False, 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'dtype': 'f64',
+            'data': 'synthetic (seeded random clusters and basis)',
+            'config': {'workload': 'batch_256', 'geometries': len(systems), 'nbf_min': min(sizes),
+                       'nbf_max': max(sizes), 'nbf_total': sum(sizes), 'schwarz_cutoff': CUTOFF},
+            'geometries_per_s': len(systems) / (ms * 1e-3), 'jk_ms_eager': ms_jk, 'grad_ms_eager': ms_grad,
+            'bf_quartets_per_s_jk': st['bf_quartets'] / (ms_jk * 1e-3),
+            'autotuned_items_per_wg': {'%d%d%d%d' % k: v[0] for k, v in tuned.items()}}
+
+
+def bench_d(args):
+    """30-atom polarised cluster (255 functions, d shells): FP64 and mixed J/K
+    and the FP64 gradient at an SCF density."""
+    import torch
+    from gaussium_amd import workloads
+    from gaussium_amd.engine import Engine
+    nuc, basis, ne, info = workloads.cluster_255bf()
+    eng = Engine(basis, nuc)
+    P, W, hf = _scf_density(eng, nuc, basis, ne, args.scf_iter)
+    H = hf.core_hamiltonian
+    ms64 = _median_ms(lambda: eng.jk(P, cutoff=CUTOFF), args.steps, args.warmup)
+    msmx = _median_ms(lambda: eng.jk(P, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True), args.steps, args.warmup)
+    msg = _median_ms(lambda: eng.grad(P, W, cutoff=CUTOFF), args.steps, args.warmup)
+    e = {}
+    for mode, kw in (('fp64', {}), ('mixed', dict(fp32_bound=FP32_BOUND, mixed=True))):
+        J, K = eng.jk(P, cutoff=CUTOFF, **kw)
+        e[mode] = 0.5 * float(torch.sum(P * (2.0 * H + J - 0.5 * K)))
+    st = eng.jk_stats(cutoff=CUTOFF)
+    return {'metric': 'J/K build ms (255 bf with d shells), FP64 and mixed', 'value': msmx, 'unit': 'ms',
+            'higher_is_better': False, 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+            'dtype': 'mixed-f32/f64', 'data': 'synthetic (seeded random cluster and basis)',
+            'config': {'workload': info['workload'], 'nbf': info['nbf'], 'atoms': info['n_atoms'],
+                       'electrons': ne, 'schwarz_cutoff': CUTOFF, 'fp32_bound': FP32_BOUND},
+            'jk_ms_fp64': ms64, 'jk_ms_mixed': msmx, 'grad_ms_fp64': msg,
+            'dE_mixed_Eh_fixed_density': abs(e['mixed'] - e['fp64']),
+            'bf_quartets_per_s_fp64': st['bf_quartets'] / (ms64 * 1e-3)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -131,6 +257,10 @@ def main():
         sys.exit(2)
     if args.kernels_only:
         return kernels_only(args)
+    extra = {'gradient_66bf': bench_gradient, 'batch_256': bench_batch, 'cluster_255bf_d': bench_d}
+    if args.workload in extra:
+        print(json.dumps(extra[args.workload](args)), flush=True)
+        return
     import torch
     from gaussium_amd import workloads
     from gaussium_amd.basis import nuclear_repulsion

@@ -600,6 +600,100 @@ def emit_grad_prog(la, lb, lc, ld):
     return '\n'.join(s)
 
 
+# ---------------------------------------------------------------------------
+# Per-lane gradient code for the small classes: the gradient program written
+# out as straight-line code (each VRR slot a register, the HRR slots renamed
+# on every write), one shell quartet per lane.
+GRAD_LANE_NACC = 70  # gradient classes with at most this many weighted intermediates run per lane
+
+
+def emit_grad_lane(la, lb, lc, ld):
+    g = grad_program(la, lb, lc, ld)
+    L = g['L']
+    na, nb, nc, nd = ncart(la), ncart(lb), ncart(lc), ncart(ld)
+    axes = 'xyz'
+    lines = []
+
+    def var(slot):
+        if slot == 0:
+            return 'R(0)'
+        if 1 <= slot <= L + 1:
+            return 'F[%d]' % (slot - 1)
+        return 'v%d' % slot
+
+    for (x, y, z, w) in g['vrr']:
+        dest, kind, ax = x & 0xffff, (x >> 16) & 1, (x >> 17) & 3
+        k1, k2 = (x >> 20) & 15, (x >> 24) & 15
+        s0, s1, s2, s3, s4 = y & 0xffff, y >> 16, z & 0xffff, z >> 16, w
+        cA = ('QC' if kind else 'PA') + axes[ax]
+        cB = ('WQ' if kind else 'WP') + axes[ax]
+        terms = ['%s * %s' % (cA, var(s0)), '%s * %s' % (cB, var(s1))]
+        if k1:
+            o, r = ('o2q', 'rho_q') if kind else ('o2p', 'rho_p')
+            terms.append('R(%d) * %s * (%s - %s * %s)' % (k1, o, var(s2), r, var(s3)))
+        if k2:
+            terms.append('R(%d) * o2pq * %s' % (k2, var(s4)))
+        lines.append('    const R v%d = %s;' % (dest, ' + '.join(terms)))
+    wname = ['', 'w1 * ', 'w2 * ', 'w3 * ']
+    for t, e in enumerate(g['acc']):
+        lines.append('    acc[%d] += %s%s;' % (t, wname[e >> 16], var(e & 0xffff)))
+    fin = []
+    cur = {t: 'acc[%d]' % t for t in range(g['NACC'])}
+    cur[g['ZSLOT']] = 'R(0)'
+    for n_, (x, y) in enumerate(g['hrr']):
+        dest, kind, ax = x & 0xffff, (x >> 16) & 1, (x >> 17) & 3
+        s0, s1 = y & 0xffff, y >> 16
+        X = ('CD' if kind else 'AB') + axes[ax]
+        fin.append('    const R h%d = %s + %s * %s;' % (n_, cur[s0], X, cur[s1]))
+        cur[dest] = 'h%d' % n_
+    k = 0
+    for ia in range(na):
+        for ib in range(nb):
+            for ic in range(nc):
+                for id_ in range(nd):
+                    fin.append('    {')
+                    fin.append('      const R D = f * (R(4) * Pab[%d] * Pcd[%d] - Pac[%d] * Pbd[%d] - Pad[%d] * Pbc[%d]);'
+                               % (ia * nb + ib, ic * nd + id_, ia * nc + ic, ib * nd + id_, ia * nd + id_,
+                                  ib * nc + ic))
+                    for j in range(9):
+                        plus, mword = g['terms'][9 * k + j]
+                        minus, n = mword & 0xffff, mword >> 16
+                        if n:
+                            fin.append('      g[%d] += D * (%s - R(%d) * %s);' % (j, cur[plus], n, cur[minus]))
+                        else:
+                            fin.append('      g[%d] += D * %s;' % (j, cur[plus]))
+                    fin.append('    }')
+                    k += 1
+    s = []
+    s.append('template <> struct EriGradLane<%d, %d, %d, %d> {' % (la, lb, lc, ld))
+    s.append('  static constexpr int L = %d, NACC = %d;' % (L, g['NACC']))
+    s.append('  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;' % (na, nb, nc, nd))
+    s.append('  template <typename R>')
+    s.append('  static __device__ __forceinline__ void prim(R (&acc)[NACC], const R (&F)[L + 1],')
+    s.append('      R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,')
+    s.append('      R WQx, R WQy, R WQz, R o2p, R o2q, R o2pq, R rho_p, R rho_q, R w1, R w2, R w3) {')
+    s.append('    (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz;')
+    s.append('    (void)QCx; (void)QCy; (void)QCz; (void)WQx; (void)WQy; (void)WQz;')
+    s.append('    (void)o2p; (void)o2q; (void)o2pq; (void)rho_p; (void)rho_q; (void)w1; (void)w2; (void)w3;')
+    s += lines
+    s.append('  }')
+    s.append('  // HRR of the weighted sets and contraction with')
+    s.append('  // D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc):  g[3 X + i] += D_k d(ab|cd)_k / dX_i')
+    s.append('  template <typename R>')
+    s.append('  static __device__ __forceinline__ void finish(const R (&acc)[NACC], R ABx, R ABy, R ABz,')
+    s.append('      R CDx, R CDy, R CDz, const R* Pab, const R* Pcd, const R* Pac, const R* Pad,')
+    s.append('      const R* Pbc, const R* Pbd, R f, R (&g)[9]) {')
+    s.append('    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;')
+    s += fin
+    s.append('  }')
+    s.append('};')
+    return '\n'.join(s)
+
+
+def grad_is_lane(c):
+    return grad_program(*c)['NACC'] <= GRAD_LANE_NACC
+
+
 def emit_prog(la, lb, lc, ld):
     g = coop_program(la, lb, lc, ld)
     tag = '%d%d%d%d' % (la, lb, lc, ld)
@@ -644,7 +738,8 @@ def main():
              '#pragma once',
              'template <int LA, int LB, int LC, int LD> struct EriClass;',
              'template <int LA, int LB, int LC, int LD> struct EriProg;',
-             'template <int LA, int LB, int LC, int LD> struct EriGradProg;']
+             'template <int LA, int LB, int LC, int LD> struct EriGradProg;',
+             'template <int LA, int LB, int LC, int LD> struct EriGradLane;']
     lane = [c for c in classes(lmax) if not is_coop(c)]
     coop = [c for c in classes(lmax) if is_coop(c)]
     for c in lane:
@@ -653,6 +748,9 @@ def main():
         parts.append(emit_prog(*c))
     for c in classes(lmax):
         parts.append(emit_grad_prog(*c))
+    glane = [c for c in classes(lmax) if grad_is_lane(c)]
+    for c in glane:
+        parts.append(emit_grad_lane(*c))
     parts.append('#define GX_ERI_LMAX %d' % lmax)
     parts.append('#define GX_FOR_EACH_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in classes(lmax)))
     parts.append('#define GX_FOR_EACH_LANE_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in lane))
@@ -665,6 +763,11 @@ def main():
         'X(%d, %d)' % c for c in pcs if is_coop(c + c)))
     parts.append('constexpr bool gx_class_is_coop(int la, int lb, int lc, int ld) {\n  return false' + ''.join(
         ' ||\n         (la == %d && lb == %d && lc == %d && ld == %d)' % c for c in coop) + ';\n}')
+    parts.append('#define GX_FOR_EACH_GRAD_LANE_CLASS(X) ' + ' '.join('X(%d, %d, %d, %d)' % c for c in glane))
+    parts.append('#define GX_FOR_EACH_GRAD_COOP_CLASS(X) ' + ' '.join(
+        'X(%d, %d, %d, %d)' % c for c in classes(lmax) if c not in glane))
+    parts.append('constexpr bool gx_grad_is_lane(int la, int lb, int lc, int ld) {\n  return false' + ''.join(
+        ' ||\n         (la == %d && lb == %d && lc == %d && ld == %d)' % c for c in glane) + ';\n}')
     text = '\n\n'.join(parts) + '\n'
     with open(out_path, 'w') as f:
         f.write(text)

@@ -16,6 +16,7 @@
 // Single translation unit: the kernels are templates instantiated below.
 #include "gx_eri.hip"
 #include "gx_coop.hip"
+#include "gx_grad.hip"
 #include "gx_one.hip"
 
 namespace {
@@ -379,7 +380,8 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
 }
 
 // Two-electron gradient of one class combination (gx_coop.hip, k_grad_coop).
-size_t grad_lds_bytes(int la, int lb, int lc, int ld) {
+size_t grad_lds_bytes(int la, int lb, int lc, int ld, int N) {
+  if (gx_grad_is_lane(la, lb, lc, ld)) return (size_t)(ncart(la) + ncart(lb)) * N * sizeof(double);
 #define GX_CASE(A, B, C, D)                                                                     \
   if (la == A && lb == B && lc == C && ld == D) {                                               \
     using G = EriGradProg<A, B, C, D>;                                                          \
@@ -392,8 +394,24 @@ size_t grad_lds_bytes(int la, int lb, int lc, int ld) {
 }
 
 hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s, const gx::GradArgs& a,
-                       int* per_cu) {
-  const size_t lds = grad_lds_bytes(la, lb, lc, ld);
+                       int* per_cu, int N) {
+  const size_t lds = grad_lds_bytes(la, lb, lc, ld, N);
+#define GX_CASE(A, B, C, D)                                                                   \
+  if (la == A && lb == B && lc == C && ld == D) {                                             \
+    const void* fn = (const void*)gx::k_grad_lane<A, B, C, D>;                               \
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                            \
+    if (*per_cu <= 0) {                                                                       \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, JK_TEAM, lds);             \
+      if (e != hipSuccess) return e;                                                          \
+      if (*per_cu < 1) *per_cu = 1;                                                           \
+    }                                                                                         \
+    const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
+    hipLaunchKernelGGL((gx::k_grad_lane<A, B, C, D>), dim3(grid), dim3(JK_TEAM), lds, s, a); \
+    return hipGetLastError();                                                                 \
+  }
+  GX_FOR_EACH_GRAD_LANE_CLASS(GX_CASE)
+#undef GX_CASE
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
     const void* fn = (const void*)gx::k_grad_coop<A, B, C, D>;                               \
@@ -408,7 +426,7 @@ hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s
     hipLaunchKernelGGL((gx::k_grad_coop<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a); \
     return hipGetLastError();                                                                 \
   }
-  GX_FOR_EACH_CLASS(GX_CASE)
+  GX_FOR_EACH_GRAD_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   return hipErrorInvalidValue;
 }
@@ -572,7 +590,8 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
   // per bra: virtual layout and per-chunk cost (widest ket of the chunk).
   // Per-lane classes: chunks of 64 kets, four waves per item; cooperative
   // classes: one ket per chunk, one wave per item.
-  const bool coop = grad || combo_is_coop(X, Y);
+  const bool coop = grad ? !gx_grad_is_lane(class_la(X), class_lb(X), class_la(Y), class_lb(Y))
+                         : combo_is_coop(X, Y);
   const int csz = coop ? 1 : GX_WAVE;
   const double waves = coop ? 1.0 : 4.0;
   std::vector<std::vector<double>> chunk(nb);
@@ -1313,7 +1332,7 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y) {
       if (c->cls_start[X + 1] == c->cls_start[X] || c->cls_start[Y + 1] == c->cls_start[Y]) continue;
-      const size_t lds = grad_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y));
+      const size_t lds = grad_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->nmax);
       if (lds > (size_t)c->max_lds)
         return fail(GX_ERR_ARG, "gx_grad: class (" + std::to_string(class_la(X)) + std::to_string(class_lb(X)) +
                                     "|" + std::to_string(class_la(Y)) + std::to_string(class_lb(Y)) +
@@ -1329,8 +1348,16 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
   hipLaunchKernelGGL(gx::k_fill, dim3((3 * c->natom + 255) / 256), dim3(256), 0, c->stream, (long)3 * c->natom,
                      grad, 0.0);
   const gx::OneArgs oa = one_args(c);
-  hipLaunchKernelGGL(gx::k_one_grad, dim3((c->npair + 63) / 64), dim3(64), 0, c->stream, oa, c->d_atom, P, W,
-                     grad);
+  int max_atoms = 0;
+  for (int k = 0; k < c->nsys; ++k) {
+    const int hi = (c->sys_atom.size() == 2 && c->sys_atom[1] < 0) ? c->natom : c->sys_atom[k + 1];
+    max_atoms = std::max(max_atoms, hi - c->sys_atom[k]);
+  }
+  if (max_atoms > GX_GRAD_MAXATOM)
+    return fail(GX_ERR_ARG, "gx_grad: a system has " + std::to_string(max_atoms) + " nuclei (at most " +
+                                std::to_string(GX_GRAD_MAXATOM) + ")");
+  hipLaunchKernelGGL(gx::k_one_grad, dim3((c->npair + 63) / 64, 1 + max_atoms), dim3(64), 0, c->stream, oa,
+                     c->d_atom, P, W, grad);
   hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_atom_sys,
                      c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
   GX_HIP(hipGetLastError());
@@ -1353,9 +1380,10 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
       a.tab64 = c->d_tab64;
       a.P = c->d_Ps;
       a.atom = c->d_atom;
+      a.sys_atom = c->d_sys_atom;
       a.grad = grad;
       GX_HIP(launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, c->stream, a,
-                         &c->wg_per_cu_grad[combo]));
+                         &c->wg_per_cu_grad[combo], c->nmax));
     }
   return GX_OK;
 }

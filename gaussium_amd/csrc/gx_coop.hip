@@ -324,8 +324,14 @@ struct GradArgs {
   const double* tab64;
   const double* P;     // packed densities, pre-scaled by s_k s_l
   const int* atom;     // shell -> atom (over all systems)
+  const int* sys_atom; // per system: first nucleus (nsys + 1)
   double* grad;        // 3 natom
 };
+
+// Gradient contributions of one work item collect in LDS per nucleus of the
+// item's system (at most GX_GRAD_MAXATOM) and go to HBM once per item: the
+// few nuclei of a system would otherwise take every quartet's atomics.
+#define GX_GRAD_MAXATOM 256
 
 template <class G>
 constexpr int grad_lds() {
@@ -342,6 +348,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
   double* __restrict__ H = smem;
   double* __restrict__ pblk = smem + grad_lds<G>();
   __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
+  __shared__ double gsh[3 * GX_GRAD_MAXATOM];
   const int lane = threadIdx.x;
   const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
   const int2* __restrict__ gt = G::gterm();
@@ -355,6 +362,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
     const long mb = a.sy.mb[sys];
     const int s0 = a.cseg[sys];
     const int nseg = a.cseg[sys + 1] - s0;
+    const int at0 = a.sys_atom[sys], nat3 = 3 * (a.sys_atom[sys + 1] - at0);
     const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
     const PrimPair64* __restrict__ bra = a.pr.pp64 + a.pr.poff[ip] + p0;
     const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
@@ -363,8 +371,9 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
     __syncthreads();
     if (lane < nseg) {
       r_start[lane] = a.seg_lo[s0 + lane];
-      r_pre[lane + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + lane];  // FP64 counts; cooperative classes have no FP32 part
+      r_pre[lane + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + lane];  // gradients are FP64 only
     }
+    for (int t = lane; t < nat3; t += GX_WAVE) gsh[t] = 0.0;
     __syncthreads();
     if (lane == 0) {
       r_pre[0] = 0;
@@ -423,18 +432,20 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
         g[j] = x;
       }
       if (lane == 0) {
-        const int at[3] = {a.atom[A], a.atom[B], a.atom[Cs]};
-        const int atD = a.atom[Ds];
+        const int at[3] = {a.atom[A] - at0, a.atom[B] - at0, a.atom[Cs] - at0};
+        const int atD = a.atom[Ds] - at0;
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-          const double gd = -(g[x] + g[3 + x] + g[6 + x]);
-          atomicAdd(&a.grad[3 * at[0] + x], g[x]);
-          atomicAdd(&a.grad[3 * at[1] + x], g[3 + x]);
-          atomicAdd(&a.grad[3 * at[2] + x], g[6 + x]);
-          atomicAdd(&a.grad[3 * atD + x], gd);
+          gsh[3 * at[0] + x] += g[x];
+          gsh[3 * at[1] + x] += g[3 + x];
+          gsh[3 * at[2] + x] += g[6 + x];
+          gsh[3 * atD + x] -= g[x] + g[3 + x] + g[6 + x];
         }
       }
     }
+    __syncthreads();
+    for (int t = lane; t < nat3; t += GX_WAVE)
+      if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * at0 + t], gsh[t]);
   }
 }
 

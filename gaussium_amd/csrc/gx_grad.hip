@@ -1,0 +1,163 @@
+// Per-lane two-electron gradient kernels for the small classes
+// ((ss|ss), (ps|ss), (ps|ps), (pp|ss), (ds|ss)): the quartets of a bra are
+// swept 64 kets at a time, one ket shell pair per lane, as in the fused J/K
+// kernel; the lane accumulates the weighted [e0|f0] sets of EriGradLane,
+// then its HRR and the contraction with
+//   D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc)
+// give the nine derivatives on A, B, C (D by translational invariance).
+// The larger classes use k_grad_coop (gx_coop.hip).
+
+namespace gx {
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
+  using C = EriGradLane<LA, LB, LC, LD>;
+  constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+  extern __shared__ double prow[];  // P rows of the bra's shells
+  __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
+  __shared__ int next_chunk;
+  __shared__ double gsh[3 * GX_GRAD_MAXATOM];
+  const int tid = threadIdx.x;
+  const int lane = tid & (GX_WAVE - 1);
+  const int wave = tid / GX_WAVE;
+  constexpr int nwave = JK_TEAM / GX_WAVE;
+  const double* __restrict__ tab = a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
+  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
+    const int4 it = a.items[w];
+    const int ip = a.bra0 + it.x;
+    const int A = a.pr.A[ip], B = a.pr.B[ip];
+    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
+    const int rowB = (A == B) ? 0 : NA;
+    const int sys = a.sh.sys[A];
+    const int N = a.sy.n[sys];
+    const double* __restrict__ Pm = a.P + a.sy.mb[sys];
+    const int s0 = a.cseg[sys];
+    const int nseg = a.cseg[sys + 1] - s0;
+    const int at0 = a.sys_atom[sys], nat3 = 3 * (a.sys_atom[sys + 1] - at0);
+    const int p0 = __builtin_amdgcn_readfirstlane(it.w & 0xffff);
+    const int nbp = __builtin_amdgcn_readfirstlane((it.w >> 16) - (it.w & 0xffff));
+    const int bpoff = __builtin_amdgcn_readfirstlane(a.pr.poff[ip]) + p0;
+    const PrimPair64* __restrict__ bra = a.pr.pp64 + bpoff;
+    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
+                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
+    __syncthreads();  // the previous item's readers of prow / r_* are done
+    for (int i = tid; i < (NA + NB) * N; i += JK_TEAM) {
+      const int r = i / N, c = i - r * N;
+      prow[i] = Pm[(long)(r < NA ? ao + r : bo + (r - NA)) * N + c];
+    }
+    if (tid == 0) next_chunk = nwave;
+    for (int t = tid; t < nat3; t += JK_TEAM) gsh[t] = 0.0;
+    if (tid < nseg) {
+      r_start[tid] = a.seg_lo[s0 + tid];
+      r_pre[tid + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + tid];  // gradients are FP64 only
+    }
+    __syncthreads();
+    if (tid == 0) {
+      r_pre[0] = 0;
+      for (int r = 0; r < nseg; ++r) r_pre[r + 1] += r_pre[r];
+    }
+    __syncthreads();
+    const int total = min(r_pre[nseg], it.z);
+    const double fA = (A == B) ? 0.5 : 1.0;
+    double gab[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) gab[j] = 0.0;
+    for (int v0 = it.y + wave * GX_WAVE; v0 < total;) {
+      const int v = v0 + lane;
+      int r = 0;
+      if (v < total)
+        while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
+      const bool act = v < total;
+      if (act) {
+        const int jp = r_start[r] + (v - r_pre[r]);
+        const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
+        const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
+        const PrimPair64* __restrict__ kp = a.pr.pp64 + a.pr.poff[jp];
+        const int nk = a.pr.np[jp];
+        double acc[C::NACC];
+#pragma unroll
+        for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
+        for (int ik = 0; ik < nk; ++ik) {
+          const PrimPair64 k = kp[ik];
+          const double w3 = 2.0 * k.ea;
+          for (int ib = 0; ib < nbp; ++ib) {
+            const PrimPair64 b = bra[ib];
+            const double rs = gx_rsqrt(b.p + k.p);
+            const double is = rs * rs;
+            const double rho = b.p * k.p * is;
+            const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            double F[C::L + 1];
+            boys64_nb<C::L>(T, b.Kip * k.Kip * rs, tab, F);
+            const double rho_p = rho * b.ip, rho_q = rho * k.ip;
+            C::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
+                                     k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip,
+                                     0.5 * k.ip, 0.5 * is, rho_p, rho_q, 2.0 * b.ea, 2.0 * (b.p - b.ea), w3);
+          }
+        }
+        double Pab[NA * NB], Pcd[NC * ND], Pac[NA * NC], Pad[NA * ND], Pbc[NB * NC], Pbd[NB * ND];
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) Pab[ia * NB + ib] = prow[ia * N + bo + ib];
+#pragma unroll
+          for (int ic = 0; ic < NC; ++ic) Pac[ia * NC + ic] = prow[ia * N + co + ic];
+#pragma unroll
+          for (int id = 0; id < ND; ++id) Pad[ia * ND + id] = prow[ia * N + dof + id];
+        }
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+          for (int ic = 0; ic < NC; ++ic) Pbc[ib * NC + ic] = prow[(rowB + ib) * N + co + ic];
+#pragma unroll
+          for (int id = 0; id < ND; ++id) Pbd[ib * ND + id] = prow[(rowB + ib) * N + dof + id];
+        }
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pm[(long)(co + ic) * N + dof + id];
+        double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
+        if (jp == ip) f *= 0.5;
+        double g[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) g[j] = 0.0;
+        C::template finish<double>(acc, ABx, ABy, ABz, a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds],
+                                   a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
+                                   a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], Pab, Pcd, Pac, Pad, Pbc, Pbd, f,
+                                   g);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) gab[j] += g[j];
+        const int atC = a.atom[Cs] - at0, atD = a.atom[Ds] - at0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          atomicAdd(&gsh[3 * atC + x], g[6 + x]);
+          atomicAdd(&gsh[3 * atD + x], -(g[x] + g[3 + x] + g[6 + x]));
+        }
+      }
+      int ck = 0;
+      if (lane == 0) ck = atomicAdd(&next_chunk, 1);
+      v0 = it.y + __builtin_amdgcn_readfirstlane(ck) * GX_WAVE;
+    }
+    // A and B are the item's bra: one wave reduction, one atomic per wave
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double x = gab[j];
+#pragma unroll
+      for (int off = GX_WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, GX_WAVE);
+      gab[j] = x;
+    }
+    if (lane == 0) {
+      const int atA = a.atom[A] - at0, atB = a.atom[B] - at0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        atomicAdd(&gsh[3 * atA + x], gab[x]);
+        atomicAdd(&gsh[3 * atB + x], gab[3 + x]);
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < nat3; t += JK_TEAM)
+      if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * at0 + t], gsh[t]);
+  }
+}
+
+}  // namespace gx

@@ -310,8 +310,11 @@ __device__ inline void hermite_e_g(int imax, int jmax, double PA, double PB, dou
 __global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restrict__ satom,
                                                  const double* __restrict__ P, const double* __restrict__ W,
                                                  double* __restrict__ grad) {
+  // grid: x over shell pairs, y over the parts of one pair -- 0: overlap and
+  // kinetic, y >= 1: the nuclear attraction of the system's (y - 1)-th nucleus
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.npair) return;
+  const int slot = blockIdx.y;
   const int si = a.pA[t], sj = a.pB[t];
   const int* __restrict__ sl = a.sl;
   const int* __restrict__ snp = a.snp;
@@ -324,6 +327,7 @@ __global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restric
   const double* __restrict__ master = a.master;
   const int sys = a.ssys[si];
   const int c0 = a.sys_atom ? a.sys_atom[sys] : 0, c1 = a.sys_atom ? a.sys_atom[sys + 1] : 0;
+  if (slot > 0 && c0 + slot - 1 >= c1) return;
   const int la = sl[si], lb = sl[sj];
   const int na = ncart_d(la), nb = ncart_d(lb);
   const double A[3] = {ctr1[3 * si], ctr1[3 * si + 1], ctr1[3 * si + 2]};
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restric
   double F[ONE_RDIM + 1];
   double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
   // overlap and kinetic: depend on A - B only
-  for (int ia = 0; ia < snp[si]; ++ia) {
+  for (int ia = 0; slot == 0 && ia < snp[si]; ++ia) {
     const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
     for (int ib = 0; ib < snp[sj]; ++ib) {
       const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
@@ -398,9 +402,9 @@ __global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restric
       }
     }
   }
-  // nuclear attraction, one nucleus at a time
+  // nuclear attraction of one nucleus
   const int L = la + lb + 1;
-  for (int c = c0; c < c1; ++c) {
+  for (int c = c0 + slot - 1; slot > 0 && c == c0 + slot - 1; ++c) {
     double gC[3] = {0.0, 0.0, 0.0};
     for (int ia = 0; ia < snp[si]; ++ia) {
       const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];

@@ -347,6 +347,72 @@ extern "C" int host_quartet_grad(const int* l, const double* ctr, const int* n, 
   return -1;
 }
 
+// Per-lane gradient code (EriGradLane): g[3 X + i] of the quartet contracted
+// with D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc); blocks in `pb` in the
+// order P_ab, P_cd, P_ac, P_ad, P_bc, P_bd.
+template <int LA, int LB, int LC, int LD>
+static int quartet_grad_lane(const Shell& A, const Shell& B, const Shell& C, const Shell& D, const double* pb,
+                             double f, double* out) {
+  using G = EriGradLane<LA, LB, LC, LD>;
+  double acc[G::NACC] = {0};
+  const double two_pi_52 = 34.98683665524972;
+  for (int i = 0; i < A.n; ++i)
+    for (int j = 0; j < B.n; ++j) {
+      const double a = A.a[i], b = B.a[j], p = a + b;
+      double P[3], ab2 = 0;
+      for (int x = 0; x < 3; ++x) {
+        P[x] = (a * A.c[x] + b * B.c[x]) / p;
+        ab2 += (A.c[x] - B.c[x]) * (A.c[x] - B.c[x]);
+      }
+      const double Kab = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2);
+      for (int k = 0; k < C.n; ++k)
+        for (int l = 0; l < D.n; ++l) {
+          const double c = C.a[k], d = D.a[l], q = c + d;
+          double Q[3], cd2 = 0, pq2 = 0;
+          for (int x = 0; x < 3; ++x) {
+            Q[x] = (c * C.c[x] + d * D.c[x]) / q;
+            cd2 += (C.c[x] - D.c[x]) * (C.c[x] - D.c[x]);
+            pq2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+          }
+          const double Kcd = C.w[k] * D.w[l] * std::exp(-c * d / q * cd2);
+          const double rho = p * q / (p + q);
+          const double pref = two_pi_52 / (p * q * std::sqrt(p + q)) * Kab * Kcd;
+          double F[G::L + 1];
+          boys_host(G::L, rho * pq2, pref, F);
+          const double rp = rho / p, rq = rho / q;
+          double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+          G::template prim<double>(acc, F, P[0] - A.c[0], P[1] - A.c[1], P[2] - A.c[2], -rp * PQ[0], -rp * PQ[1],
+                                   -rp * PQ[2], Q[0] - C.c[0], Q[1] - C.c[1], Q[2] - C.c[2], rq * PQ[0],
+                                   rq * PQ[1], rq * PQ[2], 0.5 / p, 0.5 / q, 0.5 / (p + q), rp, rq, 2.0 * a,
+                                   2.0 * b, 2.0 * c);
+        }
+    }
+  const int nab = G::NA * G::NB, ncd = G::NC * G::ND, nac = G::NA * G::NC, nad = G::NA * G::ND,
+            nbc = G::NB * G::NC;
+  const double* Pab = pb;
+  const double* Pcd = Pab + nab;
+  const double* Pac = Pcd + ncd;
+  const double* Pad = Pac + nac;
+  const double* Pbc = Pad + nad;
+  const double* Pbd = Pbc + nbc;
+  double g[9] = {0};
+  G::template finish<double>(acc, A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2], C.c[0] - D.c[0],
+                             C.c[1] - D.c[1], C.c[2] - D.c[2], Pab, Pcd, Pac, Pad, Pbc, Pbd, f, g);
+  for (int j = 0; j < 9; ++j) out[j] = g[j];
+  return 0;
+}
+
+extern "C" int host_grad_lane(const int* l, const double* ctr, const int* n, const double* a, const double* w,
+                              const double* pb, double f, double* out) {
+  Shell s[4];
+  unpack(n, ctr, a, w, s);
+#define GX_CASE(A, B, C, D) \
+  if (l[0] == A && l[1] == B && l[2] == C && l[3] == D) return quartet_grad_lane<A, B, C, D>(s[0], s[1], s[2], s[3], pb, f, out);
+  GX_FOR_EACH_GRAD_LANE_CLASS(GX_CASE)
+#undef GX_CASE
+  return -1;
+}
+
 // Every class through its cooperative program (returns < 0 on a malformed program).
 extern "C" int host_quartet_prog(const int* l, const double* ctr, const int* n, const double* a,
                                  const double* w, double* out) {

@@ -166,3 +166,39 @@ def test_gradient_program_matches_finite_differences(host_lib, cls):
         fdD.append((8 * (vals[1] - vals[2]) - (vals[0] - vals[3])) / (12 * h))
     for i in range(3):
         assert np.abs(-got[:, :, i].sum(axis=1) - fdD[i]).max() <= 1e-8 * scale
+
+
+LANE_GRAD_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0)]
+
+
+@pytest.mark.parametrize('cls', LANE_GRAD_CLASSES, ids=lambda c: '%d%d%d%d' % c)
+def test_lane_gradient_code_matches_program(host_lib, cls):
+    """The per-lane gradient code (straight-line VRR, renamed HRR, contraction
+    with D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc)) equals the gradient
+    program's derivative integrals contracted with the same D_k."""
+    host_lib.host_grad_lane.restype = ctypes.c_int
+    rng = np.random.default_rng(sum(v * 10 ** i for i, v in enumerate(cls)) + 53)
+    centers = rng.uniform(-1.2, 1.2, (4, 3))
+    shells = [_shell(rng, l, c) for l, c in zip(cls, centers)]
+    l = np.array([s[0] for s in shells], dtype=np.int32)
+    ctr = np.ascontiguousarray(np.concatenate([s[1] for s in shells]))
+    n = np.array([len(s[2]) for s in shells], dtype=np.int32)
+    a = np.ascontiguousarray(np.concatenate([s[2] for s in shells]))
+    w = np.ascontiguousarray(np.concatenate([s[3] for s in shells]))
+    NA, NB, NC, ND = (len(COMPS[x]) for x in cls)
+    nout = NA * NB * NC * ND
+    d = np.zeros(9 * nout)
+    p_ = lambda v: v.ctypes.data_as(ctypes.c_void_p)
+    assert host_lib.host_quartet_grad(p_(l), p_(ctr), p_(n), p_(a), p_(w), p_(d)) == 0
+    d = d.reshape(NA, NB, NC, ND, 9)
+    blocks = [rng.standard_normal(k) for k in (NA * NB, NC * ND, NA * NC, NA * ND, NB * NC, NB * ND)]
+    Pab, Pcd, Pac, Pad, Pbc, Pbd = (b.reshape(s) for b, s in zip(
+        blocks, [(NA, NB), (NC, ND), (NA, NC), (NA, ND), (NB, NC), (NB, ND)]))
+    f = 0.5
+    D = f * (4 * np.einsum('ab,cd->abcd', Pab, Pcd) - np.einsum('ac,bd->abcd', Pac, Pbd)
+             - np.einsum('ad,bc->abcd', Pad, Pbc))
+    want = np.einsum('abcd,abcdj->j', D, d)
+    pb = np.ascontiguousarray(np.concatenate(blocks))
+    out = np.zeros(9)
+    assert host_lib.host_grad_lane(p_(l), p_(ctr), p_(n), p_(a), p_(w), p_(pb), ctypes.c_double(f), p_(out)) == 0
+    assert np.abs(out - want).max() <= 1e-11 * max(1.0, np.abs(want).max())

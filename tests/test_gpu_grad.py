@@ -172,3 +172,40 @@ def test_autograd_function(surface_c2h4):
     (gx,) = torch.autograd.grad(2.0 * e, xyz)
     _, g = s.energy_and_gradient(s.xyz0)
     assert torch.allclose(gx, 2.0 * torch.from_numpy(g), atol=1e-9, rtol=0)
+
+
+def test_geometry_optimisation_heh():
+    """start(..., geometry_optimization=...) on HeH+/STO-3G: the gradient-driven
+    optimiser reaches the bond length that minimises a fine energy scan."""
+    from gaussium_amd import start
+    from gaussium_amd.basis import read_basis_set_file, read_mol_file
+    from gaussium_amd.gradient import RHFSurface
+    from tests.helpers import golden
+    e_opt = start(golden('HeH+.mol'), golden('STO-3G.gbs'), 'RHF', 1, geometry_optimization='NelderMead')
+    nuc, ne, _ = read_mol_file(golden('HeH+.mol'))
+    basis = read_basis_set_file(golden('STO-3G.gbs'), nuc)
+    s = RHFSurface(nuc, basis, ne)
+    x0 = s.xyz0.copy()
+    axis = (x0[1] - x0[0]) / np.linalg.norm(x0[1] - x0[0])
+
+    def at(r):
+        x = x0.copy()
+        x[1] = x0[0] + r * axis
+        return s.energy(x)
+
+    lo, hi = 1.0, 2.5  # golden-section search on the bond length
+    phi = (np.sqrt(5) - 1) / 2
+    a, b = hi - phi * (hi - lo), lo + phi * (hi - lo)
+    fa, fb = at(a), at(b)
+    for _ in range(40):
+        if fa < fb:
+            hi, b, fb = b, a, fa
+            a = hi - phi * (hi - lo)
+            fa = at(a)
+        else:
+            lo, a, fa = a, b, fb
+            b = lo + phi * (hi - lo)
+            fb = at(b)
+    e_scan = min(fa, fb)
+    assert e_opt <= e_scan + 1e-7
+    assert abs(e_opt - e_scan) < 1e-6
