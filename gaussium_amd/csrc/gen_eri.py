@@ -604,7 +604,7 @@ def emit_grad_prog(la, lb, lc, ld):
 # Per-lane gradient code for the small classes: the gradient program written
 # out as straight-line code (each VRR slot a register, the HRR slots renamed
 # on every write), one shell quartet per lane.
-GRAD_LANE_NACC = 70  # gradient classes with at most this many weighted intermediates run per lane
+GRAD_LANE_NACC = 200  # gradient classes with at most this many weighted intermediates run per lane
 
 
 def emit_grad_lane(la, lb, lc, ld):
