@@ -447,7 +447,8 @@ size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed) {
   GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   const int na = ncart(la), nb = ncart(lb);
-  size_t b = mixed ? 0 : (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
+  size_t b = mixed ? (GX_MIXED_LDS32 ? (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float) : 0)
+                   : (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
   return b + 2 * (size_t)(na + nb) * N * sizeof(double);
 }
 
@@ -593,7 +594,7 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
   const bool coop = grad ? !gx_grad_is_lane(class_la(X), class_lb(X), class_la(Y), class_lb(Y))
                          : combo_is_coop(X, Y);
   const int csz = coop ? 1 : GX_WAVE;
-  const double waves = coop ? 1.0 : 4.0;
+  const double waves = coop ? 1.0 : (double)(JK_TEAM / GX_WAVE);
   std::vector<std::vector<double>> chunk(nb);
   double total = 0.0;
   std::vector<int> vk;  // virtual index -> ket pair id (-1 = padding)

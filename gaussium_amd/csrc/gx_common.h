@@ -70,8 +70,11 @@ struct PairView {
   const PrimPair32* pp32;
 };
 
-// Reciprocal and reciprocal square root to full FP64 precision from the
-// hardware estimates plus Newton steps (quadratic convergence).
+// Reciprocal and reciprocal square root from the hardware estimates (about
+// 2^-24 relative on gfx950, measured) plus Newton steps (quadratic
+// convergence): two steps reach full FP64 precision for the reciprocal; one
+// step leaves the reciprocal square root within ~2^-47 (4e-15) relative,
+// far below the 1e-10 absolute bar of the integrals it scales.
 __device__ __forceinline__ double gx_rcp(double x) {
   double r = __builtin_amdgcn_rcp(x);
   double e = fma(-x, r, 1.0);
@@ -80,14 +83,20 @@ __device__ __forceinline__ double gx_rcp(double x) {
   return fma(r, e, r);
 }
 
+#ifndef GX_RSQ_NR
+#define GX_RSQ_NR 1  // Newton steps after the hardware estimate
+#endif
+
 __device__ __forceinline__ double gx_rsqrt(double x) {
   double y = __builtin_amdgcn_rsq(x);
-  // y <- y (3 - x y^2) / 2, twice
-  double h = 0.5 * x;
-  double t = fma(-h * y, y, 0.5);
-  y = fma(y, t, y);
-  t = fma(-h * y, y, 0.5);
-  return fma(y, t, y);
+  // y <- y (3 - x y^2) / 2
+  const double h = 0.5 * x;
+#pragma unroll
+  for (int k = 0; k < GX_RSQ_NR; ++k) {
+    const double t = fma(-h * y, y, 0.5);
+    y = fma(y, t, y);
+  }
+  return y;
 }
 
 __device__ __forceinline__ float gx_rcp(float x) { return __builtin_amdgcn_rcpf(x); }

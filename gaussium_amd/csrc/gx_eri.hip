@@ -153,6 +153,9 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 #ifndef GX_JKG32
 #define GX_JKG32 2
 #endif
+#ifndef GX_MIXED_LDS32
+#define GX_MIXED_LDS32 1  // mixed builds: FP32 Taylor table in LDS (else read through L1)
+#endif
 #ifndef GX_DYN_CHUNK
 #define GX_DYN_CHUNK 1
 #endif
@@ -350,7 +353,9 @@ __global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int 
 //                            whole wave would otherwise hit the same two rows)
 // with f the 8-fold degeneracy weight; J = Jt + Jt^T, K = Kt + Kt^T.
 #define JK_MAXSEG 16
-#define JK_TEAM 256  // threads working on one bra
+#ifndef JK_TEAM
+#define JK_TEAM 256  // threads working on one work item
+#endif
 
 template <int L>
 struct JKShape {
@@ -381,7 +386,12 @@ struct JKArgs {
 // FP32 and both tables are read through the vector-memory path (L1), which
 // leaves the LDS to the P/K rows and raises occupancy.
 template <int LA, int LB, int LC, int LD, bool MIXED>
-__global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
+#ifdef GX_JK_WPE
+#define GX_JK_ATTR __attribute__((amdgpu_waves_per_eu(GX_JK_WPE)))
+#else
+#define GX_JK_ATTR
+#endif
+__global__ __launch_bounds__(JK_TEAM) GX_JK_ATTR void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
                                                 const PrimPair32* __restrict__ pp32) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
@@ -393,8 +403,14 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
   // dynamic LDS: Boys FP64 (pure FP64 build) | P rows | K rows (rows of the
   // item's system)
   const double* __restrict__ tab64 = !MIXED ? smem : a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
+#if GX_MIXED_LDS32
+  const float* __restrict__ tab32 = MIXED ? reinterpret_cast<const float*>(smem) : a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
+  double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : BOYS_NT * BOYS_ROW32 / 2);
+  if (MIXED) boys_stage32(C::L, a.master, reinterpret_cast<float*>(smem));
+#else
   const float* __restrict__ tab32 = a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
   double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : 0);
+#endif
 
   if (!MIXED) boys_stage64(C::L, a.master, smem);
 
