@@ -26,6 +26,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0
 CUTOFF = 1e-11
 FP32_BOUND = 1e-4
+PRIM_EPS = 1e-18  # primitive-pair screening threshold (the library default)
 COPY_BYTES = 256 << 20
 
 
@@ -269,7 +270,7 @@ def main():
     from gaussium_amd.hartree_fock import RestrictedHF
 
     nuc, basis, ne, info = workloads.cluster_220bf(seed=args.seed)
-    eng = Engine(basis, nuc)
+    eng = Engine(basis, nuc, prim_eps=PRIM_EPS)
     e_nuc = nuclear_repulsion(nuc)
 
     # SCF in both modes (untimed set-up).  The reference's DIIS does not
@@ -293,6 +294,16 @@ def main():
         J, K = eng.jk(P, cutoff=CUTOFF, **kw)
         energy_at[mode] = 0.5 * float(torch.sum(P * (2.0 * H + J - 0.5 * K))) + e_nuc
     dE = abs(energy_at['mixed'] - energy_at['fp64'])
+    # the same FP64 build over every primitive pair (no primitive-pair
+    # screening): energy change and time, reported beside the headline
+    full = Engine(basis, nuc, prim_eps=0.0)
+    J, K = full.jk(P, cutoff=CUTOFF)
+    e_full = 0.5 * float(torch.sum(P * (2.0 * H + J - 0.5 * K))) + e_nuc
+    fp64_full = timed_builds(full, P, args.steps, args.warmup, cutoff=CUTOFF)
+    prim = {'kept': eng.n_primitive_pairs, 'all': full.n_primitive_pairs, 'threshold': PRIM_EPS,
+            'dE_fp64_vs_unscreened_Eh': abs(energy_at['fp64'] - e_full),
+            'jk_ms_fp64_unscreened': fp64_full['median_ms']}
+    full.close()
     fp64 = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF)
     mixed = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
     st64 = eng.jk_stats(cutoff=CUTOFF)
@@ -344,6 +355,7 @@ def main():
         'dE_mixed_Eh': dE,
         'dE_definition': 'fixed density: |E[P] mixed J/K - E[P] FP64 J/K|, P = final FP64 SCF density',
         'energy_at_density': energy_at,
+        'primitive_pairs': prim,
         'scf': {m: {'energy': scf[m]['energy'], 'iterations': scf[m]['iterations'],
                     'converged': scf[m]['converged'], 'wall_s': scf[m]['wall_s']} for m in scf},
         'quartets': {'canonical_bf': st64['bf_quartets'], 'canonical_shell': st64['shell_quartets'],

@@ -41,6 +41,8 @@ SIGNATURES = {
     'gx_set_nuclei': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     'gx_nbf': (ctypes.c_int, [ctypes.c_void_p]),
     'gx_npair': (ctypes.c_int, [ctypes.c_void_p]),
+    'gx_set_prim_screen': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    'gx_nprimpair': (ctypes.c_int, [ctypes.c_void_p]),
     'gx_schwarz': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     'gx_eri_quartets': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
     'gx_eri_dense': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

@@ -126,6 +126,10 @@ struct gx_ctx {
   // pairs
   int npair = 0, npp = 0;
   std::vector<int> pA, pB, pcls, ppoff, pnp;
+  // Primitive-pair screening: pair p keeps the primitive pairs
+  // kept_ab[kept_off[p] .. kept_off[p + 1]) (ia << 16 | ib); empty = all.
+  double prim_eps = GX_PRIM_EPS_DEFAULT;
+  std::vector<int> kept_off, kept_ab;
   std::vector<double> Q;
   std::vector<int> cls_start;  // NCLS + 1
   std::vector<int> pairmap;    // nshell*nshell -> pair id
@@ -457,17 +461,27 @@ int build_primpairs(gx_ctx* c) {
   std::vector<int> pair_of, pia, pib;
   c->ppoff.assign(c->npair, 0);
   c->pnp.assign(c->npair, 0);
+  const bool all = c->kept_off.empty();
   int off = 0;
   for (int p = 0; p < c->npair; ++p) {
     const int A = c->pA[p], B = c->pB[p];
     c->ppoff[p] = off;
-    c->pnp[p] = c->np[A] * c->np[B];
-    for (int i = 0; i < c->np[A]; ++i)
-      for (int j = 0; j < c->np[B]; ++j) {
+    if (all) {
+      c->pnp[p] = c->np[A] * c->np[B];
+      for (int i = 0; i < c->np[A]; ++i)
+        for (int j = 0; j < c->np[B]; ++j) {
+          pair_of.push_back(p);
+          pia.push_back(i);
+          pib.push_back(j);
+        }
+    } else {
+      c->pnp[p] = c->kept_off[p + 1] - c->kept_off[p];
+      for (int k = c->kept_off[p]; k < c->kept_off[p + 1]; ++k) {
         pair_of.push_back(p);
-        pia.push_back(i);
-        pib.push_back(j);
+        pia.push_back(c->kept_ab[k] >> 16);
+        pib.push_back(c->kept_ab[k] & 0xffff);
       }
+    }
     off += c->pnp[p];
   }
   c->npp = off;
@@ -499,6 +513,74 @@ int build_primpairs(gx_ctx* c) {
   if (e != hipSuccess || e2 != hipSuccess)
     return fail(GX_ERR_HIP, std::string("k_primpairs: ") + hipGetErrorString(e != hipSuccess ? e : e2));
   return GX_OK;
+}
+
+// Primitive-pair screening.  For every primitive pair (a, b) of every shell
+// pair, S_ab = sqrt(max over components |(ab|ab)|) of the scaled primitive
+// product (the Schwarz kernels run on a view in which each primitive pair is
+// its own shell pair).  By the Schwarz inequality a primitive quartet is
+// bounded by S_ab S_cd <= S_ab S_max, so a primitive pair with
+// S_ab S_max < prim_eps changes no contracted integral by more than
+// prim_eps times its number of primitive quartets; it is dropped.  Each shell
+// pair keeps at least its largest primitive pair.  Expects the records of
+// every primitive pair (kept_off empty); leaves kept_off / kept_ab and the
+// screened records.
+int screen_primpairs(gx_ctx* c) {
+  const int npp = c->npp;
+  std::vector<int> vA(npp), vB(npp), voff(npp), vnp(npp, 1);
+  for (int p = 0; p < c->npair; ++p)
+    for (int k = 0; k < c->pnp[p]; ++k) {
+      vA[c->ppoff[p] + k] = c->pA[p];
+      vB[c->ppoff[p] + k] = c->pB[p];
+    }
+  std::iota(voff.begin(), voff.end(), 0);
+  int *d_vA = nullptr, *d_vB = nullptr, *d_voff = nullptr, *d_vnp = nullptr;
+  double* d_S = nullptr;
+  std::vector<double> S(npp, 0.0);
+  int rc = GX_OK;
+  auto run = [&]() -> int {
+    GX_TRY(upload(&d_vA, vA, c->stream));
+    GX_TRY(upload(&d_vB, vB, c->stream));
+    GX_TRY(upload(&d_voff, voff, c->stream));
+    GX_TRY(upload(&d_vnp, vnp, c->stream));
+    GX_TRY(dalloc(&d_S, (size_t)npp));
+    const gx::PairView pv{d_vA, d_vB, d_voff, d_vnp, nullptr, c->d_pp64, c->d_pp32};
+    for (int k = 0; k < NCLS; ++k) {
+      const int p0 = c->cls_start[k], p1 = c->cls_start[k + 1];
+      if (p1 == p0) continue;
+      const int q0 = c->ppoff[p0], q1 = c->ppoff[p1 - 1] + c->pnp[p1 - 1];
+      if (q1 == q0) continue;
+      GX_HIP(launch_schwarz(class_la(k), class_lb(k), c->stream, c->shells(), pv, q0, q1 - q0, c->d_boys,
+                            c->d_tab64, c->d_scale, d_S));
+    }
+    GX_HIP(hipMemcpyAsync(S.data(), d_S, npp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GX_HIP(hipStreamSynchronize(c->stream));
+    return GX_OK;
+  };
+  rc = run();
+  dfree(&d_vA);
+  dfree(&d_vB);
+  dfree(&d_voff);
+  dfree(&d_vnp);
+  dfree(&d_S);
+  if (rc != GX_OK) return rc;
+  double smax = 0.0;
+  for (double v : S) smax = std::max(smax, v);
+  c->kept_off.assign(c->npair + 1, 0);
+  c->kept_ab.clear();
+  for (int p = 0; p < c->npair; ++p) {
+    const int nb = c->np[c->pB[p]];
+    int best = 0;
+    const size_t first = c->kept_ab.size();
+    for (int k = 0; k < c->pnp[p]; ++k) {
+      const double s = S[c->ppoff[p] + k];
+      if (s > S[c->ppoff[p] + best]) best = k;
+      if (s * smax >= c->prim_eps) c->kept_ab.push_back((k / nb) << 16 | (k % nb));
+    }
+    if (c->kept_ab.size() == first) c->kept_ab.push_back((best / nb) << 16 | (best % nb));
+    c->kept_off[p + 1] = (int)c->kept_ab.size();
+  }
+  return build_primpairs(c);
 }
 
 int compute_schwarz(gx_ctx* c) {
@@ -757,6 +839,7 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed) {
   a.P = c->d_Ps;
   a.Jt = c->d_Jt;
   a.Kt = c->d_Kt;
+  a.jstride = c->mtot;
   a.mixed = mixed;
   if (c->wg_per_cu.empty()) c->wg_per_cu.assign(2 * combo_count(), 0);
   return launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
@@ -923,7 +1006,7 @@ int gx_set_basis_batch(gx_ctx* c, int nsys, const int* sys_shell, const int* sys
   GX_TRY(upload(&c->d_sbfb, c->sbfb, c->stream));
   GX_TRY(upload(&c->d_smb, c->smb, c->stream));
   GX_TRY(dalloc(&c->d_Ps, (size_t)c->mtot));
-  GX_TRY(dalloc(&c->d_Jt, (size_t)c->mtot));
+  GX_TRY(dalloc(&c->d_Jt, (size_t)c->mtot * GX_JT_COPIES));
   GX_TRY(dalloc(&c->d_Kt, (size_t)c->mtot));
 
   // Shell pairs inside each system, oriented l_A >= l_B, sorted by class,
@@ -958,7 +1041,10 @@ int gx_set_basis_batch(gx_ctx* c, int nsys, const int* sys_shell, const int* sys
     while (p < c->npair && c->pcls[p] < k) ++p;
     c->cls_start[k] = p;
   }
+  c->kept_off.clear();
+  c->kept_ab.clear();
   GX_TRY(build_primpairs(c));
+  if (c->prim_eps > 0.0) GX_TRY(screen_primpairs(c));
   GX_TRY(compute_schwarz(c));
 
   // Segment key per pair: its primitive-pair count, or floor(log2) of it when
@@ -1009,6 +1095,17 @@ int gx_set_basis_batch(gx_ctx* c, int nsys, const int* sys_shell, const int* sys
     c->pcls.swap(C2);
     c->Q.swap(Q2);
     key.swap(K2);
+    if (!c->kept_off.empty()) {
+      std::vector<int> off2(c->npair + 1, 0), ab2;
+      ab2.reserve(c->kept_ab.size());
+      for (int p = 0; p < c->npair; ++p) {
+        const int q = perm[p];
+        ab2.insert(ab2.end(), c->kept_ab.begin() + c->kept_off[q], c->kept_ab.begin() + c->kept_off[q + 1]);
+        off2[p + 1] = (int)ab2.size();
+      }
+      c->kept_off.swap(off2);
+      c->kept_ab.swap(ab2);
+    }
   }
   GX_TRY(build_primpairs(c));
   GX_TRY(upload(&c->d_Q, c->Q, c->stream));
@@ -1083,6 +1180,15 @@ int gx_set_nuclei(gx_ctx* c, int natom, const double* xyz, const double* charge)
 
 int gx_nbf(gx_ctx* c) { return c ? c->nbf : fail(GX_ERR_ARG, "gx_nbf: null context"); }
 int gx_npair(gx_ctx* c) { return c ? c->npair : fail(GX_ERR_ARG, "gx_npair: null context"); }
+
+int gx_set_prim_screen(gx_ctx* c, double eps) {
+  if (!c) return fail(GX_ERR_ARG, "gx_set_prim_screen: null context");
+  if (!(eps >= 0.0)) return fail(GX_ERR_ARG, "gx_set_prim_screen: threshold must be >= 0");
+  c->prim_eps = eps;
+  return GX_OK;
+}
+
+int gx_nprimpair(gx_ctx* c) { return c ? c->npp : fail(GX_ERR_ARG, "gx_nprimpair: null context"); }
 
 int gx_schwarz(gx_ctx* c, double* Q, int* shellA, int* shellB) {
   if (!c) return fail(GX_ERR_ARG, "gx_schwarz: null context");
@@ -1225,7 +1331,8 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   const unsigned g = (unsigned)((nn + 255) / 256);
   hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, P, c->d_Ps);
-  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Jt, 0.0);
+  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((nn * GX_JT_COPIES + 255) / 256)), dim3(256), 0, c->stream,
+                     nn * GX_JT_COPIES, c->d_Jt, 0.0);
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, fp32_bound, mixed));

@@ -18,6 +18,11 @@
 
 #define GX_WAVE 64
 
+// Copies of the J accumulator (see jt_copy in gx_eri.hip).
+#ifndef GX_JT_COPIES
+#define GX_JT_COPIES 8
+#endif
+
 // Boys-function interpolation grid.
 #define BOYS_DT 0.1
 #define BOYS_INV_DT 10.0

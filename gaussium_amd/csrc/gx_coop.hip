@@ -291,8 +291,8 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
         int u = t;
         double* dst = a.Kt;
         double m = 1.0;
-        if (u < SAB) { r0 = ao; c0 = bo; s = NB; dst = a.Jt; m = 2.0; }
-        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; dst = a.Jt; m = 2.0; }
+        if (u < SAB) { r0 = ao; c0 = bo; s = NB; dst = jt_copy(a); m = 2.0; }
+        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; dst = jt_copy(a); m = 2.0; }
         else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; s = NC; }
         else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
         else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }

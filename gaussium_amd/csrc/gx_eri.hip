@@ -376,10 +376,19 @@ struct JKArgs {
   const double* tab64;   // per-L FP64 Taylor tables (BOYS_NT x BOYS_ROW64 each)
   const float* tab32;    // per-L FP32 Taylor tables
   const double* P;       // packed densities, pre-scaled by s_k s_l
-  double* Jt;            // packed accumulators
+  double* Jt;            // packed accumulators: GX_JT_COPIES copies of jstride doubles for Jt
   double* Kt;
+  long jstride;
   int mixed;
 };
+
+// Workgroups are dispatched to the eight XCDs round-robin, so copy
+// blockIdx % GX_JT_COPIES of Jt collects the J atomics of (mostly) one XCD:
+// the same address is then not contended from every XCD at once.  The copies
+// are summed by k_symmetrize.
+__device__ __forceinline__ double* jt_copy(const JKArgs& a) {
+  return a.Jt + (long)(blockIdx.x % GX_JT_COPIES) * a.jstride;
+}
 
 // MIXED = false: pure FP64 build, the FP32 path is compiled out and the FP64
 // Taylor table is staged in LDS.  MIXED = true: most primitive quartets are
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(JK_TEAM) GX_JK_ATTR void k_jk(JKArgs a, const PrimP
     const int sys = a.sh.sys[A];
     const int N = a.sy.n[sys];
     const double* __restrict__ Pm = a.P + a.sy.mb[sys];
-    double* __restrict__ Jm = a.Jt + a.sy.mb[sys];
+    double* __restrict__ Jm = jt_copy(a) + a.sy.mb[sys];
     double* __restrict__ Km = a.Kt + a.sy.mb[sys];
     double* krow = prow + (NA + NB) * N;
     const int s0 = a.cseg[sys];
@@ -638,7 +647,8 @@ __device__ __forceinline__ void packed_index(long t, int nsys, const long* __res
   j = (int)(u - (long)i * n[s]);
 }
 
-// J = s (Jt + Jt^T) s, K = s (Kt + Kt^T) s per system (scale by global index).
+// J = s (Jt + Jt^T) s, K = s (Kt + Kt^T) s per system (scale by global
+// index); Jt is the sum of its GX_JT_COPIES copies (stride `total`).
 __global__ void k_symmetrize(long total, int nsys, const long* __restrict__ mb, const int* __restrict__ n,
                              const int* __restrict__ bfb, const double* __restrict__ scale,
                              const double* __restrict__ Jt, const double* __restrict__ Kt,
@@ -649,7 +659,10 @@ __global__ void k_symmetrize(long total, int nsys, const long* __restrict__ mb, 
   packed_index(t, nsys, mb, n, s, i, j);
   const double f = scale[bfb[s] + i] * scale[bfb[s] + j];
   const long tt = mb[s] + (long)j * n[s] + i;
-  J[t] = f * (Jt[t] + Jt[tt]);
+  double x = 0.0;
+#pragma unroll
+  for (int k = 0; k < GX_JT_COPIES; ++k) x += Jt[k * total + t] + Jt[k * total + tt];
+  J[t] = f * x;
   K[t] = f * (Kt[t] + Kt[tt]);
 }
 

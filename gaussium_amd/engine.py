@@ -24,10 +24,12 @@ class Engine:
 
     ``Engine(basis, nuclei)`` packs the contracted functions into shells,
     builds primitive pairs and Schwarz bounds on the device and keeps them for
-    every later call.
+    every later call.  ``prim_eps`` is the primitive-pair screening
+    threshold (``gx_set_prim_screen``; None keeps the library default, 0
+    keeps every primitive pair).
     """
 
-    def __init__(self, basis=None, nuclei=None, device=None):
+    def __init__(self, basis=None, nuclei=None, device=None, prim_eps=None):
         if not torch.cuda.is_available():
             raise _lib.GxError('gaussium_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
         self.device = torch.device('cuda', torch.cuda.current_device() if device is None else device)
@@ -37,6 +39,8 @@ class Engine:
         self.ctx = h
         self.shells = None
         self.nuclei = None
+        if prim_eps is not None:
+            self.set_prim_screen(prim_eps)
         if basis is not None:
             self.set_basis(basis, nuclei)
         if nuclei is not None:
@@ -57,6 +61,15 @@ class Engine:
     def _bind_stream(self):
         s = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self.lib.gx_set_stream(self.ctx, ctypes.c_void_p(s)), 'gx_set_stream')
+
+    def set_prim_screen(self, eps):
+        """Primitive-pair screening threshold for the next set_basis / set_systems."""
+        _lib.check(self.lib.gx_set_prim_screen(self.ctx, ctypes.c_double(eps)), 'gx_set_prim_screen')
+
+    @property
+    def n_primitive_pairs(self):
+        """Primitive pairs kept for the current basis."""
+        return int(self.lib.gx_nprimpair(self.ctx))
 
     def set_basis(self, basis, nuclei=None):
         sh = basis if isinstance(basis, ShellSet) else pack_shells(basis, nuclei)

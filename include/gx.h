@@ -50,6 +50,9 @@ extern "C" {
 #define GX_JK_MIXED 1        /* quartets with Schwarz bound < fp32_bound in FP32 */
 #define GX_JK_TIMING 2       /* record HIP events around each class-combination launch */
 
+/* Default primitive-pair screening threshold (gx_set_prim_screen). */
+#define GX_PRIM_EPS_DEFAULT 1e-18
+
 typedef struct gx_ctx gx_ctx;
 
 typedef struct gx_stats {
@@ -110,6 +113,21 @@ int gx_set_basis_batch(gx_ctx* ctx, int nsys, const int* sys_shell, const int* s
                        const int* l, const int* nprim, const int* prim_off, const int* bf_off, const double* ctr,
                        const double* ctr1, const int* atom, int nprim_total, const double* exps,
                        const double* coefs, int nbf, const double* bf_scale);
+
+/* Primitive-pair screening threshold for the following gx_set_basis /
+ * gx_set_basis_batch.  A primitive pair (a, b) of a shell pair is dropped
+ * when S_ab * S_max < eps, with S_ab = sqrt(max over components |(ab|ab)|)
+ * of the scaled primitive product and S_max the largest S over the basis:
+ * by the Schwarz inequality every primitive quartet left out is below eps,
+ * so a contracted integral moves by less than eps times its number of
+ * primitive quartets.  Each shell pair keeps at least its largest primitive
+ * pair.  eps = 0 keeps every primitive pair.  Default GX_PRIM_EPS_DEFAULT.
+ * No counterpart in the reference, which sums every primitive quartet
+ * (integrals/twoelectronrepulsion/obara_saika_scheme.py:41-80). */
+int gx_set_prim_screen(gx_ctx* ctx, double eps);
+
+/* Primitive pairs kept for the current basis (summed over shell pairs). */
+int gx_nprimpair(gx_ctx* ctx);
 
 /* Nuclei (host arrays): xyz[3*natom] in bohr, charge[natom]. */
 int gx_set_nuclei(gx_ctx* ctx, int natom, const double* xyz, const double* charge);
