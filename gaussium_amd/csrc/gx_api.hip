@@ -145,7 +145,7 @@ struct gx_ctx {
   struct WorkSet {
     std::vector<int*> d_items, d_rng;
     std::vector<int> n_items;
-    std::vector<double> ipw;  // items per resident workgroup, per combo (autotuned)
+    std::vector<double> ipw;  // items per resident workgroup, per combo and mode (FP64, mixed; autotuned)
     double cutoff = -1.0, fp32 = -1.0;
     int mixed = -1;
     void release() {
@@ -703,9 +703,12 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
     }
   }
   // item size: about `ipw` items per resident workgroup (four per CU
-  // assumed), each spread over the waves of its workgroup
+  // assumed), each spread over the waves of its workgroup, and not below the
+  // per-item overhead scaled by GX_ITEMS_PER_WG / ipw (so that the autotuner's
+  // larger ipw candidates reach the small items that light combinations with
+  // few bras need to fill the device)
   const double slots = (double)c->num_cu * 4.0;
-  const double target = std::max(total / (slots * waves * ipw), kItemOverhead);
+  const double target = std::max(total / (slots * waves * ipw), kItemOverhead * GX_ITEMS_PER_WG / ipw);
   std::vector<JKItem> items;
   std::vector<double> icost;
   for (int i = 0; i < nb; ++i) {
@@ -765,10 +768,11 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = f
   ws.d_items.assign(combo_count(), nullptr);
   ws.d_rng.assign(combo_count(), nullptr);
   ws.n_items.assign(combo_count(), 0);
-  if ((int)ws.ipw.size() != combo_count()) ws.ipw.assign(combo_count(), GX_ITEMS_PER_WG);
+  if ((int)ws.ipw.size() != 2 * combo_count()) ws.ipw.assign(2 * combo_count(), GX_ITEMS_PER_WG);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo) GX_TRY(build_combo(c, ws, X, Y, combo, cutoff, fp32, mixed, grad, ws.ipw[combo]));
+    for (int Y = 0; Y <= X; ++Y, ++combo)
+      GX_TRY(build_combo(c, ws, X, Y, combo, cutoff, fp32, mixed, grad, ws.ipw[combo + mixed * combo_count()]));
   ws.cutoff = cutoff;
   ws.fp32 = fp32;
   ws.mixed = mixed;
@@ -1390,7 +1394,8 @@ int gx_jk_autotune(gx_ctx* c, const double* P, double cutoff, double fp32_bound,
         rc = fail(GX_ERR_ARG, "gx_jk_autotune: output too small");
         break;
       }
-      const double before = c->jk.ipw[combo];
+      double& ipw = c->jk.ipw[combo + mixed * combo_count()];
+      const double before = ipw;
       double best = before, best_ms = 1e30, before_ms = 1e30;
       for (int k = 0; k < ncand && rc == GX_OK; ++k) {
         rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, cand[k]);
@@ -1410,7 +1415,7 @@ int gx_jk_autotune(gx_ctx* c, const double* P, double cutoff, double fp32_bound,
         if (cand[k] == before) before_ms = ms_min;
       }
       if (rc != GX_OK) break;
-      c->jk.ipw[combo] = best;
+      ipw = best;
       rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, best);
       double* r = out + 6 * row++;
       r[0] = class_la(X);

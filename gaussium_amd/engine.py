@@ -209,8 +209,8 @@ class Engine:
                    'gx_jk')
         return J, K
 
-    def autotune(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False, candidates=(1.0, 2.0, 4.0, 8.0, 16.0),
-                 reps=3):
+    def autotune(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False,
+                 candidates=(1.0, 2.0, 4.0, 8.0, 16.0, 32.0, 64.0), reps=3):
         """Pick the work-item size of each class combination by timing (see
         gx_jk_autotune); returns {(la, lb, lc, ld): (items per workgroup, ms)}."""
         P = self._dev(P)
