@@ -164,6 +164,8 @@ def bench_gradient(args):
     nuc, basis, ne, info = workloads.cluster_66bf()
     eng = Engine(basis, nuc)
     P, W, hf = _scf_density(eng, nuc, basis, ne, args.scf_iter)
+    tuned = eng.grad_autotune(P, W, cutoff=CUTOFF)  # untimed set-up
+    eng.autotune(P, cutoff=CUTOFF)
     ms = _median_ms(lambda: eng.grad(P, W, cutoff=CUTOFF), args.steps, args.warmup)
     ms_jk = _median_ms(lambda: eng.jk(P, cutoff=CUTOFF), args.steps, args.warmup)
     g = eng.grad(P, W, cutoff=CUTOFF)
@@ -173,7 +175,8 @@ def bench_gradient(args):
             'config': {'workload': info['workload'], 'nbf': info['nbf'], 'atoms': info['n_atoms'],
                        'schwarz_cutoff': CUTOFF},
             'jk_ms_same_density': ms_jk, 'max_abs_gradient': float(g.abs().max()),
-            'translation_residual': float(g.sum(0).abs().max())}
+            'translation_residual': float(g.sum(0).abs().max()),
+            'autotuned_grad_items_per_wg': {'%d%d%d%d' % k: v[0] for k, v in tuned.items()}}
 
 
 def bench_batch(args):
@@ -196,6 +199,7 @@ def bench_batch(args):
     P = eng.pack(mats)
     print('batch: %d systems set up' % len(systems), file=sys.stderr, flush=True)
     tuned = eng.autotune(P, cutoff=CUTOFF, reps=2)
+    tuned_grad = eng.grad_autotune(P, P, cutoff=CUTOFF, reps=1)
     print('batch: autotuned', file=sys.stderr, flush=True)
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
@@ -220,7 +224,8 @@ def bench_batch(args):
                        'nbf_max': max(sizes), 'nbf_total': sum(sizes), 'schwarz_cutoff': CUTOFF},
             'geometries_per_s': len(systems) / (ms * 1e-3), 'jk_ms_eager': ms_jk, 'grad_ms_eager': ms_grad,
             'bf_quartets_per_s_jk': st['bf_quartets'] / (ms_jk * 1e-3),
-            'autotuned_items_per_wg': {'%d%d%d%d' % k: v[0] for k, v in tuned.items()}}
+            'autotuned_items_per_wg': {'%d%d%d%d' % k: v[0] for k, v in tuned.items()},
+            'autotuned_grad_items_per_wg': {'%d%d%d%d' % k: v[0] for k, v in tuned_grad.items()}}
 
 
 def bench_d(args):
@@ -233,6 +238,9 @@ def bench_d(args):
     eng = Engine(basis, nuc)
     P, W, hf = _scf_density(eng, nuc, basis, ne, args.scf_iter)
     H = hf.core_hamiltonian
+    eng.autotune(P, cutoff=CUTOFF)  # untimed set-up
+    eng.autotune(P, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
+    eng.grad_autotune(P, W, cutoff=CUTOFF)
     ms64 = _median_ms(lambda: eng.jk(P, cutoff=CUTOFF), args.steps, args.warmup)
     msmx = _median_ms(lambda: eng.jk(P, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True), args.steps, args.warmup)
     msg = _median_ms(lambda: eng.grad(P, W, cutoff=CUTOFF), args.steps, args.warmup)

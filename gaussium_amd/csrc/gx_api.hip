@@ -851,6 +851,29 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed) {
                    &c->wg_per_cu[2 * combo + mixed]);
 }
 
+// Two-electron gradient launch of one class combination (items of c->gr).
+hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad) {
+  if (c->gr.n_items[combo] == 0) return hipSuccess;
+  if (c->wg_per_cu_grad.empty()) c->wg_per_cu_grad.assign(combo_count(), 0);
+  gx::GradArgs a;
+  a.sh = c->shells();
+  a.pr = c->pairs();
+  a.sy = c->systems();
+  a.bra0 = c->cls_start[X];
+  a.seg_lo = c->d_seg_lo;
+  a.cseg = c->d_cseg + (size_t)Y * (c->nsys + 1);
+  a.items = reinterpret_cast<const int4*>(c->gr.d_items[combo]);
+  a.nitem = c->gr.n_items[combo];
+  a.rng = c->gr.d_rng[combo];
+  a.tab64 = c->d_tab64;
+  a.P = c->d_Ps;
+  a.atom = c->d_atom;
+  a.sys_atom = c->d_sys_atom;
+  a.grad = grad;
+  return launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, c->stream, a,
+                     &c->wg_per_cu_grad[combo], c->nmax);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1475,30 +1498,67 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
                      c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, 0.0, 0, true));
-  if (c->wg_per_cu_grad.empty()) c->wg_per_cu_grad.assign(combo_count(), 0);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo) {
-      if (c->gr.n_items[combo] == 0) continue;
-      gx::GradArgs a;
-      a.sh = c->shells();
-      a.pr = c->pairs();
-      a.sy = c->systems();
-      a.bra0 = c->cls_start[X];
-      a.seg_lo = c->d_seg_lo;
-      a.cseg = c->d_cseg + (size_t)Y * (c->nsys + 1);
-      a.items = reinterpret_cast<const int4*>(c->gr.d_items[combo]);
-      a.nitem = c->gr.n_items[combo];
-      a.rng = c->gr.d_rng[combo];
-      a.tab64 = c->d_tab64;
-      a.P = c->d_Ps;
-      a.atom = c->d_atom;
-      a.sys_atom = c->d_sys_atom;
-      a.grad = grad;
-      GX_HIP(launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, c->stream, a,
-                         &c->wg_per_cu_grad[combo], c->nmax));
-    }
+    for (int Y = 0; Y <= X; ++Y, ++combo) GX_HIP(launch_grad_combo(c, X, Y, combo, grad));
   return GX_OK;
+}
+
+int gx_grad_autotune(gx_ctx* c, const double* P, const double* W, double cutoff, const double* cand, int ncand,
+                     int reps, double* out, int cap) {
+  if (!c || !P || !W || !cand || !out || ncand <= 0 || reps <= 0)
+    return fail(GX_ERR_ARG, "gx_grad_autotune: bad argument");
+  for (int k = 0; k < ncand; ++k)
+    if (!(cand[k] > 0.0)) return fail(GX_ERR_ARG, "gx_grad_autotune: candidates must be positive");
+  double* dg = nullptr;
+  GX_TRY(dalloc(&dg, (size_t)3 * std::max(c->natom, 1)));
+  // one untimed gradient: scaled density, default items, sizes checked
+  int rc = gx_grad(c, P, W, dg, cutoff);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == GX_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(GX_ERR_HIP, "gx_grad_autotune: hipEventCreate");
+  int row = 0, combo = 0;
+  for (int X = 0; X < NCLS && rc == GX_OK; ++X)
+    for (int Y = 0; Y <= X && rc == GX_OK; ++Y, ++combo) {
+      if (c->gr.n_items[combo] == 0) continue;
+      if (row >= cap) {
+        rc = fail(GX_ERR_ARG, "gx_grad_autotune: output too small");
+        break;
+      }
+      double& ipw = c->gr.ipw[combo];
+      double best = ipw, best_ms = 1e30;
+      for (int k = 0; k < ncand && rc == GX_OK; ++k) {
+        rc = build_combo(c, c->gr, X, Y, combo, cutoff, 0.0, 0, true, cand[k]);
+        if (rc != GX_OK) break;
+        double ms_min = 1e30;
+        for (int r = 0; r < reps; ++r) {
+          float ms = 0.f;
+          if (hipEventRecord(e0, c->stream) != hipSuccess || launch_grad_combo(c, X, Y, combo, dg) != hipSuccess ||
+              hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(GX_ERR_HIP, "gx_grad_autotune: timed launch");
+            break;
+          }
+          ms_min = std::min(ms_min, (double)ms);
+        }
+        if (ms_min < best_ms) best_ms = ms_min, best = cand[k];
+      }
+      if (rc != GX_OK) break;
+      ipw = best;
+      rc = build_combo(c, c->gr, X, Y, combo, cutoff, 0.0, 0, true, best);
+      double* r = out + 6 * row++;
+      r[0] = class_la(X);
+      r[1] = class_lb(X);
+      r[2] = class_la(Y);
+      r[3] = class_lb(Y);
+      r[4] = best;
+      r[5] = best_ms;
+    }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->stream);
+  dfree(&dg);
+  return rc == GX_OK ? row : rc;
 }
 
 int gx_jk_stats(gx_ctx* c, double cutoff, double fp32_bound, int flags, gx_stats* out) {

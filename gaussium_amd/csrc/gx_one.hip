@@ -21,6 +21,11 @@ namespace gx {
 #define ONE_RMAX (2 * ONE_LMAX)  // Hermite order for V
 #define ONE_RDIM (2 * ONE_LMAX + 2)  // R_tuv index bound (the gradient raises L by one)
 #define ONE_TMAX (ONE_LMAX + ONE_EMAX)  // Hermite index bound of E
+// A primitive pair whose Gaussian-product factor |c_a c_b| exp(-ab/(a+b) R^2)
+// is below this is skipped: its contribution to any one-electron integral or
+// derivative is that factor times bounded polynomial and (2 pi / p) terms,
+// far below 1e-12.
+#define ONE_PRIM_EPS 1e-20
 
 // Boys F_0..F_n(T) from the master table in global memory (Taylor + downward).
 __device__ inline void boys_master(int n, double T, const double* __restrict__ master, double* F) {
@@ -191,12 +196,13 @@ __global__ __launch_bounds__(64) void k_one_electron(OneArgs a, double* __restri
       const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
       const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
       double P[3], r2 = 0.0;
+      for (int x = 0; x < 3; ++x) r2 += (A[x] - B[x]) * (A[x] - B[x]);
+      const double Kab = exp(-mu * r2) * ca * cb;
+      if (fabs(Kab) < ONE_PRIM_EPS) continue;
       for (int x = 0; x < 3; ++x) {
         P[x] = (a * A[x] + b * B[x]) / p;
-        r2 += (A[x] - B[x]) * (A[x] - B[x]);
         hermite_e(la, lb + 2, P[x] - A[x], P[x] - B[x], o2p, E[x]);
       }
-      const double Kab = exp(-mu * r2) * ca * cb;
       const double s00 = Kab * pow(M_PI / p, 1.5);
       // overlap and kinetic
       for (int ka = 0; ka < na; ++ka) {
@@ -355,12 +361,14 @@ __global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restric
       const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
       const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
       double Pc[3], r2 = 0.0;
+      for (int x = 0; x < 3; ++x) r2 += (A[x] - B[x]) * (A[x] - B[x]);
+      const double kab = exp(-mu * r2) * ca * cb;
+      if (fabs(kab) < ONE_PRIM_EPS) continue;
       for (int x = 0; x < 3; ++x) {
         Pc[x] = (a * A[x] + b * B[x]) / p;
-        r2 += (A[x] - B[x]) * (A[x] - B[x]);
         hermite_e_g(la + 1, lb + 2, Pc[x] - A[x], Pc[x] - B[x], o2p, E[x]);
       }
-      const double s00 = exp(-mu * r2) * ca * cb * pow(M_PI / p, 1.5);
+      const double s00 = kab * pow(M_PI / p, 1.5);
       auto S1 = [&](int x, int i, int j) { return E[x][i][j][0]; };
       // <a|T|b> for exponent triples ea, eb (the reference's del-operator expansion on b)
       auto Tval = [&](const int* ea, const int* eb) {
@@ -412,16 +420,18 @@ __global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restric
         const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
         const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
         double Pc[3], r2 = 0.0;
+        for (int x = 0; x < 3; ++x) r2 += (A[x] - B[x]) * (A[x] - B[x]);
+        const double kab = exp(-mu * r2) * ca * cb;
+        if (fabs(kab) < ONE_PRIM_EPS) continue;
         for (int x = 0; x < 3; ++x) {
           Pc[x] = (a * A[x] + b * B[x]) / p;
-          r2 += (A[x] - B[x]) * (A[x] - B[x]);
           hermite_e_g(la + 1, lb + 1, Pc[x] - A[x], Pc[x] - B[x], o2p, E[x]);
         }
         const double PC[3] = {Pc[0] - nxyz[3 * c], Pc[1] - nxyz[3 * c + 1], Pc[2] - nxyz[3 * c + 2]};
         const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
         boys_master(L, Tb, master, F);
         hermite_r(L, p, PC, F, R, Rs);
-        const double pref = -nZ[c] * 2.0 * M_PI / p * exp(-mu * r2) * ca * cb;
+        const double pref = -nZ[c] * 2.0 * M_PI / p * kab;
         auto Vval = [&](const int* ea, const int* eb) {
           double sum = 0.0;
           for (int tt = 0; tt <= ea[0] + eb[0]; ++tt)

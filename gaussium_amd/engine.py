@@ -224,6 +224,20 @@ class Engine:
             _lib.check(n, 'gx_jk_autotune')
         return {tuple(int(x) for x in r[:4]): (float(r[4]), float(r[5])) for r in out[:n]}
 
+    def grad_autotune(self, P, W, cutoff=0.0, candidates=(1.0, 2.0, 4.0, 8.0, 16.0, 32.0, 64.0), reps=2):
+        """Pick the gradient work-item size of each class combination by timing
+        (gx_grad_autotune); returns {(la, lb, lc, ld): (items per workgroup, ms)}."""
+        P, W = self._dev(P), self._dev(W)
+        self._matrix_shape(P, 'P')
+        cand = np.ascontiguousarray(candidates, dtype=np.float64)
+        out = np.zeros((64, 6))
+        self._bind_stream()
+        n = self.lib.gx_grad_autotune(self.ctx, _p(P), _p(W), float(cutoff), _p(cand), len(cand), int(reps),
+                                      _p(out), 64)
+        if n < 0:
+            _lib.check(n, 'gx_grad_autotune')
+        return {tuple(int(x) for x in r[:4]): (float(r[4]), float(r[5])) for r in out[:n]}
+
     def jk_launch_times(self):
         """{(la, lb, lc, ld): (total ms, launches)} over the ``jk(..., timing=True)``
         calls since the previous read."""

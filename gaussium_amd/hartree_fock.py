@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from .engine import Engine
-from .scf import LinearAlgebra, RestrictedSCF
+from .scf import LinearAlgebra, PopleNesbetBerthier, RestrictedSCF
 
 
 class TwoElectronRepulsionMatrixOS:
@@ -61,6 +61,28 @@ class FockMatrixRestricted:
                               mixed=self.mixed)
         self.builds += 1
         return self.core_hamiltonian + J - 0.5 * K
+
+
+class FockMatrixUnrestricted:
+    """F_a = H + J(P_a + P_b) - K(P_a), F_b = H + J(P_a + P_b) - K(P_b)
+    (hartreefock/fock_matrix.py:25-55): one direct J/K build per spin density
+    (J is linear, so J(P_a) + J(P_b) is the Coulomb matrix of the total)."""
+
+    def __init__(self, core_hamiltonian, engine, schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64'):
+        self.core_hamiltonian = core_hamiltonian
+        self.engine = engine
+        self.schwarz_cutoff = schwarz_cutoff
+        self.fp32_bound = fp32_bound
+        self.mixed = precision == 'mixed'
+        self.builds = 0
+
+    def create(self, density_matrix_alph, density_matrix_beta):
+        kw = dict(cutoff=self.schwarz_cutoff, fp32_bound=self.fp32_bound, mixed=self.mixed)
+        ja, ka = self.engine.jk(density_matrix_alph, **kw)
+        jb, kb = self.engine.jk(density_matrix_beta, **kw)
+        self.builds += 2
+        j = ja + jb
+        return self.core_hamiltonian + j - ka, self.core_hamiltonian + j - kb
 
 
 class RestrictedHF:
@@ -127,3 +149,39 @@ class RestrictedHF:
         c = self.scf_method.orbital_coefficients[:, :self.electrons // 2]
         W = 2.0 * (c * eps[:self.electrons // 2]) @ c.T
         return self.engine.grad(self.scf_method.density, W, cutoff=self.schwarz_cutoff)
+
+
+class UnrestrictedHF:
+    """Unrestricted Hartree-Fock (hartree_fock.py:87-122, Pople-Nesbet-Berthier
+    iterations of scf_procedure.py:45-89) on one GPU."""
+
+    def __init__(self, nuclei_array, basis_set_array, electrons, multiplicity, symmetry=None, processes=1,
+                 schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64', max_iter=None, engine=None,
+                 threshold=1e-12):
+        self.nuclei_array = nuclei_array
+        self.basis_set_array = basis_set_array
+        self.electrons = electrons
+        self.multiplicity = multiplicity
+        self.symmetry = symmetry
+        self.engine = engine if engine is not None else Engine(basis_set_array, nuclei_array)
+        if engine is not None and basis_set_array is not None:
+            self.engine.set_basis(basis_set_array, nuclei_array)
+            self.engine.set_nuclei(nuclei_array)
+        S, T, V = self.engine.one_electron()
+        self.orbital_overlap = S
+        self.core_hamiltonian = T + V
+        self.linear_algebra = LinearAlgebra(S)
+        self.fock_builder = FockMatrixUnrestricted(self.core_hamiltonian, self.engine, schwarz_cutoff,
+                                                   fp32_bound, precision)
+        self.scf_method = PopleNesbetBerthier(self.linear_algebra, electrons, multiplicity, self.fock_builder,
+                                              threshold=threshold, max_iter=max_iter)
+
+    def initial_guess(self):
+        return self.linear_algebra.diagonalize(self.core_hamiltonian)
+
+    def begin_scf(self):
+        e0, c0 = self.initial_guess()
+        return self.scf_method.begin_iterations(e0, c0)
+
+    def energies(self):
+        return self.begin_scf()[0], 0.0

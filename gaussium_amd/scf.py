@@ -29,12 +29,23 @@ def density_matrix_restricted(c, electrons):
     return 2.0 * (occ @ occ.T)
 
 
+def density_matrix_unrestricted(c, electrons):
+    """P = C_occ C_occ^T for one spin (matrixelements/density_matrix.py:12-17)."""
+    occ = c[:, :electrons]
+    return occ @ occ.T
+
+
 class TotalEnergy:
     def __init__(self, core_hamiltonian):
         self.core_hamiltonian = core_hamiltonian
 
     def restricted(self, density_matrix, hamiltonian):
         return 0.5 * float(torch.sum(density_matrix * (self.core_hamiltonian + hamiltonian)))
+
+    def unrestricted(self, density_alpha, density_beta, fock_alpha, fock_beta):
+        """1/2 sum (P_a + P_b) H + P_a F_a + P_b F_b (hartreefock/total_energy.py:37-63)."""
+        return 0.5 * float(torch.sum((density_alpha + density_beta) * self.core_hamiltonian
+                                     + density_alpha * fock_alpha + density_beta * fock_beta))
 
 
 class LinearAlgebra:
@@ -162,3 +173,48 @@ class RestrictedSCF:
         self.orbital_energies = orbital_energies
         self.orbital_coefficients = orbital_coefficients
         return total, orbital_energies, orbital_coefficients
+
+
+class PopleNesbetBerthier:
+    """Unrestricted SCF (hartreefock/scf_procedure.py:45-89): n_a = (n + m - 1) / 2,
+    n_b = (n - m + 1) / 2 electrons; the first iteration takes the alpha density
+    from every orbital of the guess and an empty beta density, as the
+    reference does; no DIIS; stop at |dE| < threshold (plus the optional
+    iteration cap)."""
+
+    def __init__(self, linear_algebra, electrons, multiplicity, hamiltonian_matrix_factory, threshold=1e-12,
+                 max_iter=None):
+        self.linear_algebra = linear_algebra
+        self.electrons = electrons
+        self.hamiltonian_matrix_factory = hamiltonian_matrix_factory
+        self.threshold = threshold
+        self.max_iter = max_iter
+        self.calculate = TotalEnergy(hamiltonian_matrix_factory.core_hamiltonian)
+        self.electrons_alph = (electrons + multiplicity - 1) // 2
+        self.electrons_beta = (electrons - multiplicity + 1) // 2
+        self.iterations = 0
+        self.converged = False
+
+    def begin_iterations(self, orbital_energies, orbital_coefficients):
+        ca = cb = orbital_coefficients
+        ea = eb = orbital_energies
+        total = previous = 0.0
+        while True:
+            if self.iterations == 0:
+                pa = density_matrix_unrestricted(ca, ca.shape[0])
+                pb = density_matrix_unrestricted(cb, 0)
+            else:
+                pa = density_matrix_unrestricted(ca, self.electrons_alph)
+                pb = density_matrix_unrestricted(cb, self.electrons_beta)
+            fa, fb = self.hamiltonian_matrix_factory.create(pa, pb)
+            total = self.calculate.unrestricted(pa, pb, fa, fb)
+            delta = previous - total
+            previous = total
+            self.iterations += 1
+            self.converged = abs(delta) < self.threshold
+            if self.converged or (self.max_iter is not None and self.iterations >= self.max_iter):
+                break
+            ea, ca = self.linear_algebra.diagonalize(fa)
+            eb, cb = self.linear_algebra.diagonalize(fb)
+        self.density_alpha, self.density_beta = pa, pb
+        return total, ea, eb, ca, cb

@@ -178,6 +178,14 @@ int gx_jk_autotune(gx_ctx* ctx, const double* P_dev, double cutoff, double fp32_
  * by its optimiser (geometryoptimization/nelder_mead.py:126-144). */
 int gx_grad(gx_ctx* ctx, const double* P_dev, const double* W_dev, double* grad_dev, double cutoff);
 
+/* Autotuner of the gradient's work-item size per class combination, as
+ * gx_jk_autotune (rows of la, lb, lc, ld, chosen value, milliseconds); the
+ * choice is kept for later gx_grad calls with this cutoff (until the next
+ * gx_set_basis).  Returns the number of rows or an error code.  No
+ * counterpart in the reference. */
+int gx_grad_autotune(gx_ctx* ctx, const double* P_dev, const double* W_dev, double cutoff,
+                     const double* candidates, int ncand, int reps, double* out, int cap);
+
 /* Shell-quartet and flop counts that gx_jk would evaluate (host-side count). */
 int gx_jk_stats(gx_ctx* ctx, double cutoff, double fp32_bound, int flags, gx_stats* out);
 

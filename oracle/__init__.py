@@ -24,4 +24,7 @@ Pinning (what checks each part, besides restating the formula):
   jk / scf    -- reference RHF energies (HeH+/STO-3G, HeH+/6-311+G**,
                  C2H4/3-21G, mock HeH+ -4.227529), reference total-energy
                  fixture, density fixture, E_nuc fixtures.
+  uhf         -- the reference's UHF energy of O2/STO-3G (-147.634028141).
+  mp2         -- the reference's MP2 energy of CO/STO-3G (-111.354512528);
+                 the MO transformation against an einsum contraction.
 """

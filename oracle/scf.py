@@ -150,3 +150,74 @@ def rhf(h, s, electrons, fock_builder, threshold=1e-12, max_iter=None, diis_cond
         f = diis.fock_matrix(f, p)
         eps, c = la.diagonalize(f)
     return e, eps, c, it
+
+
+def density_matrix_unrestricted(c, electrons):
+    """density_matrix.py:12-17: P = sum_{i < n} c_i c_i^T (one spin)."""
+    occ = c[:, :electrons]
+    return occ @ occ.T
+
+
+def total_energy_unrestricted(h, pa, pb, fa, fb):
+    """total_energy.py:37-63: 1/2 sum (P_a + P_b) H + P_a F_a + P_b F_b."""
+    return 0.5 * float(np.sum((pa + pb) * h + pa * fa + pb * fb))
+
+
+def uhf(h, s, electrons, multiplicity, fock_builder, threshold=1e-12, max_iter=None):
+    """Pople-Nesbet-Berthier SCF (scf_procedure.py:45-89, hartree_fock.py:87-122).
+
+    ``fock_builder(P_a, P_b) -> (F_a, F_b)`` with F_a = H + J(P_a + P_b) - K(P_a)
+    (fock_matrix.py:25-55).  The first iteration takes the alpha density from
+    every orbital of the core-H guess and an empty beta density, as the
+    reference does; no DIIS.  Returns (E_elec, iterations)."""
+    la = LinearAlgebra(s)
+    _, c = la.diagonalize(h)
+    ca = cb = c
+    na, nb = (electrons + multiplicity - 1) // 2, (electrons - multiplicity + 1) // 2
+    prev = 0.0
+    it = 0
+    while True:
+        if it == 0:
+            pa, pb = density_matrix_unrestricted(ca, ca.shape[0]), density_matrix_unrestricted(cb, 0)
+        else:
+            pa, pb = density_matrix_unrestricted(ca, na), density_matrix_unrestricted(cb, nb)
+        fa, fb = fock_builder(pa, pb)
+        e = total_energy_unrestricted(h, pa, pb, fa, fb)
+        de = prev - e
+        prev = e
+        it += 1
+        if abs(de) < threshold or (max_iter is not None and it >= max_iter):
+            break
+        _, ca = la.diagonalize(fa)
+        _, cb = la.diagonalize(fb)
+    return e, it
+
+
+def molecular_orbitals(eri, c):
+    """transformations.py:5-26: (pq|rs) = sum C_mp C_nq C_lr C_ss (mn|ls), one
+    N x N slice at a time as the reference does (on a copy)."""
+    out = np.array(eri, copy=True)
+    n = c.shape[0]
+    for r in range(n):
+        for s_ in range(n):
+            out[r, s_, :, :] = c.T @ out[r, s_, :, :] @ c
+    for t in range(n):
+        for u in range(n):
+            out[:, :, t, u] = c.T @ out[:, :, t, u] @ c
+    return out
+
+
+def mp2_correlation(eri_mo, eps, electrons):
+    """moeller_plesset.py:15-23: sum over occupied i, j and virtual a, b of
+    [2 (ia|jb)^2 - (ia|jb)(ib|ja)] / (e_i + e_j - e_a - e_b), as explicit loops."""
+    nocc = electrons // 2
+    n = len(eps)
+    corr = 0.0
+    for i in range(nocc):
+        for j in range(nocc):
+            for a in range(nocc, n):
+                for b in range(nocc, n):
+                    d = eps[i] + eps[j] - eps[a] - eps[b]
+                    corr += 2.0 * eri_mo[i, a, j, b] ** 2 / d
+                    corr -= eri_mo[i, a, j, b] * eri_mo[i, b, j, a] / d
+    return corr

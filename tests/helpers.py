@@ -86,3 +86,55 @@ def canonical_sample(n, count, rng):
             a, b, c, d = c, d, a, b
         out.add((a, b, c, d))
     return np.array(sorted(out), dtype=np.int32)
+
+
+def oracle_uhf(nuclei, basis, electrons, multiplicity, exact_boys=True):
+    """Oracle UHF total energy from the C-oracle dense tensor and the oracle S/T/V."""
+    from oracle import boys as oboys
+    from oracle import eri_c
+    from oracle.basis import Nuclei, nuclear_repulsion
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from oracle.scf import jk_from_dense, uhf
+    if exact_boys:
+        with eri_c.exact_boys():
+            eri = eri_c.eri_dense(basis)
+    else:
+        eri = eri_c.eri_dense(basis)
+    onuc = [Nuclei(n.element, n.charge, n.mass, n.coordinates) for n in nuclei]
+    S = overlap_matrix(basis)
+    with oboys.exact(exact_boys):
+        H = kinetic_matrix(basis) + nuclear_matrix(basis, onuc)
+
+    def fb(Pa, Pb):
+        Ja, Ka = jk_from_dense(eri, Pa)
+        Jb, Kb = jk_from_dense(eri, Pb)
+        return H + Ja + Jb - Ka, H + Ja + Jb - Kb
+
+    e, it = uhf(H, S, electrons, multiplicity, fb)
+    return e + nuclear_repulsion(onuc), it
+
+
+def oracle_mp2(nuclei, basis, electrons, exact_boys=True):
+    """Oracle RHF + MP2 total energy (moeller_plesset.py:7-26)."""
+    from oracle import boys as oboys
+    from oracle import eri_c
+    from oracle.basis import Nuclei, nuclear_repulsion
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from oracle.scf import jk_from_dense, molecular_orbitals, mp2_correlation, rhf
+    if exact_boys:
+        with eri_c.exact_boys():
+            eri = eri_c.eri_dense(basis)
+    else:
+        eri = eri_c.eri_dense(basis)
+    onuc = [Nuclei(n.element, n.charge, n.mass, n.coordinates) for n in nuclei]
+    S = overlap_matrix(basis)
+    with oboys.exact(exact_boys):
+        H = kinetic_matrix(basis) + nuclear_matrix(basis, onuc)
+
+    def fb(P):
+        J, K = jk_from_dense(eri, P)
+        return H + J - 0.5 * K
+
+    e, eps, c, it = rhf(H, S, electrons, fb, diis_cond_limit=1e12)
+    corr = mp2_correlation(molecular_orbitals(eri, c), eps, electrons)
+    return e + nuclear_repulsion(onuc), corr

@@ -328,3 +328,40 @@ def test_oracle_rhf_published_energies(system, published):
     np.testing.assert_approx_equal(e2, published, 6)
     # exact vs reference Boys series and the DIIS conditioning rule move E by < 1e-9
     assert abs(e - e2) < 1e-9
+
+
+def test_oracle_uhf_published_energy():
+    """tests/testslow/start/test_start.py:33-37: UHF O2/STO-3G -147.634028141."""
+    from tests.helpers import oracle_uhf
+    from tests.helpers import golden
+    from oracle.basis import read_basis_set_file, read_mol_file
+    nuc, ne, mult = read_mol_file(golden('O2.mol'))
+    assert (ne, mult) == (16, 3)
+    basis = read_basis_set_file(golden('STO-3G.gbs'), nuc)
+    e, it = oracle_uhf(nuc, basis, ne, mult, exact_boys=False)
+    np.testing.assert_approx_equal(e, -147.634028141, 6)
+
+
+def test_oracle_mp2_published_energy():
+    """tests/testslow/start/test_start.py:45-49: MP2 CO/STO-3G -111.354512528."""
+    from tests.helpers import oracle_mp2
+    from tests.helpers import golden
+    from oracle.basis import read_basis_set_file, read_mol_file
+    nuc, ne, mult = read_mol_file(golden('CO.mol'))
+    basis = read_basis_set_file(golden('STO-3G.gbs'), nuc)
+    e_hf, corr = oracle_mp2(nuc, basis, ne, exact_boys=False)
+    np.testing.assert_approx_equal(e_hf + corr, -111.354512528, 6)
+    assert -0.2 < corr < -0.01
+
+
+def test_oracle_mo_transform_matches_einsum():
+    from oracle.scf import molecular_orbitals
+    rng = np.random.default_rng(3)
+    n = 5
+    a = rng.standard_normal((n, n, n, n))
+    eri = a + a.transpose(1, 0, 2, 3)
+    eri = eri + eri.transpose(0, 1, 3, 2)
+    eri = eri + eri.transpose(2, 3, 0, 1)
+    c = rng.standard_normal((n, n))
+    ref = np.einsum('mp,nq,lr,st,mnls->pqrt', c, c, c, c, eri)
+    assert np.abs(molecular_orbitals(eri, c) - ref).max() < 1e-10
