@@ -451,8 +451,9 @@ size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed) {
   GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   const int na = ncart(la), nb = ncart(lb);
-  size_t b = mixed ? (GX_MIXED_LDS32 ? (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float) : 0)
-                   : (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double);
+  const bool t64 = !mixed || GX_MIXED_LDS64, t32 = mixed && GX_MIXED_LDS32;
+  const size_t b = (t64 ? (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double) : 0) +
+                   (t32 ? (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float) : 0);
   return b + 2 * (size_t)(na + nb) * N * sizeof(double);
 }
 
@@ -849,6 +850,33 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed) {
   return launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
                    jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->nmax, mixed), c->stream, a,
                    &c->wg_per_cu[2 * combo + mixed]);
+}
+
+// One-electron gradient of the shell pairs of class (la, lb): one thread per
+// (pair, primitive part) and system part (grid y: overlap/kinetic, then each
+// nucleus of the pair's system).  Small jobs deal the primitives of shell A
+// to four threads.
+hipError_t launch_one_grad(gx_ctx* c, int la, int lb, const gx::OneArgs& oa, int p0, int np, int max_atoms,
+                           const double* P, const double* W, double* grad) {
+  const bool split = (long)np * (1 + max_atoms) < 200000;
+#define GX_ONE_GRAD(A, B, KS)                                                                          \
+  {                                                                                                    \
+    hipLaunchKernelGGL((gx::k_one_grad_t<A, B, KS, false>), dim3((np * KS + 63) / 64, 1), dim3(64), 0, \
+                       c->stream, oa, p0, np, c->d_atom, P, W, grad);                                 \
+    if (max_atoms > 0)                                                                                 \
+      hipLaunchKernelGGL((gx::k_one_grad_t<A, B, KS, true>), dim3((np * KS + 63) / 64, max_atoms),     \
+                         dim3(64), 0, c->stream, oa, p0, np, c->d_atom, P, W, grad);                  \
+  }
+#define GX_CASE(A, B)                 \
+  if (la == A && lb == B) {           \
+    if (split) GX_ONE_GRAD(A, B, 4)   \
+    else GX_ONE_GRAD(A, B, 1)         \
+    return hipGetLastError();         \
+  }
+  GX_FOR_EACH_PAIRCLASS(GX_CASE)
+#undef GX_CASE
+#undef GX_ONE_GRAD
+  return hipErrorInvalidValue;
 }
 
 // Two-electron gradient launch of one class combination (items of c->gr).
@@ -1492,8 +1520,10 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
   if (max_atoms > GX_GRAD_MAXATOM)
     return fail(GX_ERR_ARG, "gx_grad: a system has " + std::to_string(max_atoms) + " nuclei (at most " +
                                 std::to_string(GX_GRAD_MAXATOM) + ")");
-  hipLaunchKernelGGL(gx::k_one_grad, dim3((c->npair + 63) / 64, 1 + max_atoms), dim3(64), 0, c->stream, oa,
-                     c->d_atom, P, W, grad);
+  for (int k = 0; k < NCLS; ++k) {
+    const int p0 = c->cls_start[k], np = c->cls_start[k + 1] - p0;
+    if (np > 0) GX_HIP(launch_one_grad(c, class_la(k), class_lb(k), oa, p0, np, max_atoms, P, W, grad));
+  }
   hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_atom_sys,
                      c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
   GX_HIP(hipGetLastError());

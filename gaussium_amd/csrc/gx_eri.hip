@@ -156,6 +156,9 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 #ifndef GX_MIXED_LDS32
 #define GX_MIXED_LDS32 1  // mixed builds: FP32 Taylor table in LDS (else read through L1)
 #endif
+#ifndef GX_MIXED_LDS64
+#define GX_MIXED_LDS64 0  // mixed builds: FP64 Taylor table in LDS as well (else read through L1)
+#endif
 #ifndef GX_DYN_CHUNK
 #define GX_DYN_CHUNK 1
 #endif
@@ -411,17 +414,16 @@ __global__ __launch_bounds__(JK_TEAM) GX_JK_ATTR void k_jk(JKArgs a, const PrimP
   const int tid = threadIdx.x;
   // dynamic LDS: Boys FP64 (pure FP64 build) | P rows | K rows (rows of the
   // item's system)
-  const double* __restrict__ tab64 = !MIXED ? smem : a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
-#if GX_MIXED_LDS32
-  const float* __restrict__ tab32 = MIXED ? reinterpret_cast<const float*>(smem) : a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
-  double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : BOYS_NT * BOYS_ROW32 / 2);
-  if (MIXED) boys_stage32(C::L, a.master, reinterpret_cast<float*>(smem));
-#else
-  const float* __restrict__ tab32 = a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
-  double* prow = smem + (!MIXED ? BOYS_NT * BOYS_ROW64 : 0);
-#endif
-
-  if (!MIXED) boys_stage64(C::L, a.master, smem);
+  constexpr bool T64 = !MIXED || GX_MIXED_LDS64;  // FP64 table staged in LDS
+  constexpr bool T32 = MIXED && GX_MIXED_LDS32;   // FP32 table staged in LDS
+  constexpr int off32 = T64 ? BOYS_NT * BOYS_ROW64 : 0;              // in doubles
+  constexpr int offrow = off32 + (T32 ? BOYS_NT * BOYS_ROW32 / 2 : 0);
+  const double* __restrict__ tab64 = T64 ? smem : a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
+  const float* __restrict__ tab32 =
+      T32 ? reinterpret_cast<const float*>(smem + off32) : a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
+  double* prow = smem + offrow;
+  if (T64) boys_stage64(C::L, a.master, smem);
+  if (T32) boys_stage32(C::L, a.master, reinterpret_cast<float*>(smem + off32));
 
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = tid / GX_WAVE;

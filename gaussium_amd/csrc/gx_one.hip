@@ -26,6 +26,7 @@ namespace gx {
 // derivative is that factor times bounded polynomial and (2 pi / p) terms,
 // far below 1e-12.
 #define ONE_PRIM_EPS 1e-20
+#define ONE_GRAD_NAT 128  // nuclei per block summed in LDS by the gradient kernels
 
 // Boys F_0..F_n(T) from the master table in global memory (Taylor + downward).
 __device__ inline void boys_master(int n, double T, const double* __restrict__ master, double* F) {
@@ -275,209 +276,321 @@ __global__ __launch_bounds__(64) void k_one_electron(OneArgs a, double* __restri
 namespace gx {
 
 // ---------------------------------------------------------------------------
-// One-electron part of the RHF nuclear gradient (the reference has no
-// gradient; its optimiser differentiates energies numerically,
-// geometryoptimization/nelder_mead.py:126-144):
+// One-electron gradient, k_one_grad_t<LA, LB, KSPLIT, VPART>, for the shell
+// pairs of class (LA, LB) (positions [p0, p0 + np) of the pair list), with the
+// angular momenta as template parameters so that the Hermite tables live in
+// registers:
 //   g_X += sum_ab P_ab d(T + V)_ab / dX  -  W_ab dS_ab / dX
-// with d/dA_i <a|O|b> = 2 alpha <a+1i|O|b> - a_i <a-1i|O|b> (likewise B);
-// S and T depend on A - B only (d/dB = -d/dA); the nuclear-attraction term of
-// nucleus C moves with C:  d/dC = -(d/dA + d/dB).
-// One thread per shell pair (i >= j); atomics into grad[3 * atom + axis].
-#define ONE_GRAD_I (ONE_LMAX + 1)      // bra raised by 1
-#define ONE_GRAD_J (ONE_LMAX + 2)      // ket raised by 2 (kinetic) or 1
-#define ONE_GRAD_T (ONE_GRAD_I + ONE_GRAD_J)
+// with d/dA_i <a|O|b> = 2 alpha <a+1i|O|b> - a_i <a-1i|O|b> (likewise B); S
+// and T depend on A - B only (d/dB = -d/dA); the nuclear-attraction term of
+// nucleus C moves with C: d/dC = -(d/dA + d/dB).  The kinetic integrals use
+// the reference's del-operator expansion on b (factory/primitive_basis_factory.py:89-122).
+// One thread per (pair, primitive part): VPART = false computes the overlap
+// and kinetic terms, VPART = true the nuclear attraction of the system's
+// blockIdx.y-th nucleus (two kernels, each with its own register budget);
+// with KSPLIT > 1 the primitives of shell A are dealt to KSPLIT threads
+// (more threads for small systems).  Atomics into grad[3 * atom + axis].
 
-__device__ inline void hermite_e_g(int imax, int jmax, double PA, double PB, double o2p,
-                                   double (*E)[ONE_GRAD_J + 1][ONE_GRAD_T + 1]) {
-  for (int i = 0; i <= imax; ++i)
-    for (int j = 0; j <= jmax; ++j)
-      for (int t = 0; t <= i + j; ++t) E[i][j][t] = 0.0;
-  E[0][0][0] = 1.0;
-  for (int i = 0; i <= imax; ++i) {
-    if (i > 0) {
-      for (int t = 0; t <= i; ++t) {
-        double v = (t <= i - 1) ? PA * E[i - 1][0][t] : 0.0;
-        if (t > 0) v += o2p * E[i - 1][0][t - 1];
-        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
-        E[i][0][t] = v;
-      }
-    }
-    for (int j = 1; j <= jmax; ++j) {
-      for (int t = 0; t <= i + j; ++t) {
-        double v = (t <= i + j - 1) ? PB * E[i][j - 1][t] : 0.0;
-        if (t > 0) v += o2p * E[i][j - 1][t - 1];
-        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+// Cartesian exponent of component k of a shell of momentum l along axis x,
+// in the reference's order s | x, y, z | xx, yy, zz, xy, xz, yz.
+__host__ __device__ constexpr int cart_e(int l, int k, int x) {
+  return l == 0 ? 0
+         : l == 1 ? (k == x ? 1 : 0)
+         : k < 3 ? (k == x ? 2 : 0)
+         : k == 3 ? (x < 2 ? 1 : 0)
+         : k == 4 ? (x != 1 ? 1 : 0)
+         : (x > 0 ? 1 : 0);
+}
+
+// 1D Hermite coefficients E[i][j][t], i <= I, j <= J (no exp factor).
+template <int I, int J>
+__device__ __forceinline__ void herm_e_t(double PA, double PB, double o2p, double (&E)[I + 1][J + 1][I + J + 1]) {
+#pragma unroll
+  for (int i = 0; i <= I; ++i)
+#pragma unroll
+    for (int j = 0; j <= J; ++j)
+#pragma unroll
+      for (int t = 0; t <= I + J; ++t) {
+        double v = 0.0;
+        if (i == 0 && j == 0) {
+          v = (t == 0) ? 1.0 : 0.0;
+        } else if (j == 0) {
+          if (t <= i - 1) v = PA * E[i - 1][0][t];
+          if (t > 0 && t - 1 <= i - 1) v += o2p * E[i - 1][0][t - 1];
+          if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
+        } else {
+          if (t <= i + j - 1) v = PB * E[i][j - 1][t];
+          if (t > 0 && t - 1 <= i + j - 1) v += o2p * E[i][j - 1][t - 1];
+          if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+        }
         E[i][j][t] = v;
       }
+}
+
+// Hermite Coulomb integrals R_{tuv} = R^{(0)}_{tuv}, t + u + v <= L.
+template <int L>
+__device__ __forceinline__ void herm_r_t(double p, const double (&PC)[3], const double (&F)[L + 1],
+                                         double (&R)[L + 1][L + 1][L + 1]) {
+  double Rn[L + 1][L + 1][L + 1][L + 1];
+  double m2p = 1.0;
+#pragma unroll
+  for (int n = 0; n <= L; ++n) {
+    Rn[n][0][0][0] = m2p * F[n];
+    m2p *= -2.0 * p;
+  }
+#pragma unroll
+  for (int n = L - 1; n >= 0; --n)
+#pragma unroll
+    for (int t = 0; t <= L - n; ++t)
+#pragma unroll
+      for (int u = 0; u <= L - n - t; ++u)
+#pragma unroll
+        for (int v = 0; v <= L - n - t - u; ++v) {
+          if (t + u + v == 0) continue;
+          double r;
+          if (t > 0) {
+            r = PC[0] * Rn[n + 1][t - 1][u][v];
+            if (t > 1) r += (t - 1) * Rn[n + 1][t - 2][u][v];
+          } else if (u > 0) {
+            r = PC[1] * Rn[n + 1][t][u - 1][v];
+            if (u > 1) r += (u - 1) * Rn[n + 1][t][u - 2][v];
+          } else {
+            r = PC[2] * Rn[n + 1][t][u][v - 1];
+            if (v > 1) r += (v - 1) * Rn[n + 1][t][u][v - 2];
+          }
+          Rn[n][t][u][v] = r;
+        }
+#pragma unroll
+  for (int t = 0; t <= L; ++t)
+#pragma unroll
+    for (int u = 0; u <= L - t; ++u)
+#pragma unroll
+      for (int v = 0; v <= L - t - u; ++v) R[t][u][v] = Rn[0][t][u][v];
+}
+
+// Boys F_0..F_L(T) from the master table (as boys_master, fixed L).
+template <int L>
+__device__ __forceinline__ void boys_master_t(double T, const double* __restrict__ master, double (&F)[L + 1]) {
+  if (T < BOYS_TMAX) {
+    const int k0 = (int)(T * BOYS_INV_DT + 0.5);
+    const double x = k0 * BOYS_DT - T;
+    const double* row = master + k0 * (BOYS_MCOL + 1);
+    const double invf[8] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+    double f = row[L + 7] * invf[7];
+#pragma unroll
+    for (int k = 6; k >= 0; --k) f = fma(f, x, row[L + k] * invf[k]);
+    double e = 1.0 / 5040.0;
+    e = fma(e, x, 1.0 / 720.0);
+    e = fma(e, x, 1.0 / 120.0);
+    e = fma(e, x, 1.0 / 24.0);
+    e = fma(e, x, 1.0 / 6.0);
+    e = fma(e, x, 0.5);
+    e = fma(e, x, 1.0);
+    e = fma(e, x, 1.0);
+    e *= row[BOYS_MCOL];
+    F[L] = f;
+#pragma unroll
+    for (int m = L; m >= 1; --m) {
+      f = fma(2.0 * T, f, e) * (1.0 / (2 * m - 1));
+      F[m - 1] = f;
+    }
+  } else {
+    double f = 0.88622692545275801365 * sqrt(1.0 / T);
+    F[0] = f;
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      f *= (m + 0.5) / T;
+      F[m + 1] = f;
     }
   }
 }
 
-__global__ __launch_bounds__(64) void k_one_grad(OneArgs a, const int* __restrict__ satom,
-                                                 const double* __restrict__ P, const double* __restrict__ W,
-                                                 double* __restrict__ grad) {
-  // grid: x over shell pairs, y over the parts of one pair -- 0: overlap and
-  // kinetic, y >= 1: the nuclear attraction of the system's (y - 1)-th nucleus
+template <int LA, int LB, int KSPLIT, bool VPART>
+__global__ __launch_bounds__(64) void k_one_grad_t(OneArgs a, int p0, int np, const int* __restrict__ satom,
+                                                   const double* __restrict__ P, const double* __restrict__ W,
+                                                   double* __restrict__ grad) {
+  constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
+  // per-block sums of the nuclear contributions in LDS (atoms from the first
+  // nucleus of the block's first pair's system; others go straight to grad)
+  __shared__ double gs[3 * ONE_GRAD_NAT];
+  for (int i = threadIdx.x; i < 3 * ONE_GRAD_NAT; i += blockDim.x) gs[i] = 0.0;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.npair) return;
-  const int slot = blockIdx.y;
-  const int si = a.pA[t], sj = a.pB[t];
-  const int* __restrict__ sl = a.sl;
-  const int* __restrict__ snp = a.snp;
-  const int* __restrict__ spoff = a.spoff;
-  const double* __restrict__ ctr1 = a.ctr1;
-  const double* __restrict__ exps = a.exps;
-  const double* __restrict__ coefs = a.coefs;
-  const double* __restrict__ nxyz = a.nxyz;
-  const double* __restrict__ nZ = a.nZ;
-  const double* __restrict__ master = a.master;
+  const int tc = min(t, np * KSPLIT - 1);
+  const int pos = p0 + tc / KSPLIT, part = tc % KSPLIT;
+  const int slot = VPART ? blockIdx.y + 1 : 0;
+  const int si = a.pA[pos], sj = a.pB[pos];
   const int sys = a.ssys[si];
   const int c0 = a.sys_atom ? a.sys_atom[sys] : 0, c1 = a.sys_atom ? a.sys_atom[sys + 1] : 0;
-  if (slot > 0 && c0 + slot - 1 >= c1) return;
-  const int la = sl[si], lb = sl[sj];
-  const int na = ncart_d(la), nb = ncart_d(lb);
-  const double A[3] = {ctr1[3 * si], ctr1[3 * si + 1], ctr1[3 * si + 2]};
-  const double B[3] = {ctr1[3 * sj], ctr1[3 * sj + 1], ctr1[3 * sj + 2]};
+  const int first = p0 + min((int)(blockIdx.x * blockDim.x), np * KSPLIT - 1) / KSPLIT;
+  const int base = a.sys_atom ? a.sys_atom[a.ssys[a.pA[first]]] : 0;
+  const int npa = a.snp[si], npb = a.snp[sj];
+  const bool active = t < np * KSPLIT && !(slot > 0 && c0 + slot - 1 >= c1) && part < npa;
+  __syncthreads();
+  const double A[3] = {a.ctr1[3 * si], a.ctr1[3 * si + 1], a.ctr1[3 * si + 2]};
+  const double B[3] = {a.ctr1[3 * sj], a.ctr1[3 * sj + 1], a.ctr1[3 * sj + 2]};
   const int ao = a.sbf[si], bo = a.sbf[sj];
   const int la_ = a.slbf[si], lb_ = a.slbf[sj];
   const long N = a.sy.n[sys], mb = a.sy.mb[sys];
   const double wpair = (si == sj) ? 1.0 : 2.0;  // both triangles of an off-diagonal block
-  double pab[36], wab[36];
-  for (int ka = 0; ka < na; ++ka)
-    for (int kb = 0; kb < nb; ++kb) {
+  double pab[NA * NB], wab[NA * NB];
+#pragma unroll
+  for (int ka = 0; ka < NA; ++ka)
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
       const long i = la_ + ka, j = lb_ + kb;
-      const double s = wpair * a.scale[ao + ka] * a.scale[bo + kb];
-      pab[ka * nb + kb] = s * P[mb + i * N + j];
-      wab[ka * nb + kb] = s * W[mb + i * N + j];
+      const double sc = wpair * a.scale[ao + ka] * a.scale[bo + kb];
+      pab[ka * NB + kb] = sc * P[mb + i * N + j];
+      wab[ka * NB + kb] = slot == 0 ? sc * W[mb + i * N + j] : 0.0;
     }
-  double E[3][ONE_GRAD_I + 1][ONE_GRAD_J + 1][ONE_GRAD_T + 1];
-  double R[ONE_RDIM][ONE_RDIM][ONE_RDIM], Rs[ONE_RDIM][ONE_RDIM][ONE_RDIM];
-  double F[ONE_RDIM + 1];
-  double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-  // overlap and kinetic: depend on A - B only
-  for (int ia = 0; slot == 0 && ia < snp[si]; ++ia) {
-    const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
-    for (int ib = 0; ib < snp[sj]; ++ib) {
-      const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
-      const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
-      double Pc[3], r2 = 0.0;
-      for (int x = 0; x < 3; ++x) r2 += (A[x] - B[x]) * (A[x] - B[x]);
+  double r2 = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) r2 += (A[x] - B[x]) * (A[x] - B[x]);
+  double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
+  const int c = c0 + slot - 1;
+  for (int ia = part; active && ia < npa; ia += KSPLIT) {
+    const double al = a.exps[a.spoff[si] + ia], ca = a.coefs[a.spoff[si] + ia];
+    for (int ib = 0; ib < npb; ++ib) {
+      const double be = a.exps[a.spoff[sj] + ib], cb = a.coefs[a.spoff[sj] + ib];
+      const double p = al + be, o2p = 0.5 / p, mu = al * be / p;
       const double kab = exp(-mu * r2) * ca * cb;
       if (fabs(kab) < ONE_PRIM_EPS) continue;
+      double Pc[3], PA[3], PB[3];
+#pragma unroll
       for (int x = 0; x < 3; ++x) {
-        Pc[x] = (a * A[x] + b * B[x]) / p;
-        hermite_e_g(la + 1, lb + 2, Pc[x] - A[x], Pc[x] - B[x], o2p, E[x]);
+        Pc[x] = (al * A[x] + be * B[x]) / p;
+        PA[x] = Pc[x] - A[x];
+        PB[x] = Pc[x] - B[x];
       }
-      const double s00 = kab * pow(M_PI / p, 1.5);
-      auto S1 = [&](int x, int i, int j) { return E[x][i][j][0]; };
-      // <a|T|b> for exponent triples ea, eb (the reference's del-operator expansion on b)
-      auto Tval = [&](const int* ea, const int* eb) {
-        const double sx = S1(0, ea[0], eb[0]), sy = S1(1, ea[1], eb[1]), sz = S1(2, ea[2], eb[2]);
-        const int lbt = eb[0] + eb[1] + eb[2];
-        double tv = b * (2 * lbt + 3) * sx * sy * sz;
-        tv -= 2.0 * b * b * (S1(0, ea[0], eb[0] + 2) * sy * sz + sx * S1(1, ea[1], eb[1] + 2) * sz +
-                             sx * sy * S1(2, ea[2], eb[2] + 2));
-        if (eb[0] >= 2) tv -= 0.5 * eb[0] * (eb[0] - 1) * S1(0, ea[0], eb[0] - 2) * sy * sz;
-        if (eb[1] >= 2) tv -= 0.5 * eb[1] * (eb[1] - 1) * sx * S1(1, ea[1], eb[1] - 2) * sz;
-        if (eb[2] >= 2) tv -= 0.5 * eb[2] * (eb[2] - 1) * sx * sy * S1(2, ea[2], eb[2] - 2);
-        return tv;
-      };
-      auto Sval = [&](const int* ea, const int* eb) {
-        return S1(0, ea[0], eb[0]) * S1(1, ea[1], eb[1]) * S1(2, ea[2], eb[2]);
-      };
-      for (int ka = 0; ka < na; ++ka) {
-        int ea[3];
-        cart_d(la, ka, ea);
-        for (int kb = 0; kb < nb; ++kb) {
-          int eb[3];
-          cart_d(lb, kb, eb);
-          const double pw = pab[ka * nb + kb], ww = wab[ka * nb + kb];
-          for (int x = 0; x < 3; ++x) {
-            int up[3] = {ea[0], ea[1], ea[2]};
-            up[x] += 1;
-            double dS = 2.0 * a * Sval(up, eb), dT = 2.0 * a * Tval(up, eb);
-            if (ea[x] > 0) {
-              int dn[3] = {ea[0], ea[1], ea[2]};
-              dn[x] -= 1;
-              dS -= ea[x] * Sval(dn, eb);
-              dT -= ea[x] * Tval(dn, eb);
+      if constexpr (!VPART) {
+        // overlap and kinetic (depend on A - B only): E_0 of i <= LA + 1, j <= LB + 2
+        double Ex[LA + 2][LB + 3][LA + LB + 4], Ey[LA + 2][LB + 3][LA + LB + 4], Ez[LA + 2][LB + 3][LA + LB + 4];
+        herm_e_t<LA + 1, LB + 2>(PA[0], PB[0], o2p, Ex);
+        herm_e_t<LA + 1, LB + 2>(PA[1], PB[1], o2p, Ey);
+        herm_e_t<LA + 1, LB + 2>(PA[2], PB[2], o2p, Ez);
+        const double s00 = kab * pow(M_PI / p, 1.5);
+        auto S1 = [&](int x, int i, int j) -> double {
+          return x == 0 ? Ex[i][j][0] : x == 1 ? Ey[i][j][0] : Ez[i][j][0];
+        };
+#pragma unroll
+        for (int ka = 0; ka < NA; ++ka)
+#pragma unroll
+          for (int kb = 0; kb < NB; ++kb) {
+            const double pw = pab[ka * NB + kb], ww = wab[ka * NB + kb];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+              // <a +- 1_x | S, T | b> for the derivative on A
+              double dS = 0.0, dT = 0.0;
+#pragma unroll
+              for (int sgn = 0; sgn < 2; ++sgn) {
+                const int dx = sgn == 0 ? 1 : -1;
+                const int eax = cart_e(LA, ka, x) + dx;
+                if (eax < 0) continue;
+                const double w = sgn == 0 ? 2.0 * al : -(double)cart_e(LA, ka, x);
+                int ea[3], eb[3];
+#pragma unroll
+                for (int y = 0; y < 3; ++y) {
+                  ea[y] = cart_e(LA, ka, y) + (y == x ? dx : 0);
+                  eb[y] = cart_e(LB, kb, y);
+                }
+                const double sx = S1(0, ea[0], eb[0]), sy = S1(1, ea[1], eb[1]), sz = S1(2, ea[2], eb[2]);
+                const int lbt = eb[0] + eb[1] + eb[2];
+                double tv = be * (2 * lbt + 3) * sx * sy * sz;
+                tv -= 2.0 * be * be * (S1(0, ea[0], eb[0] + 2) * sy * sz + sx * S1(1, ea[1], eb[1] + 2) * sz +
+                                       sx * sy * S1(2, ea[2], eb[2] + 2));
+                if (eb[0] >= 2) tv -= 0.5 * eb[0] * (eb[0] - 1) * S1(0, ea[0], eb[0] - 2) * sy * sz;
+                if (eb[1] >= 2) tv -= 0.5 * eb[1] * (eb[1] - 1) * sx * S1(1, ea[1], eb[1] - 2) * sz;
+                if (eb[2] >= 2) tv -= 0.5 * eb[2] * (eb[2] - 1) * sx * sy * S1(2, ea[2], eb[2] - 2);
+                dS += w * sx * sy * sz;
+                dT += w * tv;
+              }
+              const double g = s00 * (pw * dT - ww * dS);
+              gA[x] += g;
+              gB[x] -= g;
             }
-            const double g = s00 * (pw * dT - ww * dS);
-            gA[x] += g;
-            gB[x] -= g;
           }
-        }
-      }
-    }
-  }
-  // nuclear attraction of one nucleus
-  const int L = la + lb + 1;
-  for (int c = c0 + slot - 1; slot > 0 && c == c0 + slot - 1; ++c) {
-    double gC[3] = {0.0, 0.0, 0.0};
-    for (int ia = 0; ia < snp[si]; ++ia) {
-      const double a = exps[spoff[si] + ia], ca = coefs[spoff[si] + ia];
-      for (int ib = 0; ib < snp[sj]; ++ib) {
-        const double b = exps[spoff[sj] + ib], cb = coefs[spoff[sj] + ib];
-        const double p = a + b, o2p = 0.5 / p, mu = a * b / p;
-        double Pc[3], r2 = 0.0;
-        for (int x = 0; x < 3; ++x) r2 += (A[x] - B[x]) * (A[x] - B[x]);
-        const double kab = exp(-mu * r2) * ca * cb;
-        if (fabs(kab) < ONE_PRIM_EPS) continue;
-        for (int x = 0; x < 3; ++x) {
-          Pc[x] = (a * A[x] + b * B[x]) / p;
-          hermite_e_g(la + 1, lb + 1, Pc[x] - A[x], Pc[x] - B[x], o2p, E[x]);
-        }
-        const double PC[3] = {Pc[0] - nxyz[3 * c], Pc[1] - nxyz[3 * c + 1], Pc[2] - nxyz[3 * c + 2]};
-        const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-        boys_master(L, Tb, master, F);
-        hermite_r(L, p, PC, F, R, Rs);
-        const double pref = -nZ[c] * 2.0 * M_PI / p * kab;
-        auto Vval = [&](const int* ea, const int* eb) {
+      } else {
+        // nuclear attraction of nucleus c: E of i <= LA + 1, j <= LB + 1, R up to LA + LB + 1
+        constexpr int L = LA + LB + 1;
+        double Ex[LA + 2][LB + 2][LA + LB + 3], Ey[LA + 2][LB + 2][LA + LB + 3], Ez[LA + 2][LB + 2][LA + LB + 3];
+        herm_e_t<LA + 1, LB + 1>(PA[0], PB[0], o2p, Ex);
+        herm_e_t<LA + 1, LB + 1>(PA[1], PB[1], o2p, Ey);
+        herm_e_t<LA + 1, LB + 1>(PA[2], PB[2], o2p, Ez);
+        const double PC[3] = {Pc[0] - a.nxyz[3 * c], Pc[1] - a.nxyz[3 * c + 1], Pc[2] - a.nxyz[3 * c + 2]};
+        double F[L + 1];
+        boys_master_t<L>(p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), a.master, F);
+        double R[L + 1][L + 1][L + 1];
+        herm_r_t<L>(p, PC, F, R);
+        const double pref = -a.nZ[c] * 2.0 * M_PI / p * kab;
+        auto Vval = [&](const int (&ea)[3], const int (&eb)[3]) -> double {
           double sum = 0.0;
-          for (int tt = 0; tt <= ea[0] + eb[0]; ++tt)
-            for (int u = 0; u <= ea[1] + eb[1]; ++u)
-              for (int v = 0; v <= ea[2] + eb[2]; ++v)
-                sum += E[0][ea[0]][eb[0]][tt] * E[1][ea[1]][eb[1]][u] * E[2][ea[2]][eb[2]][v] * R[tt][u][v];
+#pragma unroll
+          for (int tt = 0; tt <= LA + LB + 2; ++tt) {
+            if (tt > ea[0] + eb[0]) break;
+#pragma unroll
+            for (int u = 0; u <= LA + LB + 2; ++u) {
+              if (u > ea[1] + eb[1]) break;
+#pragma unroll
+              for (int v = 0; v <= LA + LB + 2; ++v) {
+                if (v > ea[2] + eb[2]) break;
+                sum += Ex[ea[0]][eb[0]][tt] * Ey[ea[1]][eb[1]][u] * Ez[ea[2]][eb[2]][v] * R[tt][u][v];
+              }
+            }
+          }
           return sum;
         };
-        for (int ka = 0; ka < na; ++ka) {
-          int ea[3];
-          cart_d(la, ka, ea);
-          for (int kb = 0; kb < nb; ++kb) {
-            int eb[3];
-            cart_d(lb, kb, eb);
-            const double pw = pref * pab[ka * nb + kb];
+#pragma unroll
+        for (int ka = 0; ka < NA; ++ka)
+#pragma unroll
+          for (int kb = 0; kb < NB; ++kb) {
+            const double pw = pref * pab[ka * NB + kb];
+#pragma unroll
             for (int x = 0; x < 3; ++x) {
-              int ua[3] = {ea[0], ea[1], ea[2]}, ub[3] = {eb[0], eb[1], eb[2]};
+              int e0[3], f0[3];
+#pragma unroll
+              for (int y = 0; y < 3; ++y) {
+                e0[y] = cart_e(LA, ka, y);
+                f0[y] = cart_e(LB, kb, y);
+              }
+              int ua[3] = {e0[0], e0[1], e0[2]}, ub[3] = {f0[0], f0[1], f0[2]};
               ua[x] += 1;
               ub[x] += 1;
-              double dA = 2.0 * a * Vval(ua, eb), dB = 2.0 * b * Vval(ea, ub);
-              if (ea[x] > 0) {
-                int da[3] = {ea[0], ea[1], ea[2]};
+              double dA = 2.0 * al * Vval(ua, f0), dB = 2.0 * be * Vval(e0, ub);
+              if (e0[x] > 0) {
+                int da[3] = {e0[0], e0[1], e0[2]};
                 da[x] -= 1;
-                dA -= ea[x] * Vval(da, eb);
+                dA -= e0[x] * Vval(da, f0);
               }
-              if (eb[x] > 0) {
-                int db[3] = {eb[0], eb[1], eb[2]};
+              if (f0[x] > 0) {
+                int db[3] = {f0[0], f0[1], f0[2]};
                 db[x] -= 1;
-                dB -= eb[x] * Vval(ea, db);
+                dB -= f0[x] * Vval(e0, db);
               }
               gA[x] += pw * dA;
               gB[x] += pw * dB;
               gC[x] -= pw * (dA + dB);
             }
           }
-        }
       }
     }
-    for (int x = 0; x < 3; ++x)
-      if (gC[x] != 0.0) atomicAdd(&grad[3 * c + x], gC[x]);
   }
-  const int atA = satom[si], atB = satom[sj];
-  for (int x = 0; x < 3; ++x) {
-    atomicAdd(&grad[3 * atA + x], gA[x]);
-    atomicAdd(&grad[3 * atB + x], gB[x]);
+  auto add = [&](int atom, int x, double v) {
+    if (v == 0.0) return;
+    const int i = atom - base;
+    if (i >= 0 && i < ONE_GRAD_NAT) atomicAdd(&gs[3 * i + x], v);
+    else atomicAdd(&grad[3 * atom + x], v);
+  };
+  if (active) {
+    const int atA = satom[si], atB = satom[sj];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      if (VPART) add(c, x, gC[x]);
+      add(atA, x, gA[x]);
+      add(atB, x, gB[x]);
+    }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * ONE_GRAD_NAT; i += blockDim.x)
+    if (gs[i] != 0.0) atomicAdd(&grad[3 * base + i], gs[i]);
 }
 
 // Nuclear repulsion gradient  dE_nuc/dR_A = -sum_B Z_A Z_B (R_A - R_B) / |R_A - R_B|^3
