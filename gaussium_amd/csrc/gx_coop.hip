@@ -328,9 +328,11 @@ struct GradArgs {
   double* grad;        // 3 natom
 };
 
-// Gradient contributions of one work item collect in LDS per nucleus of the
-// item's system (at most GX_GRAD_MAXATOM) and go to HBM once per item: the
-// few nuclei of a system would otherwise take every quartet's atomics.
+// Gradient contributions collect in LDS per nucleus of the item's system (at
+// most GX_GRAD_MAXATOM) over all the items a workgroup takes from that
+// system, and go to HBM when the system changes and at the end: the few
+// nuclei of a system would otherwise take every quartet's (or every item's)
+// atomics, all on the same addresses.
 #define GX_GRAD_MAXATOM 256
 
 template <class G>
@@ -352,6 +354,9 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
   const int lane = threadIdx.x;
   const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
   const int2* __restrict__ gt = G::gterm();
+  // nuclear sums of the current system, kept in LDS across the items of
+  // the workgroup and flushed when the system changes and at the end
+  int cur = -1, cur3 = 0;
   for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
@@ -373,7 +378,13 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
       r_start[lane] = a.seg_lo[s0 + lane];
       r_pre[lane + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + lane];  // gradients are FP64 only
     }
-    for (int t = lane; t < nat3; t += GX_WAVE) gsh[t] = 0.0;
+    if (at0 != cur) {  // another system: flush the sums held for the previous one
+      for (int t = lane; t < cur3; t += GX_WAVE)
+        if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
+      for (int t = lane; t < nat3; t += GX_WAVE) gsh[t] = 0.0;
+      cur = at0;
+      cur3 = nat3;
+    }
     __syncthreads();
     if (lane == 0) {
       r_pre[0] = 0;
@@ -443,10 +454,10 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
         }
       }
     }
-    __syncthreads();
-    for (int t = lane; t < nat3; t += GX_WAVE)
-      if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * at0 + t], gsh[t]);
   }
+  __syncthreads();
+  for (int t = lane; t < cur3; t += GX_WAVE)
+    if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
 }
 
 }  // namespace gx

@@ -22,6 +22,9 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
   const int wave = tid / GX_WAVE;
   constexpr int nwave = JK_TEAM / GX_WAVE;
   const double* __restrict__ tab = a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
+  // nuclear sums of the current system, kept in LDS across the items of
+  // the workgroup and flushed when the system changes and at the end
+  int cur = -1, cur3 = 0;
   for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
@@ -46,7 +49,13 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
       prow[i] = Pm[(long)(r < NA ? ao + r : bo + (r - NA)) * N + c];
     }
     if (tid == 0) next_chunk = nwave;
-    for (int t = tid; t < nat3; t += JK_TEAM) gsh[t] = 0.0;
+    if (at0 != cur) {  // another system: flush the sums held for the previous one
+      for (int t = tid; t < cur3; t += JK_TEAM)
+        if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
+      for (int t = tid; t < nat3; t += JK_TEAM) gsh[t] = 0.0;
+      cur = at0;
+      cur3 = nat3;
+    }
     if (tid < nseg) {
       r_start[tid] = a.seg_lo[s0 + tid];
       r_pre[tid + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + tid];  // gradients are FP64 only
@@ -154,10 +163,10 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
         atomicAdd(&gsh[3 * atB + x], gab[3 + x]);
       }
     }
-    __syncthreads();
-    for (int t = tid; t < nat3; t += JK_TEAM)
-      if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * at0 + t], gsh[t]);
   }
+  __syncthreads();
+  for (int t = tid; t < cur3; t += JK_TEAM)
+    if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
 }
 
 }  // namespace gx
