@@ -72,21 +72,16 @@ def timed_builds(eng, P, steps, warmup, **kw):
             'kernel_ms': kernels}
 
 
-def kernel_name(combo, mixed):
+def kernel_name(combo, mixed, fp64_combos=()):
+    """Kernel of class combination `combo`; in a mixed build the combinations
+    the autotuner put in FP64 (`fp64_combos`) run the FP64 kernel."""
     la, lb, lc, ld = combo
-    return 'k_jk<%d, %d, %d, %d, %s>' % (la, lb, lc, ld, 'true' if mixed else 'false')
+    m = mixed and tuple(combo) not in fp64_combos
+    return 'k_jk<%d, %d, %d, %d, %s>' % (la, lb, lc, ld, 'true' if m else 'false')
 
 
 def class_rows(eng, mixed):
-    import ctypes
-    import numpy as np
-    from gaussium_amd import _lib
-    out = np.zeros((64, 10), dtype=np.int64)
-    n = eng.lib.gx_jk_class_counts(eng.ctx, CUTOFF, FP32_BOUND, _lib.GX_JK_MIXED if mixed else 0,
-                                   out.ctypes.data_as(ctypes.c_void_p), 64)
-    if n < 0:
-        _lib.check(n, 'gx_jk_class_counts')
-    return [tuple(int(x) for x in r) for r in out[:n]]
+    return eng.jk_class_counts(cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=mixed)
 
 
 def cpu_baseline(basis, seconds):
@@ -118,9 +113,10 @@ def kernels_only(args):
     P = density_matrix_restricted(C, ne)
     out = {'workload': info['workload'], 'copy_bytes': COPY_BYTES}
     for name, kw in (('fp64_build', {}), ('mixed_build', dict(fp32_bound=FP32_BOUND, mixed=True))):
-        eng.autotune(P, cutoff=CUTOFF, reps=3, **kw)  # as in the full bench (its launches come first)
+        t = eng.autotune(P, cutoff=CUTOFF, reps=3, **kw)  # as in the full bench (its launches come first)
+        f64c = {k for k, v in t.items() if v[0] < 0}
         r = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF, **kw)
-        out[name] = {kernel_name(k, name == 'mixed_build'): v for k, v in r['kernel_ms'].items()}
+        out[name] = {kernel_name(k, name == 'mixed_build', f64c): v for k, v in r['kernel_ms'].items()}
         out[name]['median_build_ms'] = r['median_ms']
     src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device='cuda')
     dst = torch.empty_like(src)
@@ -298,6 +294,11 @@ def main():
                      'core_hamiltonian': hf.core_hamiltonian}
     P = scf['fp64']['density']
     H = scf['fp64']['core_hamiltonian']
+    # work-item sizes (and, in mixed builds, FP32 or FP64 per class
+    # combination) tuned once per mode (untimed set-up), before the energies
+    tuned = {'fp64': eng.autotune(P, cutoff=CUTOFF, reps=3),
+             'mixed': eng.autotune(P, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True, reps=3)}
+    fp64_combos = {k for k, v in tuned['mixed'].items() if v[0] < 0}
     energy_at = {}
     for mode, kw in (('fp64', {}), ('mixed', dict(fp32_bound=FP32_BOUND, mixed=True))):
         J, K = eng.jk(P, cutoff=CUTOFF, **kw)
@@ -313,9 +314,6 @@ def main():
             'dE_fp64_vs_unscreened_Eh': abs(energy_at['fp64'] - e_full),
             'jk_ms_fp64_unscreened': fp64_full['median_ms']}
     full.close()
-    # work-item sizes per class combination, tuned once per mode (untimed set-up)
-    tuned = {'fp64': eng.autotune(P, cutoff=CUTOFF, reps=3),
-             'mixed': eng.autotune(P, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True, reps=3)}
     fp64 = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF)
     mixed = timed_builds(eng, P, args.steps, args.warmup, cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=True)
     st64 = eng.jk_stats(cutoff=CUTOFF)
@@ -368,7 +366,8 @@ def main():
         'dE_definition': 'fixed density: |E[P] mixed J/K - E[P] FP64 J/K|, P = final FP64 SCF density',
         'energy_at_density': energy_at,
         'primitive_pairs': prim,
-        'autotuned_items_per_wg': {m: {'%d%d%d%d' % k: v[0] for k, v in t.items()} for m, t in tuned.items()},
+        'autotuned_items_per_wg': {m: {'%d%d%d%d' % k: abs(v[0]) for k, v in t.items()} for m, t in tuned.items()},
+        'mixed_build_fp64_combinations': ['%d%d%d%d' % k for k, v in tuned['mixed'].items() if v[0] < 0],
         'scf': {m: {'energy': scf[m]['energy'], 'iterations': scf[m]['iterations'],
                     'converged': scf[m]['converged'], 'wall_s': scf[m]['wall_s']} for m in scf},
         'quartets': {'canonical_bf': st64['bf_quartets'], 'canonical_shell': st64['shell_quartets'],
@@ -378,7 +377,7 @@ def main():
                      if ok else st64['bf_quartets_fp64'] / t},
         'roofline': {'bound': 'valu', 'achieved': k_achieved, 'peak': k_peak, 'unit': 'TFLOP/s',
                      'frac': k_achieved / k_peak, 'traffic': None,
-                     'kernel': kernel_name(combo, ok), 'kernel_ms': head['kernel_ms'][combo],
+                     'kernel': kernel_name(combo, ok, fp64_combos), 'kernel_ms': head['kernel_ms'][combo],
                      'kernel_share': head['kernel_ms'][combo] / head['median_ms'],
                      'flops_per_launch': [k64, k32],
                      'build': {'flops': b_flops, 'achieved': b_flops / t / 1e12, 'peak': b_peak,
@@ -386,7 +385,7 @@ def main():
                      'fp64_build': {'flops': f64_a, 'achieved': f64_a / (fp64['median_ms'] * 1e-3) / 1e12,
                                     'frac': f64_a / (fp64['median_ms'] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}},
         'kernel_ms': {'fp64_build': {kernel_name(k, False): v for k, v in fp64['kernel_ms'].items()},
-                      'mixed_build': {kernel_name(k, True): v for k, v in mixed['kernel_ms'].items()}},
+                      'mixed_build': {kernel_name(k, True, fp64_combos): v for k, v in mixed['kernel_ms'].items()}},
     }
     line['class_counts'] = {'columns': ['la', 'lb', 'lc', 'ld', 'shq64', 'shq32', 'primq64', 'primq32', 'bfq64',
                                         'bfq32'], 'fp64_build': rows64, 'mixed_build': rowsmx}

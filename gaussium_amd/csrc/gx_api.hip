@@ -158,6 +158,9 @@ struct gx_ctx {
       mixed = -1;
     }
   } jk, gr;
+  // per class combination: 0 when a mixed build evaluates it all in FP64
+  // (chosen by gx_jk_autotune where the FP32 kernel is not faster)
+  std::vector<char> jk_fp32;
   std::vector<int> wg_per_cu;  // [combo][mixed] resident workgroups per CU, 0 = not queried
   std::vector<int> wg_per_cu_grad;
   int* d_atom = nullptr;
@@ -208,6 +211,7 @@ struct gx_ctx {
     gr.release();
     jk.ipw.clear();
     gr.ipw.clear();
+    jk_fp32.clear();
     wg_per_cu.clear();
     wg_per_cu_grad.clear();
     dfree(&d_seg_lo);
@@ -616,6 +620,8 @@ int host_count(const gx_ctx* c, int lo, int hi, double QA, double thr) {
 template <class F>
 void for_each_bra_segment(const gx_ctx* c, int X, int Y, double cutoff, double fp32, int mixed, F&& f) {
   if (combo_is_coop(X, Y)) mixed = 0;  // cooperative classes are always FP64
+  const size_t combo = (size_t)X * (X + 1) / 2 + Y;
+  if (combo < c->jk_fp32.size() && !c->jk_fp32[combo]) mixed = 0;  // tuned to FP64
   const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
   for (int i = 0; i < nb; ++i) {
     const int ip = b0 + i;
@@ -770,6 +776,7 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = f
   ws.d_rng.assign(combo_count(), nullptr);
   ws.n_items.assign(combo_count(), 0);
   if ((int)ws.ipw.size() != 2 * combo_count()) ws.ipw.assign(2 * combo_count(), GX_ITEMS_PER_WG);
+  if ((int)c->jk_fp32.size() != combo_count()) c->jk_fp32.assign(combo_count(), 1);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo)
@@ -828,6 +835,7 @@ gx::OneArgs one_args(const gx_ctx* c) {
 
 // The J/K kernel of class combination (X|Y) over its current work items.
 hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed) {
+  if ((size_t)combo < c->jk_fp32.size() && !c->jk_fp32[combo]) mixed = 0;  // this combination runs in FP64
   gx::JKArgs a;
   a.sh = c->shells();
   a.pr = c->pairs();
@@ -1446,27 +1454,31 @@ int gx_jk_autotune(gx_ctx* c, const double* P, double cutoff, double fp32_bound,
         break;
       }
       double& ipw = c->jk.ipw[combo + mixed * combo_count()];
-      const double before = ipw;
-      double best = before, best_ms = 1e30, before_ms = 1e30;
-      for (int k = 0; k < ncand && rc == GX_OK; ++k) {
-        rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, cand[k]);
-        if (rc != GX_OK) break;
-        double ms_min = 1e30;
-        for (int r = 0; r < reps; ++r) {
-          float ms = 0.f;
-          if (hipEventRecord(e0, c->stream) != hipSuccess || launch_combo(c, X, Y, combo, mixed) != hipSuccess ||
-              hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = fail(GX_ERR_HIP, "gx_jk_autotune: timed launch");
-            break;
+      double best = ipw, best_ms = 1e30;
+      int best_fp32 = 1;
+      // mixed builds also try the combination all in FP64 (kept where it is faster)
+      for (int use32 = mixed; use32 >= 0 && rc == GX_OK; --use32) {
+        if (mixed) c->jk_fp32[combo] = (char)use32;
+        for (int k = 0; k < ncand && rc == GX_OK; ++k) {
+          rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, cand[k]);
+          if (rc != GX_OK) break;
+          double ms_min = 1e30;
+          for (int r = 0; r < reps; ++r) {
+            float ms = 0.f;
+            if (hipEventRecord(e0, c->stream) != hipSuccess || launch_combo(c, X, Y, combo, mixed) != hipSuccess ||
+                hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+              rc = fail(GX_ERR_HIP, "gx_jk_autotune: timed launch");
+              break;
+            }
+            ms_min = std::min(ms_min, (double)ms);
           }
-          ms_min = std::min(ms_min, (double)ms);
+          if (ms_min < best_ms) best_ms = ms_min, best = cand[k], best_fp32 = use32;
         }
-        if (ms_min < best_ms) best_ms = ms_min, best = cand[k];
-        if (cand[k] == before) before_ms = ms_min;
       }
       if (rc != GX_OK) break;
       ipw = best;
+      if (mixed) c->jk_fp32[combo] = (char)best_fp32;
       rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, best);
       double* r = out + 6 * row++;
       r[0] = class_la(X);
@@ -1475,7 +1487,7 @@ int gx_jk_autotune(gx_ctx* c, const double* P, double cutoff, double fp32_bound,
       r[3] = class_lb(Y);
       r[4] = best;
       r[5] = best_ms;
-      (void)before_ms;
+      if (mixed && !best_fp32) r[4] = -best;  // negative: the combination runs in FP64
     }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

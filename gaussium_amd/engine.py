@@ -212,7 +212,9 @@ class Engine:
     def autotune(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False,
                  candidates=(1.0, 2.0, 4.0, 8.0, 16.0, 32.0, 64.0), reps=3):
         """Pick the work-item size of each class combination by timing (see
-        gx_jk_autotune); returns {(la, lb, lc, ld): (items per workgroup, ms)}."""
+        gx_jk_autotune); returns {(la, lb, lc, ld): (items per workgroup, ms)}.
+        In mixed mode a negative value marks a combination that runs all in
+        FP64 because that was faster."""
         P = self._dev(P)
         self._matrix_shape(P, 'density')
         cand = np.ascontiguousarray(candidates, dtype=np.float64)
@@ -237,6 +239,17 @@ class Engine:
         if n < 0:
             _lib.check(n, 'gx_grad_autotune')
         return {tuple(int(x) for x in r[:4]): (float(r[4]), float(r[5])) for r in out[:n]}
+
+    def jk_class_counts(self, cutoff=0.0, fp32_bound=0.0, mixed=False):
+        """Per class combination: (la, lb, lc, ld, shell quartets FP64, FP32,
+        primitive quartets FP64, FP32, bf quartets FP64, FP32) that gx_jk
+        evaluates with these thresholds (gx_jk_class_counts)."""
+        out = np.zeros((64, 10), dtype=np.int64)
+        n = self.lib.gx_jk_class_counts(self.ctx, float(cutoff), float(fp32_bound),
+                                        _lib.GX_JK_MIXED if mixed else 0, _p(out), 64)
+        if n < 0:
+            _lib.check(n, 'gx_jk_class_counts')
+        return [tuple(int(x) for x in r) for r in out[:n]]
 
     def jk_launch_times(self):
         """{(la, lb, lc, ld): (total ms, launches)} over the ``jk(..., timing=True)``

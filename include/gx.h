@@ -159,9 +159,12 @@ int gx_jk(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double
  * events on the context stream) with the items sized for each of the
  * `ncand` candidate values of "items per resident workgroup", and the
  * fastest is kept for later gx_jk calls with these thresholds (until the
- * next gx_set_basis).  Writes one row of 6 doubles per combination: la, lb,
- * lc, ld, chosen value, its milliseconds.  Returns the number of rows or an
- * error code.  No counterpart in the reference. */
+ * next gx_set_basis).  With GX_JK_MIXED each combination is also timed with
+ * all its quartets in FP64, and runs so in later mixed builds where that is
+ * faster (never less accurate).  Writes one row of 6 doubles per
+ * combination: la, lb, lc, ld, chosen value (negated when the combination
+ * runs in FP64 in mixed builds), its milliseconds.  Returns the number of
+ * rows or an error code.  No counterpart in the reference. */
 int gx_jk_autotune(gx_ctx* ctx, const double* P_dev, double cutoff, double fp32_bound, int flags,
                    const double* candidates, int ncand, int reps, double* out, int cap);
 

@@ -153,3 +153,26 @@ def test_mixed_precision_jk_error_is_small(Engine):
     assert st['shell_quartets_fp32'] > 0
     assert (Jm - J).abs().max().item() < 1e-8
     assert (Km - K).abs().max().item() < 1e-8
+
+
+def test_mixed_autotune_precision_choice(Engine):
+    """The mixed-mode autotuner may put whole class combinations in FP64
+    (negative tuned values); the build stays within the mixed bar, the FP32
+    quartet count drops to zero exactly for those combinations, and the FP64
+    build is untouched."""
+    from gaussium_amd import workloads
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    P = torch.from_numpy(_random_density(len(basis), np.random.default_rng(9))).cuda()
+    J, K = eng.jk(P, cutoff=1e-11)
+    tuned = eng.autotune(P, cutoff=1e-11, fp32_bound=1e-4, mixed=True, candidates=(2.0, 8.0), reps=1)
+    assert tuned and all(abs(v[0]) in (2.0, 8.0) for v in tuned.values())
+    Jm, Km = eng.jk(P, cutoff=1e-11, fp32_bound=1e-4, mixed=True)
+    assert (Jm - J).abs().max().item() < 1e-8
+    assert (Km - K).abs().max().item() < 1e-8
+    rows = {tuple(r[:4]): r for r in eng.jk_class_counts(cutoff=1e-11, fp32_bound=1e-4, mixed=True)}
+    for combo, (val, ms) in tuned.items():
+        if val < 0:
+            assert rows[combo][5] == 0, combo
+    J2, K2 = eng.jk(P, cutoff=1e-11)
+    assert (J2 - J).abs().max().item() < 1e-11 and (K2 - K).abs().max().item() < 1e-11
