@@ -455,7 +455,7 @@ size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed) {
   GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   const int na = ncart(la), nb = ncart(lb);
-  const bool t64 = !mixed || GX_MIXED_LDS64, t32 = mixed && GX_MIXED_LDS32;
+  const bool t64 = !mixed, t32 = mixed && GX_MIXED_LDS32;
   const size_t b = (t64 ? (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double) : 0) +
                    (t32 ? (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float) : 0);
   return b + 2 * (size_t)(na + nb) * N * sizeof(double);

@@ -156,9 +156,6 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 #ifndef GX_MIXED_LDS32
 #define GX_MIXED_LDS32 1  // mixed builds: FP32 Taylor table in LDS (else read through L1)
 #endif
-#ifndef GX_MIXED_LDS64
-#define GX_MIXED_LDS64 0  // mixed builds: FP64 Taylor table in LDS as well (else read through L1)
-#endif
 #ifndef GX_DYN_CHUNK
 #define GX_DYN_CHUNK 1
 #endif
@@ -360,11 +357,6 @@ __global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int 
 #define JK_TEAM 256  // threads working on one work item
 #endif
 
-template <int L>
-struct JKShape {
-  static constexpr int THREADS = JK_TEAM;
-};
-
 struct JKArgs {
   ShellView sh;
   PairView pr;
@@ -395,15 +387,11 @@ __device__ __forceinline__ double* jt_copy(const JKArgs& a) {
 
 // MIXED = false: pure FP64 build, the FP32 path is compiled out and the FP64
 // Taylor table is staged in LDS.  MIXED = true: most primitive quartets are
-// FP32 and both tables are read through the vector-memory path (L1), which
-// leaves the LDS to the P/K rows and raises occupancy.
+// FP32; the FP32 table is staged in LDS and the FP64 one (for the few FP64
+// quartets) is read through the vector-memory path (L1), which keeps the LDS
+// small and the occupancy up (staging both measured no faster).
 template <int LA, int LB, int LC, int LD, bool MIXED>
-#ifdef GX_JK_WPE
-#define GX_JK_ATTR __attribute__((amdgpu_waves_per_eu(GX_JK_WPE)))
-#else
-#define GX_JK_ATTR
-#endif
-__global__ __launch_bounds__(JK_TEAM) GX_JK_ATTR void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
+__global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
                                                 const PrimPair32* __restrict__ pp32) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
@@ -414,7 +402,7 @@ __global__ __launch_bounds__(JK_TEAM) GX_JK_ATTR void k_jk(JKArgs a, const PrimP
   const int tid = threadIdx.x;
   // dynamic LDS: Boys FP64 (pure FP64 build) | P rows | K rows (rows of the
   // item's system)
-  constexpr bool T64 = !MIXED || GX_MIXED_LDS64;  // FP64 table staged in LDS
+  constexpr bool T64 = !MIXED;                    // FP64 table staged in LDS
   constexpr bool T32 = MIXED && GX_MIXED_LDS32;   // FP32 table staged in LDS
   constexpr int off32 = T64 ? BOYS_NT * BOYS_ROW64 : 0;              // in doubles
   constexpr int offrow = off32 + (T32 ? BOYS_NT * BOYS_ROW32 / 2 : 0);
