@@ -40,9 +40,10 @@ def parse():
     ap.add_argument('--seed', type=int, default=2)
     ap.add_argument('--scf-iter', type=int, default=30, help='SCF iterations of the untimed set-up')
     ap.add_argument('--workload', default='cluster_220bf',
-                    choices=['cluster_220bf', 'gradient_66bf', 'batch_256', 'cluster_255bf_d'],
+                    choices=['cluster_220bf', 'gradient_66bf', 'batch_256', 'cluster_255bf_d', 'uhf_220bf'],
                     help='cluster_220bf: the headline J/K line; the others print one line each for the '
-                         'gradient (66 functions), the 256-geometry batch and the d-shell cluster')
+                         'gradient (66 functions), the 256-geometry batch, the d-shell cluster and the '
+                         'unrestricted (two-density) build')
     ap.add_argument('--kernels-only', action='store_true',
                     help='counter runs: no SCF; time the J/K kernels on the core-guess density and run '
                          'one %d MiB device copy as the HBM-counter calibration' % (COPY_BYTES >> 20))
@@ -77,7 +78,7 @@ def kernel_name(combo, mixed, fp64_combos=()):
     the autotuner put in FP64 (`fp64_combos`) run the FP64 kernel."""
     la, lb, lc, ld = combo
     m = mixed and tuple(combo) not in fp64_combos
-    return 'k_jk<%d, %d, %d, %d, %s>' % (la, lb, lc, ld, 'true' if m else 'false')
+    return 'k_jk<%d, %d, %d, %d, %s, 1>' % (la, lb, lc, ld, 'true' if m else 'false')
 
 
 def class_rows(eng, mixed):
@@ -255,6 +256,41 @@ def bench_d(args):
             'bf_quartets_per_s_fp64': st['bf_quartets'] / (ms64 * 1e-3)}
 
 
+def bench_uhf(args):
+    """Unrestricted J/K of the 220-function cluster (FP64): one two-density
+    build (gx_jk_uhf) against two single-density builds, at alpha/beta
+    densities made from the SCF orbitals with one electron moved from beta to
+    alpha (a triplet occupancy)."""
+    import torch
+    from gaussium_amd import workloads
+    from gaussium_amd.engine import Engine
+    nuc, basis, ne, info = workloads.cluster_220bf()
+    eng = Engine(basis, nuc)
+    P, W, hf = _scf_density(eng, nuc, basis, ne, args.scf_iter)
+    c = hf.scf_method.orbital_coefficients
+    na, nb = ne // 2 + 1, ne // 2 - 1
+    Pa = (c[:, :na] @ c[:, :na].T).contiguous()
+    Pb = (c[:, :nb] @ c[:, :nb].T).contiguous()
+    eng.autotune(P, cutoff=CUTOFF)  # untimed set-up
+    ms_fused = _median_ms(lambda: eng.jk_uhf(Pa, Pb, cutoff=CUTOFF), args.steps, args.warmup)
+
+    def two_builds():
+        eng.jk(Pa, cutoff=CUTOFF)
+        eng.jk(Pb, cutoff=CUTOFF)
+    ms_two = _median_ms(two_builds, args.steps, args.warmup)
+    J, Ka, Kb = eng.jk_uhf(Pa, Pb, cutoff=CUTOFF)
+    Ja, Ka1 = eng.jk(Pa, cutoff=CUTOFF)
+    Jb, Kb1 = eng.jk(Pb, cutoff=CUTOFF)
+    diff = max(float((J - Ja - Jb).abs().max()), float((Ka - Ka1).abs().max()), float((Kb - Kb1).abs().max()))
+    return {'metric': 'unrestricted J/K build ms (220 bf, FP64), one two-density build', 'value': ms_fused,
+            'unit': 'ms', 'higher_is_better': False, 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+            'dtype': 'f64', 'data': 'synthetic (seeded random cluster and basis)',
+            'config': {'workload': info['workload'] + '_uhf', 'nbf': info['nbf'], 'atoms': info['n_atoms'],
+                       'n_alpha': na, 'n_beta': nb, 'schwarz_cutoff': CUTOFF},
+            'two_single_density_builds_ms': ms_two, 'speedup_vs_two_builds': ms_two / ms_fused,
+            'max_abs_diff_vs_two_builds': diff}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -263,7 +299,8 @@ def main():
         sys.exit(2)
     if args.kernels_only:
         return kernels_only(args)
-    extra = {'gradient_66bf': bench_gradient, 'batch_256': bench_batch, 'cluster_255bf_d': bench_d}
+    extra = {'gradient_66bf': bench_gradient, 'batch_256': bench_batch, 'cluster_255bf_d': bench_d,
+             'uhf_220bf': bench_uhf}
     if args.workload in extra:
         print(json.dumps(extra[args.workload](args)), flush=True)
         return

@@ -161,7 +161,7 @@ struct gx_ctx {
   // per class combination: 0 when a mixed build evaluates it all in FP64
   // (chosen by gx_jk_autotune where the FP32 kernel is not faster)
   std::vector<char> jk_fp32;
-  std::vector<int> wg_per_cu;  // [combo][mixed] resident workgroups per CU, 0 = not queried
+  std::vector<int> wg_per_cu;  // [combo][FP64, mixed, two-density] resident workgroups per CU, 0 = not queried
   std::vector<int> wg_per_cu_grad;
   int* d_atom = nullptr;
   // ket segments: per class, runs of pairs with the same primitive-count key
@@ -185,6 +185,8 @@ struct gx_ctx {
   int num_cu = 256;
   // scratch
   double *d_Ps = nullptr, *d_Jt = nullptr, *d_Kt = nullptr;
+  // two-density (unrestricted) builds, allocated on first use
+  double *d_Ps2 = nullptr, *d_Psj = nullptr, *d_Kt2 = nullptr;
   // nuclei
   int natom = 0;
   std::vector<double> nxyz, nZ;
@@ -240,6 +242,9 @@ struct gx_ctx {
     dfree(&d_Ps);
     dfree(&d_Jt);
     dfree(&d_Kt);
+    dfree(&d_Ps2);
+    dfree(&d_Psj);
+    dfree(&d_Kt2);
     dfree(&d_nxyz);
     dfree(&d_nZ);
     dfree(&d_boys);
@@ -344,11 +349,12 @@ hipError_t launch_quartets(int la, int lb, int lc, int ld, hipStream_t s, gx::Sh
 // Persistent J/K launch: as many workgroups as stay resident (queried once
 // per kernel and basis, cached in *per_cu), at most one per work item.
 hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hipStream_t s,
-                     const gx::JKArgs& a, int* per_cu) {
+                     const gx::JKArgs& a, int* per_cu, int nden = 1) {
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
-    const void* fn = a.mixed ? (const void*)gx::k_jk<A, B, C, D, true>                       \
-                             : (const void*)gx::k_jk<A, B, C, D, false>;                      \
+    const void* fn = nden == 2 ? (const void*)gx::k_jk<A, B, C, D, false, 2>                 \
+                     : a.mixed ? (const void*)gx::k_jk<A, B, C, D, true, 1>                   \
+                               : (const void*)gx::k_jk<A, B, C, D, false, 1>;                 \
     /* the attribute is per function, shared by every context: set it each time */          \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
@@ -358,11 +364,14 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
       if (*per_cu < 1) *per_cu = 1;                                                           \
     }                                                                                         \
     const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
-    if (a.mixed)                                                                              \
-      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, true>), dim3(grid), dim3(JK_TEAM), lds, s, a,  \
+    if (nden == 2)                                                                            \
+      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, false, 2>), dim3(grid), dim3(JK_TEAM), lds, s, a, \
+                         a.pr.pp64, a.pr.pp32);                                               \
+    else if (a.mixed)                                                                         \
+      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, true, 1>), dim3(grid), dim3(JK_TEAM), lds, s, a, \
                          a.pr.pp64, a.pr.pp32);                                               \
     else                                                                                      \
-      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, false>), dim3(grid), dim3(JK_TEAM), lds, s, a, \
+      hipLaunchKernelGGL((gx::k_jk<A, B, C, D, false, 1>), dim3(grid), dim3(JK_TEAM), lds, s, a, \
                          a.pr.pp64, a.pr.pp32);                                               \
     return hipGetLastError();                                                                 \
   }
@@ -445,7 +454,7 @@ bool combo_is_coop(int X, int Y) { return gx_class_is_coop(class_la(X), class_lb
 
 // Must match the layout at the top of gx::k_jk (per-lane classes) or
 // gx::k_jk_coop (cooperative classes).
-size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed) {
+size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed, int nden = 1) {
 #define GX_CASE(A, B, C, D)                                                                     \
   if (la == A && lb == B && lc == C && ld == D) {                                               \
     using G = EriProg<A, B, C, D>;                                                              \
@@ -458,7 +467,7 @@ size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed) {
   const bool t64 = !mixed, t32 = mixed && GX_MIXED_LDS32;
   const size_t b = (t64 ? (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double) : 0) +
                    (t32 ? (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float) : 0);
-  return b + 2 * (size_t)(na + nb) * N * sizeof(double);
+  return b + 2 * (size_t)nden * (na + nb) * N * sizeof(double);
 }
 
 // Build the primitive-pair records of the current pair list on the device.
@@ -834,8 +843,11 @@ gx::OneArgs one_args(const gx_ctx* c) {
 }
 
 // The J/K kernel of class combination (X|Y) over its current work items.
-hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed) {
+// nden = 2: a two-density (unrestricted) build in FP64: J of Ps + Ps2 into
+// Jt, K of Ps into Kt and of Ps2 into Kt2.
+hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden = 1) {
   if ((size_t)combo < c->jk_fp32.size() && !c->jk_fp32[combo]) mixed = 0;  // this combination runs in FP64
+  if (nden == 2) mixed = 0;
   gx::JKArgs a;
   a.sh = c->shells();
   a.pr = c->pairs();
@@ -850,14 +862,28 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed) {
   a.tab64 = c->d_tab64;
   a.tab32 = c->d_tab32;
   a.P = c->d_Ps;
+  a.P2 = nden == 2 ? c->d_Ps2 : c->d_Ps;
+  a.Pj = nden == 2 ? c->d_Psj : c->d_Ps;
   a.Jt = c->d_Jt;
   a.Kt = c->d_Kt;
+  a.Kt2 = nden == 2 ? c->d_Kt2 : c->d_Kt;
   a.jstride = c->mtot;
   a.mixed = mixed;
-  if (c->wg_per_cu.empty()) c->wg_per_cu.assign(2 * combo_count(), 0);
-  return launch_jk(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu,
-                   jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->nmax, mixed), c->stream, a,
-                   &c->wg_per_cu[2 * combo + mixed]);
+  if (c->wg_per_cu.empty()) c->wg_per_cu.assign(3 * combo_count(), 0);
+  const int la = class_la(X), lb = class_lb(X), lc = class_la(Y), ld = class_lb(Y);
+  int* per_cu = &c->wg_per_cu[3 * combo + (nden == 2 ? 2 : mixed)];
+  if (nden == 2 && combo_is_coop(X, Y)) {
+    // the cooperative kernels digest one density: run them once per density
+    // (J is linear, so both launches add their part of J(P + P2) to Jt)
+    const size_t lds = jk_lds_bytes(la, lb, lc, ld, c->nmax, 0);
+    hipError_t e = launch_jk(la, lb, lc, ld, c->num_cu, lds, c->stream, a, per_cu);
+    if (e != hipSuccess) return e;
+    a.P = a.Pj = c->d_Ps2;
+    a.Kt = c->d_Kt2;
+    return launch_jk(la, lb, lc, ld, c->num_cu, lds, c->stream, a, per_cu);
+  }
+  return launch_jk(la, lb, lc, ld, c->num_cu, jk_lds_bytes(la, lb, lc, ld, c->nmax, mixed, nden), c->stream, a,
+                   per_cu, nden);
 }
 
 // One-electron gradient of the shell pairs of class (la, lb): one thread per
@@ -1071,6 +1097,10 @@ int gx_set_basis_batch(gx_ctx* c, int nsys, const int* sys_shell, const int* sys
   GX_TRY(dalloc(&c->d_Ps, (size_t)c->mtot));
   GX_TRY(dalloc(&c->d_Jt, (size_t)c->mtot * GX_JT_COPIES));
   GX_TRY(dalloc(&c->d_Kt, (size_t)c->mtot));
+  // two-density buffers: reallocated at the new size on their next use
+  dfree(&c->d_Ps2);
+  dfree(&c->d_Psj);
+  dfree(&c->d_Kt2);
 
   // Shell pairs inside each system, oriented l_A >= l_B, sorted by class,
   // system, then primitive-pair count.
@@ -1426,6 +1456,55 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
     }
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, K);
+  GX_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int gx_jk_uhf(gx_ctx* c, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb, double cutoff,
+              int flags) {
+  if (!c || !Pa || !Pb || !J || !Ka || !Kb) return fail(GX_ERR_ARG, "gx_jk_uhf: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_jk_uhf: no basis set");
+  if (!(cutoff >= 0.0)) return fail(GX_ERR_ARG, "gx_jk_uhf: negative threshold");
+  if (flags & GX_JK_MIXED) return fail(GX_ERR_ARG, "gx_jk_uhf: two-density builds are FP64 only");
+  GX_HIP(hipSetDevice(c->device));
+  const int N = c->nmax;
+  const long nn = c->mtot;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      if (c->cls_start[X + 1] == c->cls_start[X] || c->cls_start[Y + 1] == c->cls_start[Y]) continue;
+      const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), N, 0, 2);
+      if (lds > (size_t)c->max_lds)
+        return fail(GX_ERR_ARG, "gx_jk_uhf: basis too large for the LDS row cache (nbf=" + std::to_string(N) +
+                                    ", needs " + std::to_string(lds) + " bytes)");
+    }
+  if (!c->d_Ps2) {
+    GX_TRY(dalloc(&c->d_Ps2, (size_t)nn));
+    GX_TRY(dalloc(&c->d_Psj, (size_t)nn));
+    GX_TRY(dalloc(&c->d_Kt2, (size_t)nn));
+  }
+  const unsigned g = (unsigned)((nn + 255) / 256);
+  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, Pa, c->d_Ps);
+  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, Pb, c->d_Ps2);
+  hipLaunchKernelGGL(gx::k_add, dim3(g), dim3(256), 0, c->stream, nn, c->d_Ps, c->d_Ps2, c->d_Psj);
+  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((nn * GX_JT_COPIES + 255) / 256)), dim3(256), 0, c->stream,
+                     nn * GX_JT_COPIES, c->d_Jt, 0.0);
+  // Jt holds GX_JT_COPIES copies; Kt and Kt2 one each
+  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
+  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt2, 0.0);
+  GX_HIP(hipGetLastError());
+  GX_TRY(build_orders(c, cutoff, 0.0, 0));
+  int combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      if (c->jk.n_items[combo] == 0) continue;
+      GX_HIP(launch_combo(c, X, Y, combo, 0, 2));
+    }
+  hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, Ka);
+  hipLaunchKernelGGL(gx::k_symmetrize1, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, c->d_Kt2, Kb);
   GX_HIP(hipGetLastError());
   return GX_OK;
 }

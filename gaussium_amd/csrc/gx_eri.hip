@@ -370,9 +370,12 @@ struct JKArgs {
   const double* master;  // Boys master table
   const double* tab64;   // per-L FP64 Taylor tables (BOYS_NT x BOYS_ROW64 each)
   const float* tab32;    // per-L FP32 Taylor tables
-  const double* P;       // packed densities, pre-scaled by s_k s_l
+  const double* P;       // packed densities, pre-scaled by s_k s_l (the K density)
+  const double* P2;      // second K density (two-density builds), pre-scaled
+  const double* Pj;      // the J density (P, or P + P2 in two-density builds), pre-scaled
   double* Jt;            // packed accumulators: GX_JT_COPIES copies of jstride doubles for Jt
   double* Kt;
+  double* Kt2;           // K accumulator of the second density (single copy)
   long jstride;
   int mixed;
 };
@@ -390,7 +393,9 @@ __device__ __forceinline__ double* jt_copy(const JKArgs& a) {
 // FP32; the FP32 table is staged in LDS and the FP64 one (for the few FP64
 // quartets) is read through the vector-memory path (L1), which keeps the LDS
 // small and the occupancy up (staging both measured no faster).
-template <int LA, int LB, int LC, int LD, bool MIXED>
+// NDEN = 2 (FP64 only): one evaluation of the integrals serves J of
+// P + P2 and K of P and of P2 (unrestricted Fock builds).
+template <int LA, int LB, int LC, int LD, bool MIXED, int NDEN>
 __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
                                                 const PrimPair32* __restrict__ pp32) {
   using C = EriClass<LA, LB, LC, LD>;
@@ -429,7 +434,11 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
     const double* __restrict__ Pm = a.P + a.sy.mb[sys];
     double* __restrict__ Jm = jt_copy(a) + a.sy.mb[sys];
     double* __restrict__ Km = a.Kt + a.sy.mb[sys];
-    double* krow = prow + (NA + NB) * N;
+    double* __restrict__ Km2 = (NDEN == 2 ? a.Kt2 : a.Kt) + a.sy.mb[sys];
+    const double* __restrict__ Pm2 = (NDEN == 2 ? a.P2 : a.P) + a.sy.mb[sys];
+    const double* __restrict__ Pjm = a.Pj + a.sy.mb[sys];
+    const int nrk = (NA + NB) * N;  // one density's rows
+    double* krow = prow + NDEN * nrk;
     const int s0 = a.cseg[sys];
     const int nseg = a.cseg[sys + 1] - s0;
     const int nr = 2 * nseg;  // range r < nseg: FP64 part of segment r; else FP32 part of r - nseg
@@ -443,11 +452,15 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
     const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
                  ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
     __syncthreads();  // the previous item's flush is complete
-    for (int i = tid; i < (NA + NB) * N; i += JK_TEAM) {
+    for (int i = tid; i < nrk; i += JK_TEAM) {
       const int r = i / N, c = i - r * N;
       const int g = r < NA ? ao + r : bo + (r - NA);
       prow[i] = Pm[(long)g * N + c];
       krow[i] = 0.0;
+      if (NDEN == 2) {
+        prow[nrk + i] = Pm2[(long)g * N + c];
+        krow[nrk + i] = 0.0;
+      }
     }
     if (tid < NA * NB) jab_s[tid] = 0.0;
     if (tid == 0) next_chunk = nwave;
@@ -532,40 +545,24 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
 #pragma unroll
         for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pm[(long)(co + ic) * N + dof + id];
-        double jcd[NC * ND], kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
+          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pjm[(long)(co + ic) * N + dof + id];
+        double jcd[NC * ND];
 #pragma unroll
         for (int i = 0; i < NC * ND; ++i) jcd[i] = 0.0;
-#pragma unroll
-        for (int i = 0; i < NA * NC; ++i) kac[i] = 0.0;
-#pragma unroll
-        for (int i = 0; i < NA * ND; ++i) kad[i] = 0.0;
-#pragma unroll
-        for (int i = 0; i < NB * NC; ++i) kbc[i] = 0.0;
-#pragma unroll
-        for (int i = 0; i < NB * ND; ++i) kbd[i] = 0.0;
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
           for (int ib = 0; ib < NB; ++ib) {
-            const double pab = prow[ia * N + bo + ib];
+            double pab = prow[ia * N + bo + ib];
+            if (NDEN == 2) pab += prow[nrk + ia * N + bo + ib];
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic) {
-              const double pac = prow[ia * N + co + ic];
-              const double pbc = prow[(rowB + ib) * N + co + ic];
+            for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
               for (int id = 0; id < ND; ++id) {
                 const double x = val[((ia * NB + ib) * NC + ic) * ND + id];
-                const double pad = prow[ia * N + dof + id];
-                const double pbd = prow[(rowB + ib) * N + dof + id];
                 jab[ia * NB + ib] = fma(x, Pcd[ic * ND + id], jab[ia * NB + ib]);
                 jcd[ic * ND + id] = fma(x, pab, jcd[ic * ND + id]);
-                kac[ia * NC + ic] = fma(x, pbd, kac[ia * NC + ic]);
-                kad[ia * ND + id] = fma(x, pbc, kad[ia * ND + id]);
-                kbc[ib * NC + ic] = fma(x, pad, kbc[ib * NC + ic]);
-                kbd[ib * ND + id] = fma(x, pac, kbd[ib * ND + id]);
               }
-            }
           }
 #pragma unroll
         for (int ic = 0; ic < NC; ++ic)
@@ -573,18 +570,51 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
           for (int id = 0; id < ND; ++id)
             atomicAdd(&Jm[(long)(co + ic) * N + dof + id], 2.0 * jcd[ic * ND + id]);
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia) {
+        for (int d = 0; d < NDEN; ++d) {
+          const double* __restrict__ pr = prow + d * nrk;
+          double* kr = krow + d * nrk;
+          double kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
 #pragma unroll
-          for (int ic = 0; ic < NC; ++ic) atomicAdd(&krow[ia * N + co + ic], kac[ia * NC + ic]);
+          for (int i = 0; i < NA * NC; ++i) kac[i] = 0.0;
 #pragma unroll
-          for (int id = 0; id < ND; ++id) atomicAdd(&krow[ia * N + dof + id], kad[ia * ND + id]);
-        }
+          for (int i = 0; i < NA * ND; ++i) kad[i] = 0.0;
 #pragma unroll
-        for (int ib = 0; ib < NB; ++ib) {
+          for (int i = 0; i < NB * NC; ++i) kbc[i] = 0.0;
 #pragma unroll
-          for (int ic = 0; ic < NC; ++ic) atomicAdd(&krow[(rowB + ib) * N + co + ic], kbc[ib * NC + ic]);
+          for (int i = 0; i < NB * ND; ++i) kbd[i] = 0.0;
 #pragma unroll
-          for (int id = 0; id < ND; ++id) atomicAdd(&krow[(rowB + ib) * N + dof + id], kbd[ib * ND + id]);
+          for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+              for (int ic = 0; ic < NC; ++ic) {
+                const double pac = pr[ia * N + co + ic];
+                const double pbc = pr[(rowB + ib) * N + co + ic];
+#pragma unroll
+                for (int id = 0; id < ND; ++id) {
+                  const double x = val[((ia * NB + ib) * NC + ic) * ND + id];
+                  const double pad = pr[ia * N + dof + id];
+                  const double pbd = pr[(rowB + ib) * N + dof + id];
+                  kac[ia * NC + ic] = fma(x, pbd, kac[ia * NC + ic]);
+                  kad[ia * ND + id] = fma(x, pbc, kad[ia * ND + id]);
+                  kbc[ib * NC + ic] = fma(x, pad, kbc[ib * NC + ic]);
+                  kbd[ib * ND + id] = fma(x, pac, kbd[ib * ND + id]);
+                }
+              }
+#pragma unroll
+          for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) atomicAdd(&kr[ia * N + co + ic], kac[ia * NC + ic]);
+#pragma unroll
+            for (int id = 0; id < ND; ++id) atomicAdd(&kr[ia * N + dof + id], kad[ia * ND + id]);
+          }
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) atomicAdd(&kr[(rowB + ib) * N + co + ic], kbc[ib * NC + ic]);
+#pragma unroll
+            for (int id = 0; id < ND; ++id) atomicAdd(&kr[(rowB + ib) * N + dof + id], kbd[ib * ND + id]);
+          }
         }
       }
 #if GX_DYN_CHUNK
@@ -612,11 +642,13 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
     }
     const int nrow = (A == B) ? NA : NA + NB;
     for (int i = tid; i < nrow * N; i += JK_TEAM) {
+      const int r = i / N, c = i - r * N;
+      const int g = r < NA ? ao + r : bo + (r - NA);
       const double x = krow[i];
-      if (x != 0.0) {
-        const int r = i / N, c = i - r * N;
-        const int g = r < NA ? ao + r : bo + (r - NA);
-        atomicAdd(&Km[(long)g * N + c], x);
+      if (x != 0.0) atomicAdd(&Km[(long)g * N + c], x);
+      if (NDEN == 2) {
+        const double y = krow[nrk + i];
+        if (y != 0.0) atomicAdd(&Km2[(long)g * N + c], y);
       }
     }
   }
@@ -656,6 +688,18 @@ __global__ void k_symmetrize(long total, int nsys, const long* __restrict__ mb, 
   K[t] = f * (Kt[t] + Kt[tt]);
 }
 
+// K = s (Kt + Kt^T) s per system from a single-copy accumulator.
+__global__ void k_symmetrize1(long total, int nsys, const long* __restrict__ mb, const int* __restrict__ n,
+                              const int* __restrict__ bfb, const double* __restrict__ scale,
+                              const double* __restrict__ Kt, double* __restrict__ K) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int s, i, j;
+  packed_index(t, nsys, mb, n, s, i, j);
+  const long tt = mb[s] + (long)j * n[s] + i;
+  K[t] = scale[bfb[s] + i] * scale[bfb[s] + j] * (Kt[t] + Kt[tt]);
+}
+
 // P' = s P s per system.
 __global__ void k_scale_density(long total, int nsys, const long* __restrict__ mb, const int* __restrict__ n,
                                 const int* __restrict__ bfb, const double* __restrict__ scale,
@@ -665,6 +709,12 @@ __global__ void k_scale_density(long total, int nsys, const long* __restrict__ m
   int s, i, j;
   packed_index(t, nsys, mb, n, s, i, j);
   Ps[t] = P[t] * scale[bfb[s] + i] * scale[bfb[s] + j];
+}
+
+// z = x + y, elementwise.
+__global__ void k_add(long n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ z) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) z[t] = x[t] + y[t];
 }
 
 // Per-L Taylor tables in HBM, the same layout the kernels stage into LDS.

@@ -209,6 +209,18 @@ class Engine:
                    'gx_jk')
         return J, K
 
+    def jk_uhf(self, Pa, Pb, cutoff=0.0):
+        """Unrestricted direct build (gx_jk_uhf, FP64): (J of Pa + Pb, K of Pa,
+        K of Pb) from one evaluation of the integrals."""
+        Pa, Pb = self._dev(Pa), self._dev(Pb)
+        shape = self._matrix_shape(Pa, 'alpha density')
+        self._matrix_shape(Pb, 'beta density')
+        J, Ka, Kb = self._empty(*shape), self._empty(*shape), self._empty(*shape)
+        self._bind_stream()
+        _lib.check(self.lib.gx_jk_uhf(self.ctx, _p(Pa), _p(Pb), _p(J), _p(Ka), _p(Kb), float(cutoff), 0),
+                   'gx_jk_uhf')
+        return J, Ka, Kb
+
     def autotune(self, P, cutoff=0.0, fp32_bound=0.0, mixed=False,
                  candidates=(1.0, 2.0, 4.0, 8.0, 16.0, 32.0, 64.0), reps=3):
         """Pick the work-item size of each class combination by timing (see

@@ -65,8 +65,10 @@ class FockMatrixRestricted:
 
 class FockMatrixUnrestricted:
     """F_a = H + J(P_a + P_b) - K(P_a), F_b = H + J(P_a + P_b) - K(P_b)
-    (hartreefock/fock_matrix.py:25-55): one direct J/K build per spin density
-    (J is linear, so J(P_a) + J(P_b) is the Coulomb matrix of the total)."""
+    (hartreefock/fock_matrix.py:25-55).  FP64: one two-density build
+    (``gx_jk_uhf``: the integrals are evaluated once for both spins); mixed
+    precision: one build per spin density (J is linear, so J(P_a) + J(P_b)
+    is the Coulomb matrix of the total)."""
 
     def __init__(self, core_hamiltonian, engine, schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64'):
         self.core_hamiltonian = core_hamiltonian
@@ -77,11 +79,14 @@ class FockMatrixUnrestricted:
         self.builds = 0
 
     def create(self, density_matrix_alph, density_matrix_beta):
-        kw = dict(cutoff=self.schwarz_cutoff, fp32_bound=self.fp32_bound, mixed=self.mixed)
-        ja, ka = self.engine.jk(density_matrix_alph, **kw)
-        jb, kb = self.engine.jk(density_matrix_beta, **kw)
-        self.builds += 2
-        j = ja + jb
+        if self.mixed:
+            kw = dict(cutoff=self.schwarz_cutoff, fp32_bound=self.fp32_bound, mixed=True)
+            ja, ka = self.engine.jk(density_matrix_alph, **kw)
+            jb, kb = self.engine.jk(density_matrix_beta, **kw)
+            j = ja + jb
+        else:
+            j, ka, kb = self.engine.jk_uhf(density_matrix_alph, density_matrix_beta, cutoff=self.schwarz_cutoff)
+        self.builds += 1
         return self.core_hamiltonian + j - ka, self.core_hamiltonian + j - kb
 
 

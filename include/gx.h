@@ -154,6 +154,15 @@ int gx_eri_dense(gx_ctx* ctx, double* out_dev);
 int gx_jk(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double cutoff,
           double fp32_bound, int flags);
 
+/* Unrestricted (two-density) direct build, FP64: J of P_a + P_b, K of P_a
+ * and K of P_b from one evaluation of the integrals (the cooperative d-shell
+ * classes evaluate theirs once per density), same thresholds and work items
+ * as gx_jk; all nbf x nbf device buffers.  Replaces the Coulomb and exchange
+ * sums of the reference's unrestricted Fock builder
+ * (hartreefock/fock_matrix.py:25-55): F_a = H + J - K_a, F_b = H + J - K_b. */
+int gx_jk_uhf(gx_ctx* ctx, const double* Pa_dev, const double* Pb_dev, double* J_dev, double* Ka_dev,
+              double* Kb_dev, double cutoff, int flags);
+
 /* Autotuner of the J/K work-item size ("tile") per class combination: for
  * each combination, the kernel is timed (best of `reps` launches, HIP
  * events on the context stream) with the items sized for each of the

@@ -69,3 +69,64 @@ def test_grad_autotune_keeps_result():
     assert tuned and all(v[0] in (1.0, 8.0, 64.0) for v in tuned.values())
     g1 = eng.grad(P, P, cutoff=1e-11)
     assert (g1 - g0).abs().max().item() < 1e-10
+
+
+@pytest.mark.parametrize('system', ['cluster_66bf', 'd_basis'])
+def test_jk_uhf_matches_two_single_density_builds(system):
+    """gx_jk_uhf (one evaluation of the integrals for both spins) against two
+    gx_jk builds (already checked against the oracle): J of Pa + Pb, K of Pa
+    and of Pb; the d-shell basis exercises the cooperative classes."""
+    from gaussium_amd import workloads
+    from gaussium_amd.basis import Nuclei
+    from gaussium_amd.engine import Engine
+    if system == 'd_basis':
+        from tests.test_gpu_dshell import _d_basis
+        basis = _d_basis()
+        nuc = [Nuclei('A', 3.0, 6.0, (0.0, 0.0, 0.0)), Nuclei('B', 2.0, 4.0, (1.3, -0.4, 0.6)),
+               Nuclei('C', 1.0, 1.0, (-0.5, 1.1, -0.9))]
+    else:
+        nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    n = len(basis)
+    rng = np.random.default_rng(17)
+    ca = rng.standard_normal((n, n // 3))
+    cb = rng.standard_normal((n, n // 4))
+    Pa = torch.from_numpy(ca @ ca.T / n).cuda()
+    Pb = torch.from_numpy(cb @ cb.T / n).cuda()
+    for cutoff in (0.0, 1e-11):
+        J, Ka, Kb = eng.jk_uhf(Pa, Pb, cutoff=cutoff)
+        Ja, Ka1 = eng.jk(Pa, cutoff=cutoff)
+        Jb, Kb1 = eng.jk(Pb, cutoff=cutoff)
+        assert (J - (Ja + Jb)).abs().max().item() < 1e-11
+        assert (Ka - Ka1).abs().max().item() < 1e-11
+        assert (Kb - Kb1).abs().max().item() < 1e-11
+
+
+def test_jk_uhf_batch_after_smaller_basis():
+    """Two-density build over a batch of molecules on a context that first ran
+    one on a smaller basis (the two-density buffers follow the new size)."""
+    from gaussium_amd import workloads
+    from gaussium_amd.engine import Engine
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    n = len(basis)
+    rng = np.random.default_rng(23)
+    a = rng.standard_normal((n, n))
+    P = torch.from_numpy(a @ a.T / n).cuda()
+    eng.jk_uhf(P, P, cutoff=1e-11)
+    systems = [(b, nu) for nu, b, _, _ in workloads.batch(n_geom=5, seed=9)]
+    eng.set_systems(systems)
+    assert eng.packed_size > n * n
+    mats = []
+    for b, _ in systems:
+        m = len(b)
+        c = rng.standard_normal((m, m // 3 + 1))
+        mats.append(c @ c.T / m)
+    Pa = eng.pack(mats)
+    Pb = eng.pack([0.5 * x + 0.1 * np.eye(len(x)) for x in mats])
+    J, Ka, Kb = eng.jk_uhf(Pa, Pb, cutoff=1e-11)
+    Ja, Ka1 = eng.jk(Pa, cutoff=1e-11)
+    Jb, Kb1 = eng.jk(Pb, cutoff=1e-11)
+    assert (J - (Ja + Jb)).abs().max().item() < 1e-11
+    assert (Ka - Ka1).abs().max().item() < 1e-11
+    assert (Kb - Kb1).abs().max().item() < 1e-11
