@@ -81,6 +81,17 @@ def kernel_name(combo, mixed, fp64_combos=()):
     return 'k_jk<%d, %d, %d, %d, %s, 1>' % (la, lb, lc, ld, 'true' if m else 'false')
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (None when the summary does not cover it)."""
+    path = os.path.join(ROOT, 'profiles', 'jk_220bf_traffic.json')
+    try:
+        with open(path) as f:
+            return json.load(f)['hbm_bytes_per_launch'].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def class_rows(eng, mixed):
     return eng.jk_class_counts(cutoff=CUTOFF, fp32_bound=FP32_BOUND, mixed=mixed)
 
@@ -412,8 +423,13 @@ def main():
                      'evaluated_bf_mixed_build': [stmx['bf_quartets_fp64'], stmx['bf_quartets_fp32']],
                      'evaluated_bf_per_s': (stmx['bf_quartets_fp64'] + stmx['bf_quartets_fp32']) / t
                      if ok else st64['bf_quartets_fp64'] / t},
-        'roofline': {'bound': 'valu', 'achieved': k_achieved, 'peak': k_peak, 'unit': 'TFLOP/s',
-                     'frac': k_achieved / k_peak, 'traffic': None,
+        'roofline': {'bound': 'mfma', 'achieved': k_achieved, 'peak': k_peak, 'unit': 'TFLOP/s',
+                     'frac': k_achieved / k_peak, 'traffic': measured_traffic(kernel_name(combo, ok, fp64_combos)),
+                     'compute_roof': 'FP64 arithmetic, issued on the vector ALUs: MI355X peaks at 78.6 TFLOP/s '
+                                     'FP64 on its vector and its matrix units alike (FP32 parts of a mixed '
+                                     'build priced at 157.3)',
+                     'traffic_source': 'profiles/jk_220bf_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE per '
+                                       'launch, scaled by a 256 MiB copy)',
                      'kernel': kernel_name(combo, ok, fp64_combos), 'kernel_ms': head['kernel_ms'][combo],
                      'kernel_share': head['kernel_ms'][combo] / head['median_ms'],
                      'flops_per_launch': [k64, k32],
