@@ -9,6 +9,11 @@
   computing G = J - K/2 by the direct fused J/K kernel; no N^4 tensor.
 * ``RestrictedHF`` -- hartreefock/hartree_fock.py:18-84: S, T, V on the
   device, canonical orthogonalisation, core-H guess, DIIS SCF.
+* ``FockMatrixUnrestricted`` / ``UnrestrictedHF`` -- fock_matrix.py:25-55,
+  hartree_fock.py:108-116 (Pople-Nesbet-Berthier).
+* ``BlockedFockMatrixUnrestricted`` / ``BlockedHartreeFock`` --
+  fock_matrix.py:56-81, hartree_fock.py:119-139 ('GUHF': the unrestricted SCF
+  on the 2N spin-blocked basis with DIIS).
 
 Options the reference does not have (``schwarz_cutoff``, ``fp32_bound``,
 ``precision``, ``max_iter``, ``threshold``) default to the reference's
@@ -20,7 +25,7 @@ import numpy as np
 import torch
 
 from .engine import Engine
-from .scf import LinearAlgebra, PopleNesbetBerthier, RestrictedSCF
+from .scf import BlockedLinearAlgebra, BlockedUnrestrictedSCF, LinearAlgebra, PopleNesbetBerthier, RestrictedSCF
 
 
 class TwoElectronRepulsionMatrixOS:
@@ -88,6 +93,47 @@ class FockMatrixUnrestricted:
             j, ka, kb = self.engine.jk_uhf(density_matrix_alph, density_matrix_beta, cutoff=self.schwarz_cutoff)
         self.builds += 1
         return self.core_hamiltonian + j - ka, self.core_hamiltonian + j - kb
+
+
+class BlockedFockMatrixUnrestricted:
+    """F = H + J - K on the spin-blocked 2N basis (hartreefock/fock_matrix.py:56-81,
+    over the blocked tensor of matrixelements/transformations.py:29-71).
+
+    With spin blocks P_aa, P_bb of the blocked density the diagonal blocks are
+    H + J(P_aa + P_bb) - K(P_aa) and H + J(P_aa + P_bb) - K(P_bb): one
+    two-density direct build (``gx_jk_uhf``) in FP64, two single-density
+    builds in mixed precision; no 2N tensor is formed.  The spin-off-diagonal
+    blocks of F are -K of the transposed off-diagonal density blocks; the
+    blocked SCF keeps those density blocks exactly zero (its orbitals are
+    spin-pure), and a density with nonzero ones is rejected."""
+
+    def __init__(self, core_hamiltonian, engine, schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64'):
+        self.core_hamiltonian = core_hamiltonian  # blockdiag(H, H)
+        self.engine = engine
+        self.schwarz_cutoff = schwarz_cutoff
+        self.fp32_bound = fp32_bound
+        self.mixed = precision == 'mixed'
+        self.builds = 0
+
+    def create(self, density_matrix):
+        n = density_matrix.shape[0] // 2
+        if bool(torch.any(density_matrix[:n, n:] != 0)) or bool(torch.any(density_matrix[n:, :n] != 0)):
+            raise ValueError('blocked Fock build: the density has alpha-beta blocks; only spin-blocked '
+                             'densities (spin-pure orbitals) are supported')
+        pa = density_matrix[:n, :n].contiguous()
+        pb = density_matrix[n:, n:].contiguous()
+        if self.mixed:
+            kw = dict(cutoff=self.schwarz_cutoff, fp32_bound=self.fp32_bound, mixed=True)
+            ja, ka = self.engine.jk(pa, **kw)
+            jb, kb = self.engine.jk(pb, **kw)
+            j = ja + jb
+        else:
+            j, ka, kb = self.engine.jk_uhf(pa, pb, cutoff=self.schwarz_cutoff)
+        self.builds += 1
+        fock = self.core_hamiltonian.clone()
+        fock[:n, :n] += j - ka
+        fock[n:, n:] += j - kb
+        return fock
 
 
 class RestrictedHF:
@@ -180,6 +226,44 @@ class UnrestrictedHF:
                                                    fp32_bound, precision)
         self.scf_method = PopleNesbetBerthier(self.linear_algebra, electrons, multiplicity, self.fock_builder,
                                               threshold=threshold, max_iter=max_iter)
+
+    def initial_guess(self):
+        return self.linear_algebra.diagonalize(self.core_hamiltonian)
+
+    def begin_scf(self):
+        e0, c0 = self.initial_guess()
+        return self.scf_method.begin_iterations(e0, c0)
+
+    def energies(self):
+        return self.begin_scf()[0], 0.0
+
+
+class BlockedHartreeFock:
+    """'GUHF' (hartree_fock.py:119-139): the unrestricted SCF on the spin-blocked
+    basis, S and H as blockdiag(S, S) and blockdiag(H, H), blocked canonical
+    orthogonalisation, DIIS (scf_procedure.py:92-120), on one GPU."""
+
+    def __init__(self, nuclei_array, basis_set_array, electrons, multiplicity, symmetry=None, processes=1,
+                 schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64', max_iter=None, engine=None,
+                 threshold=1e-12):
+        self.nuclei_array = nuclei_array
+        self.basis_set_array = basis_set_array
+        self.electrons = electrons
+        self.multiplicity = multiplicity
+        self.symmetry = symmetry
+        self.engine = engine if engine is not None else Engine(basis_set_array, nuclei_array)
+        if engine is not None and basis_set_array is not None:
+            self.engine.set_basis(basis_set_array, nuclei_array)
+            self.engine.set_nuclei(nuclei_array)
+        S, T, V = self.engine.one_electron()
+        self.orbital_overlap = torch.block_diag(S, S)
+        self.core_hamiltonian = torch.block_diag(T + V, T + V)
+        self.linear_algebra = BlockedLinearAlgebra(self.orbital_overlap)
+        self.fock_builder = BlockedFockMatrixUnrestricted(self.core_hamiltonian, self.engine, schwarz_cutoff,
+                                                          fp32_bound, precision)
+        self.scf_method = BlockedUnrestrictedSCF(self.linear_algebra, electrons, multiplicity,
+                                                 self.orbital_overlap, self.fock_builder, threshold=threshold,
+                                                 max_iter=max_iter)
 
     def initial_guess(self):
         return self.linear_algebra.diagonalize(self.core_hamiltonian)

@@ -1,8 +1,8 @@
 """``start()`` and ``Energy`` with the reference's signatures.
 
 main.py:11-34 and energy/energy.py:17-81 of the reference.  The 'RHF',
-'UHF' and 'MP2' methods run on this engine (the other branches of
-``Energy.calculate_energy`` -- GUHF, DFT, CCSD, TDHF, CIS -- are outside the
+'UHF', 'GUHF' and 'MP2' methods run on this engine (the other branches of
+``Energy.calculate_energy`` -- DFT, CCSD, TDHF, CIS -- are outside the
 ERI + J/K path and raise ``NotImplementedError``).
 ``symmetry=True`` is accepted; point-group quartet skipping is not
 performed (the result is the same energy, as in the reference).
@@ -10,7 +10,7 @@ performed (the result is the same energy, as in the reference).
 import time
 
 from .basis import nuclear_repulsion, read_basis_set_file, read_mol_file
-from .hartree_fock import RestrictedHF, UnrestrictedHF
+from .hartree_fock import BlockedHartreeFock, RestrictedHF, UnrestrictedHF
 
 
 class Energy:
@@ -31,12 +31,16 @@ class Energy:
         elif self.method == 'UHF':
             calc = UnrestrictedHF(nuclei_array, basis_set, self.electrons, self.multiplicity, self.symmetry_object,
                                   self.processors, **self.options)
+        elif self.method == 'GUHF':
+            calc = BlockedHartreeFock(nuclei_array, basis_set, self.electrons, self.multiplicity,
+                                      self.symmetry_object, self.processors, **self.options)
         elif self.method == 'MP2':
             from .mp2 import MoellerPlesset
             calc = MoellerPlesset(RestrictedHF(nuclei_array, basis_set, self.electrons, self.symmetry_object,
                                                self.processors, **self.options))
         else:
-            raise NotImplementedError('method %r is not provided by gaussium_amd (RHF, UHF, MP2)' % (self.method,))
+            raise NotImplementedError('method %r is not provided by gaussium_amd (RHF, UHF, GUHF, MP2)'
+                                      % (self.method,))
         electron_energy, correlation = calc.energies()
         self.last = calc
         return electron_energy + e_nuc + correlation

@@ -17,6 +17,10 @@ Mirrors the reference's classes and their numerical conventions:
 * ``RestrictedSCF``              core-H guess, stop at |dE| < 1e-12
                                  (hartreefock/scf_procedure.py:17-42); ``max_iter``
                                  adds the iteration cap the reference lacks.
+* ``PopleNesbetBerthier``        unrestricted SCF (scf_procedure.py:45-89)
+* ``BlockedUnrestrictedSCF``     unrestricted SCF on the spin-blocked 2N basis with
+                                 ``BlockedLinearAlgebra`` and DIIS (scf_procedure.py:92-120,
+                                 linear_algebra.py:36-98)
 """
 import numpy as np
 import torch
@@ -33,6 +37,15 @@ def density_matrix_unrestricted(c, electrons):
     """P = C_occ C_occ^T for one spin (matrixelements/density_matrix.py:12-17)."""
     occ = c[:, :electrons]
     return occ @ occ.T
+
+
+def blocked_density_matrix(c, electrons_alph, electrons_beta):
+    """Spin-blocked P (matrixelements/density_matrix.py:20-36): alpha columns
+    [0, n_a) and beta columns [N, N + n_b) of the 2N x 2N coefficients."""
+    half = c.shape[0] // 2
+    a = c[:, :electrons_alph]
+    b = c[:, half:half + electrons_beta]
+    return a @ a.T + b @ b.T
 
 
 class TotalEnergy:
@@ -65,6 +78,40 @@ class LinearAlgebra:
     def orthonormalize(self, matrix):
         x = self.transformation_matrix
         return x.T @ matrix @ x
+
+
+class BlockedLinearAlgebra(LinearAlgebra):
+    """Spin-blocked linear algebra (hartreefock/linear_algebra.py:36-98): X
+    from each spin block of the blocked overlap on its own; X^T F X
+    diagonalised per spin block (ascending each), C = X blockdiag(C_a, C_b),
+    energies alpha then beta.  The orbitals stay spin-pure."""
+
+    def __init__(self, orbital_overlap):
+        n = orbital_overlap.shape[0] // 2
+        self.half = n
+        x = torch.zeros_like(orbital_overlap)
+        smallest = []
+        for b in (0, 1):
+            w, u = torch.linalg.eigh(orbital_overlap[b * n:(b + 1) * n, b * n:(b + 1) * n])
+            order = torch.argsort(w)
+            w, u = w[order], u[:, order]
+            smallest.append(float(w[0]))
+            x[b * n:(b + 1) * n, b * n:(b + 1) * n] = u * w.rsqrt()
+        self.smallest_overlap_eigenvalue = min(smallest)
+        self.transformation_matrix = x
+
+    def diagonalize(self, fock_matrix):
+        n = self.half
+        x = self.transformation_matrix
+        fo = x.T @ fock_matrix @ x
+        c = torch.zeros_like(fock_matrix)
+        es = []
+        for b in (0, 1):
+            e, v = torch.linalg.eigh(fo[b * n:(b + 1) * n, b * n:(b + 1) * n])
+            order = torch.argsort(e)
+            es.append(e[order])
+            c[b * n:(b + 1) * n, b * n:(b + 1) * n] = v[:, order]
+        return torch.cat(es), x @ c
 
 
 class DIIS:
@@ -218,3 +265,47 @@ class PopleNesbetBerthier:
             eb, cb = self.linear_algebra.diagonalize(fb)
         self.density_alpha, self.density_beta = pa, pb
         return total, ea, eb, ca, cb
+
+
+class BlockedUnrestrictedSCF:
+    """Unrestricted SCF on the spin-blocked basis (hartreefock/scf_procedure.py:92-120):
+    n_a = (n + m - 1) / 2, n_b = (n - m + 1) / 2; the first density takes every
+    alpha column of the guess and no beta one, as the reference does; DIIS on
+    the 2N matrices; E = 1/2 sum P (H + F); stop at |dE| < threshold (plus the
+    optional iteration cap)."""
+
+    def __init__(self, linear_algebra, electrons, multiplicity, overlap, hamiltonian_matrix_factory,
+                 threshold=1e-12, max_iter=None):
+        self.linear_algebra = linear_algebra
+        self.electrons = electrons
+        self.hamiltonian_matrix_factory = hamiltonian_matrix_factory
+        self.threshold = threshold
+        self.max_iter = max_iter
+        self.calculate = TotalEnergy(hamiltonian_matrix_factory.core_hamiltonian)
+        self.electrons_alph = (electrons + multiplicity - 1) // 2
+        self.electrons_beta = (electrons - multiplicity + 1) // 2
+        self.diis = DIIS(overlap, linear_algebra)
+        self.iterations = 0
+        self.converged = False
+
+    def begin_iterations(self, orbital_energies, orbital_coefficients):
+        half = orbital_coefficients.shape[0] // 2
+        total = previous = 0.0
+        while True:
+            if self.iterations == 0:
+                density = blocked_density_matrix(orbital_coefficients, half, 0)
+            else:
+                density = blocked_density_matrix(orbital_coefficients, self.electrons_alph, self.electrons_beta)
+            fock = self.hamiltonian_matrix_factory.create(density)
+            total = self.calculate.restricted(density, fock)
+            delta = previous - total
+            previous = total
+            self.iterations += 1
+            self.converged = abs(delta) < self.threshold
+            if self.converged or (self.max_iter is not None and self.iterations >= self.max_iter):
+                break
+            fock = self.diis.fock_matrix(fock, density)
+            orbital_energies, orbital_coefficients = self.linear_algebra.diagonalize(fock)
+        self.density = density
+        self.fock = fock
+        return total, orbital_energies, orbital_coefficients

@@ -25,6 +25,10 @@ Pinning (what checks each part, besides restating the formula):
                  C2H4/3-21G, mock HeH+ -4.227529), reference total-energy
                  fixture, density fixture, E_nuc fixtures.
   uhf         -- the reference's UHF energy of O2/STO-3G (-147.634028141).
+  guhf        -- the reference's GUHF energy of O2/STO-3G (-147.634028141);
+                 the blocked Fock sums against spatial J/K block by block
+                 (off-diagonal spin blocks included); the blocked
+                 diagonalisation against two spatial ones.
   mp2         -- the reference's MP2 energy of CO/STO-3G (-111.354512528);
                  the MO transformation against an einsum contraction.
 """

@@ -193,6 +193,98 @@ def uhf(h, s, electrons, multiplicity, fock_builder, threshold=1e-12, max_iter=N
     return e, it
 
 
+def blocked_spin_basis_set(eri):
+    """transformations.py:29-71: spin-orbital (rs|tu) in the blocked order
+    (N alpha, then N beta functions): (rs|tu) = (r's'|t'u') of the spatial
+    functions when r, s share a spin and t, u share a spin, else 0."""
+    n = eri.shape[0]
+    out = np.zeros((2 * n,) * 4)
+    for sr in (0, 1):
+        for st in (0, 1):
+            out[sr * n:(sr + 1) * n, sr * n:(sr + 1) * n, st * n:(st + 1) * n, st * n:(st + 1) * n] = eri
+    return out
+
+
+def fock_blocked(h2, eri2, p2):
+    """fock_matrix.py:56-81: F = H + J - K on the spin-blocked basis with
+    J_ij = sum_ab P_ab (ij|ab) and K_ij = sum_ab P_ab (ib|aj)."""
+    j = np.einsum('ijab,ab->ij', eri2, p2, optimize=True)
+    k = np.einsum('ibaj,ab->ij', eri2, p2, optimize=True)
+    return h2 + j - k
+
+
+class BlockedLinearAlgebra(LinearAlgebra):
+    """linear_algebra.py:36-98: X from each spin block of the blocked overlap
+    on its own; diagonalise the two spin blocks of X^T F X separately, each
+    in ascending order; C = X blockdiag(C_a, C_b), energies alpha then beta."""
+
+    def __init__(self, s2):
+        n = s2.shape[0] // 2
+        self.n = n
+        self.x = np.zeros_like(s2)
+        for b in (0, 1):
+            w, u = np.linalg.eigh(s2[b * n:(b + 1) * n, b * n:(b + 1) * n])
+            order = np.argsort(w)
+            w, u = w[order], u[:, order]
+            self.x[b * n:(b + 1) * n, b * n:(b + 1) * n] = u @ np.diag(w ** -0.5)
+
+    def diagonalize(self, f):
+        n = self.n
+        fp = self.x.T @ f @ self.x
+        c = np.zeros_like(f)
+        es = []
+        for b in (0, 1):
+            e, v = np.linalg.eigh(fp[b * n:(b + 1) * n, b * n:(b + 1) * n])
+            order = np.argsort(e)
+            es.append(e[order])
+            c[b * n:(b + 1) * n, b * n:(b + 1) * n] = v[:, order]
+        return np.concatenate(es), self.x @ c
+
+
+def blocked_density_matrix(c, electrons_alph, electrons_beta):
+    """density_matrix.py:20-36: sum of c_i c_i^T over the alpha columns
+    [0, n_a) and the beta columns [N, N + n_b) of the 2N x 2N coefficients."""
+    half = c.shape[0] // 2
+    a = c[:, :electrons_alph]
+    b = c[:, half:half + electrons_beta]
+    return a @ a.T + b @ b.T
+
+
+def guhf(h, s, electrons, multiplicity, fock_builder, threshold=1e-12, max_iter=None, diis_cond_limit=None):
+    """Blocked unrestricted SCF (scf_procedure.py:92-120, hartree_fock.py:119-139).
+
+    ``h``, ``s``: spatial N x N; the SCF runs on blockdiag(h, h) and
+    blockdiag(s, s).  ``fock_builder(P2) -> F2`` on the 2N basis.  The first
+    density takes every alpha column of the core-H guess and no beta one, as
+    the reference does; DIIS on the 2N matrices; E = 1/2 sum P (H + F).
+    Returns (E_elec, iterations)."""
+    n = h.shape[0]
+    z = np.zeros((n, n))
+    h2 = np.block([[h, z], [z, h]])
+    s2 = np.block([[s, z], [z, s]])
+    la = BlockedLinearAlgebra(s2)
+    diis = DIIS(s2, la, cond_limit=diis_cond_limit)
+    _, c = la.diagonalize(h2)
+    na, nb = (electrons + multiplicity - 1) // 2, (electrons - multiplicity + 1) // 2
+    e = prev = 0.0
+    it = 0
+    while True:
+        if e == 0:
+            p = blocked_density_matrix(c, n, 0)
+        else:
+            p = blocked_density_matrix(c, na, nb)
+        f = fock_builder(p)
+        e = total_energy_restricted(h2, p, f)
+        de = prev - e
+        prev = e
+        it += 1
+        if abs(de) < threshold or (max_iter is not None and it >= max_iter):
+            break
+        f = diis.fock_matrix(f, p)
+        _, c = la.diagonalize(f)
+    return e, it
+
+
 def molecular_orbitals(eri, c):
     """transformations.py:5-26: (pq|rs) = sum C_mp C_nq C_lr C_ss (mn|ls), one
     N x N slice at a time as the reference does (on a copy)."""

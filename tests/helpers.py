@@ -114,6 +114,30 @@ def oracle_uhf(nuclei, basis, electrons, multiplicity, exact_boys=True):
     return e + nuclear_repulsion(onuc), it
 
 
+def oracle_guhf(nuclei, basis, electrons, multiplicity, exact_boys=True):
+    """Oracle blocked (GUHF) total energy: the dense spin-blocked tensor of
+    the C-oracle ERIs and the reference's blocked Fock sums."""
+    from oracle import boys as oboys
+    from oracle import eri_c
+    from oracle.basis import Nuclei, nuclear_repulsion
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from oracle.scf import blocked_spin_basis_set, fock_blocked, guhf
+    if exact_boys:
+        with eri_c.exact_boys():
+            eri = eri_c.eri_dense(basis)
+    else:
+        eri = eri_c.eri_dense(basis)
+    onuc = [Nuclei(n.element, n.charge, n.mass, n.coordinates) for n in nuclei]
+    S = overlap_matrix(basis)
+    with oboys.exact(exact_boys):
+        H = kinetic_matrix(basis) + nuclear_matrix(basis, onuc)
+    eri2 = blocked_spin_basis_set(eri)
+    z = np.zeros_like(H)
+    H2 = np.block([[H, z], [z, H]])
+    e, it = guhf(H, S, electrons, multiplicity, lambda p2: fock_blocked(H2, eri2, p2))
+    return e + nuclear_repulsion(onuc), it
+
+
 def oracle_mp2(nuclei, basis, electrons, exact_boys=True):
     """Oracle RHF + MP2 total energy (moeller_plesset.py:7-26)."""
     from oracle import boys as oboys

@@ -1,5 +1,5 @@
-"""UHF and MP2 on the device against the reference's published energies and
-the oracle.
+"""UHF, GUHF and MP2 on the device against the reference's published
+energies and the oracle.
 
 * UHF O2/STO-3G (triplet): -147.634028141 (tests/testslow/start/test_start.py:33-37)
   to the reference's 6 significant digits, and the oracle's UHF (exact Boys,
@@ -24,6 +24,21 @@ def test_uhf_o2_published_and_oracle():
     nuc, ne, mult = read_mol_file(golden('O2.mol'))
     basis = read_basis_set_file(golden('STO-3G.gbs'), nuc)
     e_ref, _ = oracle_uhf(nuc, basis, ne, mult)
+    assert abs(e - e_ref) < 1e-8, (e, e_ref)
+
+
+def test_guhf_o2_published_and_oracle():
+    """GUHF O2/STO-3G: -147.634028141 (tests/testslow/start/test_start.py:39-43)
+    to the reference's 6 significant digits, and the oracle's blocked SCF
+    (dense spin-blocked tensor, same DIIS) within 1e-8 Eh."""
+    from gaussium_amd import start
+    from gaussium_amd.basis import read_basis_set_file, read_mol_file
+    from tests.helpers import golden, oracle_guhf
+    e = start(golden('O2.mol'), golden('STO-3G.gbs'), 'GUHF', 1, True)
+    np.testing.assert_approx_equal(e, -147.634028141, 6)
+    nuc, ne, mult = read_mol_file(golden('O2.mol'))
+    basis = read_basis_set_file(golden('STO-3G.gbs'), nuc)
+    e_ref, _ = oracle_guhf(nuc, basis, ne, mult)
     assert abs(e - e_ref) < 1e-8, (e, e_ref)
 
 

@@ -113,3 +113,73 @@ def test_canonical_counts():
     from oracle.ref_faithful import n_canonical
     assert len(canonical_keys(7)) == 406 == n_canonical(7)
     assert n_canonical(220) == 295500205
+
+
+def _o2_sto3g_oracle_matrices():
+    from oracle import eri_c
+    from oracle.basis import nuclear_repulsion, read_basis_set_file, read_mol_file
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from tests.helpers import golden
+    nuc, ne, mult = read_mol_file(golden('O2.mol'))
+    basis = read_basis_set_file(golden('STO-3G.gbs'), nuc)
+    with eri_c.exact_boys():
+        eri = eri_c.eri_dense(basis)
+    S = overlap_matrix(basis)
+    H = kinetic_matrix(basis) + nuclear_matrix(basis, nuc)
+    return nuc, ne, mult, basis, eri, S, H, nuclear_repulsion(nuc)
+
+
+def test_blocked_scf_host_logic_matches_oracle():
+    """The device-side blocked SCF classes (torch; CPU tensors here) driven by
+    the oracle's blocked Fock sums reach the oracle GUHF energy and the
+    reference's published O2/STO-3G value."""
+    import torch
+    from gaussium_amd.scf import BlockedLinearAlgebra, BlockedUnrestrictedSCF
+    from oracle.scf import blocked_spin_basis_set, fock_blocked, guhf
+    nuc, ne, mult, basis, eri, S, H, e_nuc = _o2_sto3g_oracle_matrices()
+    z = np.zeros_like(H)
+    H2 = np.block([[H, z], [z, H]])
+    S2 = np.block([[S, z], [z, S]])
+    eri2 = blocked_spin_basis_set(eri)
+
+    class OracleBlockedFock:
+        core_hamiltonian = torch.from_numpy(H2)
+
+        def create(self, p2):
+            return torch.from_numpy(fock_blocked(H2, eri2, p2.numpy()))
+
+    la = BlockedLinearAlgebra(torch.from_numpy(S2))
+    scf = BlockedUnrestrictedSCF(la, ne, mult, torch.from_numpy(S2), OracleBlockedFock())
+    e, _, c = scf.begin_iterations(*la.diagonalize(torch.from_numpy(H2)))
+    e_ref, it = guhf(H, S, ne, mult, lambda p2: fock_blocked(H2, eri2, p2))
+    assert abs(e - e_ref) < 1e-10, (e, e_ref)
+    assert scf.iterations == it
+    np.testing.assert_approx_equal(e + e_nuc, -147.634028141, 6)
+    n = H.shape[0]
+    assert bool(torch.all(c[:n, n:] == 0)) and bool(torch.all(c[n:, :n] == 0))
+
+
+def test_unrestricted_scf_host_logic_matches_oracle():
+    import torch
+    from gaussium_amd.scf import LinearAlgebra, PopleNesbetBerthier
+    from oracle.scf import jk_from_dense, uhf
+    nuc, ne, mult, basis, eri, S, H, e_nuc = _o2_sto3g_oracle_matrices()
+
+    def fb(pa, pb):
+        ja, ka = jk_from_dense(eri, pa)
+        jb, kb = jk_from_dense(eri, pb)
+        return H + ja + jb - ka, H + ja + jb - kb
+
+    class OracleFock:
+        core_hamiltonian = torch.from_numpy(H)
+
+        def create(self, pa, pb):
+            fa, fb_ = fb(pa.numpy(), pb.numpy())
+            return torch.from_numpy(fa), torch.from_numpy(fb_)
+
+    la = LinearAlgebra(torch.from_numpy(S))
+    scf = PopleNesbetBerthier(la, ne, mult, OracleFock())
+    e = scf.begin_iterations(*la.diagonalize(torch.from_numpy(H)))[0]
+    e_ref, it = uhf(H, S, ne, mult, fb)
+    assert abs(e - e_ref) < 1e-10, (e, e_ref)
+    np.testing.assert_approx_equal(e + e_nuc, -147.634028141, 6)

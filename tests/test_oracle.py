@@ -342,6 +342,72 @@ def test_oracle_uhf_published_energy():
     np.testing.assert_approx_equal(e, -147.634028141, 6)
 
 
+def test_oracle_guhf_published_energy():
+    """tests/testslow/start/test_start.py:39-43: GUHF O2/STO-3G -147.634028141
+    (the blocked SCF reaches the UHF energy)."""
+    from tests.helpers import golden, oracle_guhf
+    from oracle.basis import read_basis_set_file, read_mol_file
+    nuc, ne, mult = read_mol_file(golden('O2.mol'))
+    basis = read_basis_set_file(golden('STO-3G.gbs'), nuc)
+    e, it = oracle_guhf(nuc, basis, ne, mult, exact_boys=False)
+    np.testing.assert_approx_equal(e, -147.634028141, 6)
+    e2, _ = oracle_guhf(nuc, basis, ne, mult)
+    assert abs(e - e2) < 1e-7
+
+
+def test_oracle_blocked_fock_matches_spatial_builds():
+    """The blocked Fock sums over the dense spin-blocked tensor against J/K
+    of the spatial tensor: diagonal spin blocks H + J(Pa + Pb) - K(Pa|Pb), the
+    alpha-beta block -K of the transposed beta-alpha density block, and the
+    beta-alpha block -K of the transposed alpha-beta block."""
+    from oracle.scf import blocked_spin_basis_set, fock_blocked, jk_from_dense
+    rng = np.random.default_rng(11)
+    n = 4
+    a = rng.standard_normal((n, n, n, n))
+    eri = a + a.transpose(1, 0, 2, 3)
+    eri = eri + eri.transpose(0, 1, 3, 2)
+    eri = eri + eri.transpose(2, 3, 0, 1)
+    h = rng.standard_normal((n, n))
+    h = h + h.T
+    p2 = rng.standard_normal((2 * n, 2 * n))
+    p2 = p2 + p2.T
+    z = np.zeros((n, n))
+    f2 = fock_blocked(np.block([[h, z], [z, h]]), blocked_spin_basis_set(eri), p2)
+    pa, pb = p2[:n, :n], p2[n:, n:]
+    pab, pba = p2[:n, n:], p2[n:, :n]
+    ja, ka = jk_from_dense(eri, pa)
+    jb, kb = jk_from_dense(eri, pb)
+    np.testing.assert_allclose(f2[:n, :n], h + ja + jb - ka, atol=1e-11)
+    np.testing.assert_allclose(f2[n:, n:], h + ja + jb - kb, atol=1e-11)
+    np.testing.assert_allclose(f2[:n, n:], -jk_from_dense(eri, pba.T)[1], atol=1e-11)
+    np.testing.assert_allclose(f2[n:, :n], -jk_from_dense(eri, pab.T)[1], atol=1e-11)
+
+
+def test_oracle_blocked_linear_algebra_is_two_spatial_problems():
+    from oracle.scf import BlockedLinearAlgebra, LinearAlgebra, blocked_density_matrix
+    rng = np.random.default_rng(12)
+    n = 5
+    m = rng.standard_normal((n, n))
+    s = m @ m.T + n * np.eye(n)
+    fa, fb = rng.standard_normal((2, n, n))
+    fa, fb = fa + fa.T, fb + fb.T
+    z = np.zeros((n, n))
+    la2 = BlockedLinearAlgebra(np.block([[s, z], [z, s]]))
+    e2, c2 = la2.diagonalize(np.block([[fa, z], [z, fb]]))
+    la = LinearAlgebra(s)
+    ea, ca = la.diagonalize(fa)
+    eb, cb = la.diagonalize(fb)
+    np.testing.assert_allclose(e2, np.concatenate([ea, eb]), atol=1e-12)
+    # spin-pure orbitals, and the blocked density of (2, 1) electrons is
+    # blockdiag(C_a,occ C_a,occ^T, C_b,occ C_b,occ^T)
+    assert np.all(c2[:n, n:] == 0.0) and np.all(c2[n:, :n] == 0.0)
+    p2 = blocked_density_matrix(c2, 2, 1)
+    pa = ca[:, :2] @ ca[:, :2].T
+    pb = cb[:, :1] @ cb[:, :1].T
+    np.testing.assert_allclose(p2[:n, :n], pa, atol=1e-10)
+    np.testing.assert_allclose(p2[n:, n:], pb, atol=1e-10)
+
+
 def test_oracle_mp2_published_energy():
     """tests/testslow/start/test_start.py:45-49: MP2 CO/STO-3G -111.354512528."""
     from tests.helpers import oracle_mp2
