@@ -164,6 +164,7 @@ struct gx_ctx {
   std::vector<int> wg_per_cu;  // [combo][FP64, mixed, two-density] resident workgroups per CU, 0 = not queried
   std::vector<int> wg_per_cu_grad;
   int* d_atom = nullptr;
+  int* d_item_ctr = nullptr;  // per class combination: [next item, workgroups done] of the J/K, then the gradient kernels
   // ket segments: per class, runs of pairs with the same primitive-count key
   std::vector<int> seg_lo, seg_hi;
   std::vector<int> cseg;  // [class][system] first segment, NCLS x (nsys + 1)
@@ -245,6 +246,7 @@ struct gx_ctx {
     dfree(&d_Ps2);
     dfree(&d_Psj);
     dfree(&d_Kt2);
+    dfree(&d_item_ctr);
     dfree(&d_nxyz);
     dfree(&d_nZ);
     dfree(&d_boys);
@@ -867,6 +869,7 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden 
   a.Jt = c->d_Jt;
   a.Kt = c->d_Kt;
   a.Kt2 = nden == 2 ? c->d_Kt2 : c->d_Kt;
+  a.item_ctr = c->d_item_ctr + 2 * combo;
   a.jstride = c->mtot;
   a.mixed = mixed;
   if (c->wg_per_cu.empty()) c->wg_per_cu.assign(3 * combo_count(), 0);
@@ -932,6 +935,7 @@ hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad) {
   a.atom = c->d_atom;
   a.sys_atom = c->d_sys_atom;
   a.grad = grad;
+  a.item_ctr = c->d_item_ctr + 2 * combo_count() + 2 * combo;
   return launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, c->stream, a,
                      &c->wg_per_cu_grad[combo], c->nmax);
 }
@@ -972,6 +976,10 @@ int gx_create(gx_ctx** out, int device) {
                        c->d_tab64, c->d_tab32);
     if (hipGetLastError() != hipSuccess) rc = fail(GX_ERR_HIP, "k_boys_tables launch");
   }
+  // item counters start at zero; each J/K launch leaves them at zero
+  if (rc == GX_OK) rc = dalloc(&c->d_item_ctr, (size_t)4 * combo_count());
+  if (rc == GX_OK && hipMemsetAsync(c->d_item_ctr, 0, sizeof(int) * 4 * combo_count(), c->stream) != hipSuccess)
+    rc = fail(GX_ERR_HIP, "gx_create: counter initialisation");
   if (rc == GX_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GX_ERR_HIP, "boys upload");
   if (rc != GX_OK) {
     c->free_all();

@@ -208,6 +208,8 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
   __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
   const int lane = threadIdx.x;
   const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
+  // static striding over the items (claiming them from a counter, as the
+  // gradient kernels do, measured slower here)
   for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
@@ -326,6 +328,7 @@ struct GradArgs {
   const int* atom;     // shell -> atom (over all systems)
   const int* sys_atom; // per system: first nucleus (nsys + 1)
   double* grad;        // 3 natom
+  int* item_ctr;       // [next item, workgroups done]: zero before a launch, reset by its last workgroup
 };
 
 // Gradient contributions collect in LDS per nucleus of the item's system (at
@@ -357,7 +360,9 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
   // nuclear sums of the current system, kept in LDS across the items of
   // the workgroup and flushed when the system changes and at the end
   int cur = -1, cur3 = 0;
-  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
+  // items in decreasing cost: the first is blockIdx.x, the rest are claimed
+  // from a counter as workgroups finish
+  for (int w = blockIdx.x; w < a.nitem;) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
     const int A = a.pr.A[ip], B = a.pr.B[ip];
@@ -454,10 +459,21 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
         }
       }
     }
+    int nx = 0;
+    if (lane == 0) nx = atomicAdd(&a.item_ctr[0], 1) + (int)gridDim.x;
+    w = __shfl(nx, 0, GX_WAVE);
   }
   __syncthreads();
   for (int t = lane; t < cur3; t += GX_WAVE)
     if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
+  // the last workgroup to finish zeroes the counters for the next launch
+  if (lane == 0) {
+    __threadfence();
+    if (atomicAdd(&a.item_ctr[1], 1) == (int)gridDim.x - 1) {
+      atomicExch(&a.item_ctr[0], 0);
+      atomicExch(&a.item_ctr[1], 0);
+    }
+  }
 }
 
 }  // namespace gx

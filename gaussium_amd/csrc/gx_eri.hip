@@ -376,6 +376,7 @@ struct JKArgs {
   double* Jt;            // packed accumulators: GX_JT_COPIES copies of jstride doubles for Jt
   double* Kt;
   double* Kt2;           // K accumulator of the second density (single copy)
+  int* item_ctr;         // [next item, workgroups done]: zero before a launch, reset by its last workgroup
   long jstride;
   int mixed;
 };
@@ -422,7 +423,11 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
   const int wave = tid / GX_WAVE;
   constexpr int nwave = JK_TEAM / GX_WAVE;
 
-  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
+  // Items are sorted by decreasing cost.  A workgroup starts on item
+  // blockIdx.x and then claims the next unclaimed one from a counter, so the
+  // cheap items at the end even out the workgroups' finishing times.
+  __shared__ int next_item;
+  for (int w = blockIdx.x; w < a.nitem;) {
     const int4 it = a.items[w];
     const int bpos = it.x;
     const int ip = a.bra0 + bpos;
@@ -650,6 +655,18 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
         const double y = krow[nrk + i];
         if (y != 0.0) atomicAdd(&Km2[(long)g * N + c], y);
       }
+    }
+    if (tid == 0) next_item = atomicAdd(&a.item_ctr[0], 1) + (int)gridDim.x;
+    __syncthreads();
+    w = next_item;
+  }
+  // the last workgroup to finish zeroes the counters for the next launch
+  // (every other workgroup has made its last claim before counting itself)
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(&a.item_ctr[1], 1) == (int)gridDim.x - 1) {
+      atomicExch(&a.item_ctr[0], 0);
+      atomicExch(&a.item_ctr[1], 0);
     }
   }
 }

@@ -25,7 +25,10 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
   // nuclear sums of the current system, kept in LDS across the items of
   // the workgroup and flushed when the system changes and at the end
   int cur = -1, cur3 = 0;
-  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
+  // items in decreasing cost: the first is blockIdx.x, the rest are claimed
+  // from a counter as workgroups finish
+  __shared__ int next_item;
+  for (int w = blockIdx.x; w < a.nitem;) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
     const int A = a.pr.A[ip], B = a.pr.B[ip];
@@ -163,10 +166,21 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
         atomicAdd(&gsh[3 * atB + x], gab[3 + x]);
       }
     }
+    if (tid == 0) next_item = atomicAdd(&a.item_ctr[0], 1) + (int)gridDim.x;
+    __syncthreads();
+    w = next_item;
   }
   __syncthreads();
   for (int t = tid; t < cur3; t += JK_TEAM)
     if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
+  // the last workgroup to finish zeroes the counters for the next launch
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(&a.item_ctr[1], 1) == (int)gridDim.x - 1) {
+      atomicExch(&a.item_ctr[0], 0);
+      atomicExch(&a.item_ctr[1], 0);
+    }
+  }
 }
 
 }  // namespace gx
