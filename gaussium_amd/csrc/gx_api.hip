@@ -461,7 +461,7 @@ size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed, int nden =
   if (la == A && lb == B && lc == C && ld == D) {                                               \
     using G = EriProg<A, B, C, D>;                                                              \
     const int nblk = (G::NA + G::NB) * (G::NC + G::ND) + G::NA * G::NB + G::NC * G::ND;          \
-    return coop_bytes<G>() + 2 * (size_t)nblk * sizeof(double);                                 \
+    return (size_t)(gx::coop_lds_ng<G, gx::coop_groups<G>()>() + 2 * nblk) * sizeof(double);   \
   }
   GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE

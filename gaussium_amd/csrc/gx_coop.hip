@@ -20,77 +20,113 @@ constexpr int coop_lds() {
   return G::NACC + G::NT > G::NH ? G::NACC + G::NT : G::NH;
 }
 
+// With NG lane groups (see coop_quartet): NG accumulator + VRR tables.
+template <class G, int NG>
+constexpr int coop_lds_ng() {
+  return NG * (G::NACC + G::NT) > G::NH ? NG * (G::NACC + G::NT) : G::NH;
+}
+
+// Lane groups of the J/K kernel: two for the smaller programs, whose VRR
+// levels (15-100 entries) leave half of a 64-lane pass idle; one for the
+// larger ones, where the doubled LDS costs more than the fuller passes gain
+// (measured on the d-shell classes: (dp|ds) -25%, (dd|ds) +25% with two).
+template <class G>
+constexpr int coop_groups() {
+  return G::NACC + G::NT <= 512 ? 2 : 1;
+}
+
 // Contracted shell quartet into H: on return H[G::out()[k]] is component k
 // (reference order), unscaled.  Every lane of the wave calls it.  With
 // GRAD, G is a gradient program (EriGradProg): accumulator entries carry a
 // weight set (1, 2 alpha_a, 2 alpha_b, 2 alpha_c) and slot NACC stays zero.
-template <class G, bool GRAD = false>
+// NG > 1: the wave splits into NG groups of 64 / NG lanes, each running the
+// VRR of every NG-th primitive quartet on its own table (H + g (NACC + NT))
+// into its own accumulators; the group sums go to H before the HRR.  Small
+// recursion levels then keep more lanes busy.
+template <class G, bool GRAD = false, int NG = 1>
 __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra, int nb,
                                              const PrimPair64* __restrict__ ket, int nk, double ABx,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
                                              const double* __restrict__ tab, double* __restrict__ H) {
+  constexpr int W = GX_WAVE / NG;  // lanes per group
   const int lane = threadIdx.x & (GX_WAVE - 1);
-  double* __restrict__ T = H + G::NACC;
+  const int grp = lane / W, sub = lane - grp * W;
+  double* __restrict__ Hg = H + grp * (G::NACC + G::NT);
+  double* __restrict__ T = Hg + G::NACC;
   const int4* __restrict__ vp = G::vrr();
   const int* __restrict__ vl = G::vlev();
   const auto* __restrict__ ap = G::acc();
   __syncthreads();  // the previous quartet's readers of H are done
-  for (int t = lane; t < G::NACC; t += GX_WAVE) H[t] = 0.0;
-  if (lane == 0) T[0] = 0.0;
-  for (int ik = 0; ik < nk; ++ik) {
+  for (int t = sub; t < G::NACC; t += W) Hg[t] = 0.0;
+  if (sub == 0) T[0] = 0.0;
+  const int nq = nk * nb;
+  for (int q0 = 0; q0 < nq; q0 += NG) {
+    const int q = q0 + grp;
+    const bool act = q < nq;  // this group has a primitive quartet this round
+    const int ik = act ? q / nb : 0, ib = act ? q - (q / nb) * nb : 0;
     const PrimPair64 k = ket[ik];
-    for (int ib = 0; ib < nb; ++ib) {
-      const PrimPair64 b = bra[ib];
-      const double rs = gx_rsqrt(b.p + k.p);
-      const double is = rs * rs;
-      const double rho = b.p * k.p * is;
-      const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
-      const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = b.Kip * k.Kip * rs;
-      double F[G::L + 1];
-      boys64_nb<G::L>(Tb, pref, tab, F);
-      const double rho_p = rho * b.ip, rho_q = rho * k.ip;
-      const double WPx = -rho_p * PQx, WPy = -rho_p * PQy, WPz = -rho_p * PQz;
-      const double WQx = rho_q * PQx, WQy = rho_q * PQy, WQz = rho_q * PQz;
-      const double o2p = 0.5 * b.ip, o2q = 0.5 * k.ip, o2pq = 0.5 * is;
+    const PrimPair64 b = bra[ib];
+    const double rs = gx_rsqrt(b.p + k.p);
+    const double is = rs * rs;
+    const double rho = b.p * k.p * is;
+    const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+    const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+    const double pref = b.Kip * k.Kip * rs;
+    double F[G::L + 1];
+    boys64_nb<G::L>(Tb, pref, tab, F);
+    const double rho_p = rho * b.ip, rho_q = rho * k.ip;
+    const double WPx = -rho_p * PQx, WPy = -rho_p * PQy, WPz = -rho_p * PQz;
+    const double WQx = rho_q * PQx, WQy = rho_q * PQy, WQz = rho_q * PQz;
+    const double o2p = 0.5 * b.ip, o2q = 0.5 * k.ip, o2pq = 0.5 * is;
 #pragma unroll
-      for (int m = 0; m <= G::L; ++m)
-        if (lane == m) T[1 + m] = F[m];
-      __syncthreads();
-      for (int lev = 1; lev <= G::L; ++lev) {
-        const int e1 = vl[lev + 1];
-        for (int t = vl[lev] + lane; t < e1; t += GX_WAVE) {
-          const int4 en = vp[t];
-          const unsigned x = (unsigned)en.x, y = (unsigned)en.y, z = (unsigned)en.z;
-          const bool ket_step = (x >> 16) & 1u;
-          const unsigned ax = (x >> 17) & 3u;
-          const double k1 = (double)((x >> 20) & 15u), k2 = (double)((x >> 24) & 15u);
-          const double pa = ax == 0 ? b.PAx : (ax == 1 ? b.PAy : b.PAz);
-          const double qc = ax == 0 ? k.PAx : (ax == 1 ? k.PAy : k.PAz);
-          const double wp = ax == 0 ? WPx : (ax == 1 ? WPy : WPz);
-          const double wq = ax == 0 ? WQx : (ax == 1 ? WQy : WQz);
-          const double cA = ket_step ? qc : pa, cB = ket_step ? wq : wp;
-          const double o = ket_step ? o2q : o2p, r = ket_step ? rho_q : rho_p;
-          double v = cA * T[y & 0xffffu];
-          v = fma(cB, T[y >> 16], v);
-          v = fma(k1 * o, T[z & 0xffffu] - r * T[z >> 16], v);
-          v = fma(k2 * o2pq, T[(unsigned)en.w], v);
-          T[x & 0xffffu] = v;
-        }
-        __syncthreads();
+    for (int m = 0; m <= G::L; ++m)
+      if (sub == m) T[1 + m] = F[m];
+    __syncthreads();
+    for (int lev = 1; lev <= G::L; ++lev) {
+      const int e1 = vl[lev + 1];
+      for (int t = vl[lev] + sub; t < e1; t += W) {
+        const int4 en = vp[t];
+        const unsigned x = (unsigned)en.x, y = (unsigned)en.y, z = (unsigned)en.z;
+        const bool ket_step = (x >> 16) & 1u;
+        const unsigned ax = (x >> 17) & 3u;
+        const double k1 = (double)((x >> 20) & 15u), k2 = (double)((x >> 24) & 15u);
+        const double pa = ax == 0 ? b.PAx : (ax == 1 ? b.PAy : b.PAz);
+        const double qc = ax == 0 ? k.PAx : (ax == 1 ? k.PAy : k.PAz);
+        const double wp = ax == 0 ? WPx : (ax == 1 ? WPy : WPz);
+        const double wq = ax == 0 ? WQx : (ax == 1 ? WQy : WQz);
+        const double cA = ket_step ? qc : pa, cB = ket_step ? wq : wp;
+        const double o = ket_step ? o2q : o2p, r = ket_step ? rho_q : rho_p;
+        double v = cA * T[y & 0xffffu];
+        v = fma(cB, T[y >> 16], v);
+        v = fma(k1 * o, T[z & 0xffffu] - r * T[z >> 16], v);
+        v = fma(k2 * o2pq, T[(unsigned)en.w], v);
+        T[x & 0xffffu] = v;
       }
+      __syncthreads();
+    }
+    if (act) {
       if constexpr (GRAD) {
         const double w1 = 2.0 * b.ea, w2 = 2.0 * (b.p - b.ea), w3 = 2.0 * k.ea;
-        for (int t = lane; t < G::NACC; t += GX_WAVE) {
+        for (int t = sub; t < G::NACC; t += W) {
           const unsigned e = ap[t], st = e >> 16;
           const double wt = st == 0 ? 1.0 : (st == 1 ? w1 : (st == 2 ? w2 : w3));
-          H[t] = fma(wt, T[e & 0xffffu], H[t]);
+          Hg[t] = fma(wt, T[e & 0xffffu], Hg[t]);
         }
       } else {
-        for (int t = lane; t < G::NACC; t += GX_WAVE) H[t] += T[ap[t]];
+        for (int t = sub; t < G::NACC; t += W) Hg[t] += T[ap[t]];
       }
-      __syncthreads();  // T is rewritten by the next primitive quartet
     }
+    __syncthreads();  // T is rewritten by the next primitive quartet
+  }
+  if constexpr (NG > 1) {
+    // group sums into the first group's accumulators
+    for (int t = lane; t < G::NACC; t += GX_WAVE) {
+      double v = H[t];
+#pragma unroll
+      for (int g = 1; g < NG; ++g) v += H[g * (G::NACC + G::NT) + t];
+      H[t] = v;
+    }
+    __syncthreads();
   }
   const int2* __restrict__ hp = G::hrr();
   const int* __restrict__ hl = G::hlev();
@@ -203,7 +239,8 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
   constexpr int NBLK = SAB + SCD + SAC + SAD + SBC + SBD;
   extern __shared__ double smem[];
   double* __restrict__ H = smem;
-  double* __restrict__ pblk = smem + coop_lds<G>();  // P_ab P_cd P_ac P_ad P_bc P_bd
+  constexpr int NG = coop_groups<G>();
+  double* __restrict__ pblk = smem + coop_lds_ng<G, NG>();  // P_ab P_cd P_ac P_ad P_bc P_bd
   double* __restrict__ jblk = pblk + NBLK;           // J_ab J_cd K_ac K_ad K_bc K_bd partials
   __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
   const int lane = threadIdx.x;
@@ -243,7 +280,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
       const int jp = r_start[r] + (v - r_pre[r]);
       const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
       const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
-      coop_quartet<G>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
+      coop_quartet<G, false, NG>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
                       a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
                       a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], tab, H);
       double f = fA * ((Cs == Ds) ? 0.5 : 1.0);

@@ -145,3 +145,17 @@ def test_jk_uhf_batch_after_smaller_basis():
     assert (J - (Ja + Jb)).abs().max().item() < 1e-11
     assert (Ka - Ka1).abs().max().item() < 1e-11
     assert (Kb - Kb1).abs().max().item() < 1e-11
+
+
+@pytest.mark.parametrize('method', ['UHF', 'GUHF'])
+def test_unrestricted_mixed_precision_and_screening(method):
+    """UHF and GUHF with Schwarz screening and the mixed-precision builds
+    (two single-density builds per iteration) stay within 1e-6 Eh of the
+    FP64 unscreened energy and the published value."""
+    from gaussium_amd import start
+    from tests.helpers import golden
+    e64 = start(golden('O2.mol'), golden('STO-3G.gbs'), method, 1)
+    emx = start(golden('O2.mol'), golden('STO-3G.gbs'), method, 1, schwarz_cutoff=1e-11, fp32_bound=1e-4,
+                precision='mixed')
+    assert abs(emx - e64) < 1e-6, (emx, e64)
+    np.testing.assert_approx_equal(emx, -147.634028141, 6)
