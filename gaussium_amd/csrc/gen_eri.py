@@ -423,7 +423,10 @@ def coop_program(la, lb, lc, ld):
 #   accumulator entry (unsigned int): T slot | set << 16
 #   gradient term (int2): x = plus, y = minus | n << 16   (minus = the zero slot if n = 0)
 # ---------------------------------------------------------------------------
-def grad_program(la, lb, lc, ld):
+def grad_program(la, lb, lc, ld, centers=(0, 1, 2)):
+    """Derivative program of (ab|cd) with respect to the centres in `centers`
+    (0 = A, 1 = B, 2 = C): the raised set of each of them (weight 2 alpha)
+    and the lowered (unweighted) set they need; 3 terms per centre."""
     L = la + lb + lc + ld + 1
     zero = (0, 0, 0)
     memo = {}
@@ -468,19 +471,22 @@ def grad_program(la, lb, lc, ld):
     def rect(e0, e1, f0, f1):
         return [(le, lf) for le in range(max(e0, 0), e1 + 1) for lf in range(max(f0, 0), f1 + 1)]
 
-    sets = {1: rect(la + 1, la + lb + 1, lc, lc + ld), 2: rect(la, la + lb + 1, lc, lc + ld),
-            3: rect(la, la + lb, lc + 1, lc + ld + 1)}
+    raised = {1: rect(la + 1, la + lb + 1, lc, lc + ld), 2: rect(la, la + lb + 1, lc, lc + ld),
+              3: rect(la, la + lb, lc + 1, lc + ld + 1)}
+    sets = {0: []}
+    for X in centers:
+        sets[X + 1] = raised[X + 1]
     low = set()
-    if la >= 1:
+    if la >= 1 and 0 in centers:
         low |= set(rect(la - 1, la + lb - 1, lc, lc + ld))
-    if lb >= 1:
+    if lb >= 1 and 1 in centers:
         low |= set(rect(la, la + lb - 1, lc, lc + ld))
-    if lc >= 1:
+    if lc >= 1 and 2 in centers:
         low |= set(rect(la, la + lb, lc - 1, lc + ld - 1))
     sets[0] = sorted(low)
     acc = []
     index = {}
-    for st in (0, 1, 2, 3):
+    for st in sorted(sets):
         for (le, lf) in sets[st]:
             for e in cart(le):
                 for f in cart(lf):
@@ -541,7 +547,7 @@ def grad_program(la, lb, lc, ld):
         for b in cart(lb):
             for c in cart(lc):
                 for d in cart(ld):
-                    for X in range(3):
+                    for X in centers:
                         for i in range(3):
                             if X == 0:
                                 plus = ket(1, add(a, i), b, c, d)
@@ -569,8 +575,8 @@ def grad_program(la, lb, lc, ld):
     for lev in range(1, nhl):
         hl.append(hl[-1] + hcount[lev])
     hrr = [e for _, e in hentries]
-    return dict(L=L, NACC=nacc, NOUT=len(terms) // 9, NT=nslot[0], NH=hslot[0], ZSLOT=zslot, vrr=vrr,
-                vlev=vlev, acc=acc, hrr=hrr, hlev=hl, NHLEV=nhl - 1, terms=terms)
+    return dict(L=L, NACC=nacc, NOUT=len(terms) // (3 * len(centers)), NT=nslot[0], NH=hslot[0], ZSLOT=zslot,
+                vrr=vrr, vlev=vlev, acc=acc, hrr=hrr, hlev=hl, NHLEV=nhl - 1, terms=terms, centers=tuple(centers))
 
 
 def emit_grad_prog(la, lb, lc, ld):
@@ -603,12 +609,25 @@ def emit_grad_prog(la, lb, lc, ld):
 # ---------------------------------------------------------------------------
 # Per-lane gradient code for the small classes: the gradient program written
 # out as straight-line code (each VRR slot a register, the HRR slots renamed
-# on every write), one shell quartet per lane.
-GRAD_LANE_NACC = 200  # gradient classes with at most this many weighted intermediates run per lane
+# on every write), one shell quartet per lane.  A class whose whole program
+# fits the registers runs in one pass; one whose per-centre programs fit runs
+# three passes over the primitives (A, B, C derivatives; the lowered set is
+# recomputed per pass) instead of spilling; the rest run wave-cooperatively.
+GRAD_LANE_NACC = 120  # one pass when the whole program has at most this many weighted intermediates
+GRAD_PASS_NACC = 120  # three passes when each per-centre program has at most this many
 
 
-def emit_grad_lane(la, lb, lc, ld):
-    g = grad_program(la, lb, lc, ld)
+def grad_lane_passes(c):
+    """1 (one pass over all centres), 3 (one pass per centre) or 0 (cooperative)."""
+    if grad_program(*c)['NACC'] <= GRAD_LANE_NACC:
+        return 1
+    if max(grad_program(*c, centers=(x,))['NACC'] for x in range(3)) <= GRAD_PASS_NACC:
+        return 3
+    return 0
+
+
+def emit_grad_lane_pass(la, lb, lc, ld, X, centers):
+    g = grad_program(la, lb, lc, ld, centers=centers)
     L = g['L']
     na, nb, nc, nd = ncart(la), ncart(lb), ncart(lc), ncart(ld)
     axes = 'xyz'
@@ -643,9 +662,10 @@ def emit_grad_lane(la, lb, lc, ld):
     for n_, (x, y) in enumerate(g['hrr']):
         dest, kind, ax = x & 0xffff, (x >> 16) & 1, (x >> 17) & 3
         s0, s1 = y & 0xffff, y >> 16
-        X = ('CD' if kind else 'AB') + axes[ax]
-        fin.append('    const R h%d = %s + %s * %s;' % (n_, cur[s0], X, cur[s1]))
+        Xs = ('CD' if kind else 'AB') + axes[ax]
+        fin.append('    const R h%d = %s + %s * %s;' % (n_, cur[s0], Xs, cur[s1]))
         cur[dest] = 'h%d' % n_
+    nterm = 3 * len(centers)
     k = 0
     for ia in range(na):
         for ib in range(nb):
@@ -655,19 +675,19 @@ def emit_grad_lane(la, lb, lc, ld):
                     fin.append('      const R D = f * (R(4) * Pab[%d] * Pcd[%d] - Pac[%d] * Pbd[%d] - Pad[%d] * Pbc[%d]);'
                                % (ia * nb + ib, ic * nd + id_, ia * nc + ic, ib * nd + id_, ia * nd + id_,
                                   ib * nc + ic))
-                    for j in range(9):
-                        plus, mword = g['terms'][9 * k + j]
+                    for j in range(nterm):
+                        gi = 3 * centers[j // 3] + j % 3
+                        plus, mword = g['terms'][nterm * k + j]
                         minus, n = mword & 0xffff, mword >> 16
                         if n:
-                            fin.append('      g[%d] += D * (%s - R(%d) * %s);' % (j, cur[plus], n, cur[minus]))
+                            fin.append('      g[%d] += D * (%s - R(%d) * %s);' % (gi, cur[plus], n, cur[minus]))
                         else:
-                            fin.append('      g[%d] += D * %s;' % (j, cur[plus]))
+                            fin.append('      g[%d] += D * %s;' % (gi, cur[plus]))
                     fin.append('    }')
                     k += 1
     s = []
-    s.append('template <> struct EriGradLane<%d, %d, %d, %d> {' % (la, lb, lc, ld))
+    s.append('template <> struct EriGradLanePass<%d, %d, %d, %d, %d> {' % (la, lb, lc, ld, X))
     s.append('  static constexpr int L = %d, NACC = %d;' % (L, g['NACC']))
-    s.append('  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;' % (na, nb, nc, nd))
     s.append('  template <typename R>')
     s.append('  static __device__ __forceinline__ void prim(R (&acc)[NACC], const R (&F)[L + 1],')
     s.append('      R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,')
@@ -679,6 +699,7 @@ def emit_grad_lane(la, lb, lc, ld):
     s.append('  }')
     s.append('  // HRR of the weighted sets and contraction with')
     s.append('  // D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc):  g[3 X + i] += D_k d(ab|cd)_k / dX_i')
+    s.append('  // for the centres X of this pass (%s)' % ', '.join('ABC'[x] for x in centers))
     s.append('  template <typename R>')
     s.append('  static __device__ __forceinline__ void finish(const R (&acc)[NACC], R ABx, R ABy, R ABz,')
     s.append('      R CDx, R CDy, R CDz, const R* Pab, const R* Pcd, const R* Pac, const R* Pad,')
@@ -690,8 +711,26 @@ def emit_grad_lane(la, lb, lc, ld):
     return '\n'.join(s)
 
 
+def emit_grad_lane(la, lb, lc, ld):
+    npass = grad_lane_passes((la, lb, lc, ld))
+    assert npass in (1, 3)
+    parts = []
+    if npass == 1:
+        parts.append(emit_grad_lane_pass(la, lb, lc, ld, 0, (0, 1, 2)))
+    else:
+        for X in range(3):
+            parts.append(emit_grad_lane_pass(la, lb, lc, ld, X, (X,)))
+    na, nb, nc, nd = ncart(la), ncart(lb), ncart(lc), ncart(ld)
+    L = la + lb + lc + ld + 1
+    parts.append('template <> struct EriGradLane<%d, %d, %d, %d> {\n'
+                 '  static constexpr int L = %d, NPASS = %d;\n'
+                 '  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;\n};'
+                 % (la, lb, lc, ld, L, npass, na, nb, nc, nd))
+    return '\n'.join(parts)
+
+
 def grad_is_lane(c):
-    return grad_program(*c)['NACC'] <= GRAD_LANE_NACC
+    return grad_lane_passes(c) > 0
 
 
 def emit_prog(la, lb, lc, ld):
@@ -739,7 +778,8 @@ def main():
              'template <int LA, int LB, int LC, int LD> struct EriClass;',
              'template <int LA, int LB, int LC, int LD> struct EriProg;',
              'template <int LA, int LB, int LC, int LD> struct EriGradProg;',
-             'template <int LA, int LB, int LC, int LD> struct EriGradLane;']
+             'template <int LA, int LB, int LC, int LD> struct EriGradLane;',
+             'template <int LA, int LB, int LC, int LD, int X> struct EriGradLanePass;']
     lane = [c for c in classes(lmax) if not is_coop(c)]
     coop = [c for c in classes(lmax) if is_coop(c)]
     for c in lane:

@@ -1,13 +1,73 @@
-// Per-lane two-electron gradient kernels for the small classes
-// ((ss|ss), (ps|ss), (ps|ps), (pp|ss), (ds|ss)): the quartets of a bra are
-// swept 64 kets at a time, one ket shell pair per lane, as in the fused J/K
-// kernel; the lane accumulates the weighted [e0|f0] sets of EriGradLane,
-// then its HRR and the contraction with
+// Per-lane two-electron gradient kernels for the smaller classes: the
+// quartets of a bra are swept 64 kets at a time, one ket shell pair per lane,
+// as in the fused J/K kernel; the lane accumulates the weighted [e0|f0] sets
+// of EriGradLanePass, then its HRR and the contraction with
 //   D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc)
 // give the nine derivatives on A, B, C (D by translational invariance).
+// Classes whose sets fit the registers at once run one pass; those whose
+// per-centre sets fit run three passes over the primitives (EriGradLane::NPASS).
 // The larger classes use k_grad_coop (gx_coop.hip).
 
 namespace gx {
+
+// One pass of the per-lane gradient: the weighted sets of pass X over the
+// bra x ket primitive pairs, then its HRR and contraction into g.  The
+// density blocks are read (from the LDS rows of the bra's shells and P_cd
+// from HBM) only after the primitive loop, so they hold no registers there.
+template <int LA, int LB, int LC, int LD, int X>
+__device__ __forceinline__ void grad_lane_pass(const PrimPair64* __restrict__ bra, int nbp,
+                                               const PrimPair64* __restrict__ kp, int nk,
+                                               const double* __restrict__ tab, double ABx, double ABy,
+                                               double ABz, double CDx, double CDy, double CDz,
+                                               const double* __restrict__ prow, const double* __restrict__ Pm,
+                                               int N, int rowB, int bo, int co, int dof, double f, double (&g)[9]) {
+  using P = EriGradLanePass<LA, LB, LC, LD, X>;
+  using C = EriGradLane<LA, LB, LC, LD>;
+  constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+  double acc[P::NACC];
+#pragma unroll
+  for (int i = 0; i < P::NACC; ++i) acc[i] = 0.0;
+  for (int ik = 0; ik < nk; ++ik) {
+    const PrimPair64 k = kp[ik];
+    const double w3 = 2.0 * k.ea;
+    for (int ib = 0; ib < nbp; ++ib) {
+      const PrimPair64 b = bra[ib];
+      const double rs = gx_rsqrt(b.p + k.p);
+      const double is = rs * rs;
+      const double rho = b.p * k.p * is;
+      const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      double F[P::L + 1];
+      boys64_nb<P::L>(T, b.Kip * k.Kip * rs, tab, F);
+      const double rho_p = rho * b.ip, rho_q = rho * k.ip;
+      P::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz, k.PAx,
+                               k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip, 0.5 * k.ip,
+                               0.5 * is, rho_p, rho_q, 2.0 * b.ea, 2.0 * (b.p - b.ea), w3);
+    }
+  }
+  double Pab[NA * NB], Pcd[NC * ND], Pac[NA * NC], Pad[NA * ND], Pbc[NB * NC], Pbd[NB * ND];
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) Pab[ia * NB + ib] = prow[ia * N + bo + ib];
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic) Pac[ia * NC + ic] = prow[ia * N + co + ic];
+#pragma unroll
+    for (int id = 0; id < ND; ++id) Pad[ia * ND + id] = prow[ia * N + dof + id];
+  }
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic) Pbc[ib * NC + ic] = prow[(rowB + ib) * N + co + ic];
+#pragma unroll
+    for (int id = 0; id < ND; ++id) Pbd[ib * ND + id] = prow[(rowB + ib) * N + dof + id];
+  }
+#pragma unroll
+  for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+    for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pm[(long)(co + ic) * N + dof + id];
+  P::template finish<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, Pab, Pcd, Pac, Pad, Pbc, Pbd, f, g);
+}
 
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
@@ -86,57 +146,21 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
         const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
         const PrimPair64* __restrict__ kp = a.pr.pp64 + a.pr.poff[jp];
         const int nk = a.pr.np[jp];
-        double acc[C::NACC];
-#pragma unroll
-        for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
-        for (int ik = 0; ik < nk; ++ik) {
-          const PrimPair64 k = kp[ik];
-          const double w3 = 2.0 * k.ea;
-          for (int ib = 0; ib < nbp; ++ib) {
-            const PrimPair64 b = bra[ib];
-            const double rs = gx_rsqrt(b.p + k.p);
-            const double is = rs * rs;
-            const double rho = b.p * k.p * is;
-            const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
-            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            double F[C::L + 1];
-            boys64_nb<C::L>(T, b.Kip * k.Kip * rs, tab, F);
-            const double rho_p = rho * b.ip, rho_q = rho * k.ip;
-            C::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz,
-                                     k.PAx, k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip,
-                                     0.5 * k.ip, 0.5 * is, rho_p, rho_q, 2.0 * b.ea, 2.0 * (b.p - b.ea), w3);
-          }
-        }
-        double Pab[NA * NB], Pcd[NC * ND], Pac[NA * NC], Pad[NA * ND], Pbc[NB * NC], Pbd[NB * ND];
-#pragma unroll
-        for (int ia = 0; ia < NA; ++ia) {
-#pragma unroll
-          for (int ib = 0; ib < NB; ++ib) Pab[ia * NB + ib] = prow[ia * N + bo + ib];
-#pragma unroll
-          for (int ic = 0; ic < NC; ++ic) Pac[ia * NC + ic] = prow[ia * N + co + ic];
-#pragma unroll
-          for (int id = 0; id < ND; ++id) Pad[ia * ND + id] = prow[ia * N + dof + id];
-        }
-#pragma unroll
-        for (int ib = 0; ib < NB; ++ib) {
-#pragma unroll
-          for (int ic = 0; ic < NC; ++ic) Pbc[ib * NC + ic] = prow[(rowB + ib) * N + co + ic];
-#pragma unroll
-          for (int id = 0; id < ND; ++id) Pbd[ib * ND + id] = prow[(rowB + ib) * N + dof + id];
-        }
-#pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pm[(long)(co + ic) * N + dof + id];
         double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
         if (jp == ip) f *= 0.5;
+        const double CDx = a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], CDy = a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
+                     CDz = a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2];
         double g[9];
 #pragma unroll
         for (int j = 0; j < 9; ++j) g[j] = 0.0;
-        C::template finish<double>(acc, ABx, ABy, ABz, a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds],
-                                   a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
-                                   a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], Pab, Pcd, Pac, Pad, Pbc, Pbd, f,
-                                   g);
+        grad_lane_pass<LA, LB, LC, LD, 0>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N, rowB,
+                                          bo, co, dof, f, g);
+        if constexpr (C::NPASS == 3) {
+          grad_lane_pass<LA, LB, LC, LD, 1>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N,
+                                            rowB, bo, co, dof, f, g);
+          grad_lane_pass<LA, LB, LC, LD, 2>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N,
+                                            rowB, bo, co, dof, f, g);
+        }
 #pragma unroll
         for (int j = 0; j < 6; ++j) gab[j] += g[j];
         const int atC = a.atom[Cs] - at0, atD = a.atom[Ds] - at0;

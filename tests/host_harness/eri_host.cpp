@@ -350,10 +350,11 @@ extern "C" int host_quartet_grad(const int* l, const double* ctr, const int* n, 
 // Per-lane gradient code (EriGradLane): g[3 X + i] of the quartet contracted
 // with D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc); blocks in `pb` in the
 // order P_ab, P_cd, P_ac, P_ad, P_bc, P_bd.
-template <int LA, int LB, int LC, int LD>
-static int quartet_grad_lane(const Shell& A, const Shell& B, const Shell& C, const Shell& D, const double* pb,
-                             double f, double* out) {
-  using G = EriGradLane<LA, LB, LC, LD>;
+template <int LA, int LB, int LC, int LD, int X>
+static void quartet_grad_lane_pass(const Shell& A, const Shell& B, const Shell& C, const Shell& D,
+                                   const double* pb, double f, double (&g)[9]) {
+  using G = EriGradLanePass<LA, LB, LC, LD, X>;
+  using Q = EriGradLane<LA, LB, LC, LD>;
   double acc[G::NACC] = {0};
   const double two_pi_52 = 34.98683665524972;
   for (int i = 0; i < A.n; ++i)
@@ -387,17 +388,29 @@ static int quartet_grad_lane(const Shell& A, const Shell& B, const Shell& C, con
                                    2.0 * b, 2.0 * c);
         }
     }
-  const int nab = G::NA * G::NB, ncd = G::NC * G::ND, nac = G::NA * G::NC, nad = G::NA * G::ND,
-            nbc = G::NB * G::NC;
+  const int nab = Q::NA * Q::NB, ncd = Q::NC * Q::ND, nac = Q::NA * Q::NC, nad = Q::NA * Q::ND,
+            nbc = Q::NB * Q::NC;
   const double* Pab = pb;
   const double* Pcd = Pab + nab;
   const double* Pac = Pcd + ncd;
   const double* Pad = Pac + nac;
   const double* Pbc = Pad + nad;
   const double* Pbd = Pbc + nbc;
-  double g[9] = {0};
   G::template finish<double>(acc, A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2], C.c[0] - D.c[0],
                              C.c[1] - D.c[1], C.c[2] - D.c[2], Pab, Pcd, Pac, Pad, Pbc, Pbd, f, g);
+}
+
+// The per-lane gradient code (one pass, or one per centre) contracted with the
+// density blocks pb in the order P_ab, P_cd, P_ac, P_ad, P_bc, P_bd.
+template <int LA, int LB, int LC, int LD>
+static int quartet_grad_lane(const Shell& A, const Shell& B, const Shell& C, const Shell& D, const double* pb,
+                             double f, double* out) {
+  double g[9] = {0};
+  quartet_grad_lane_pass<LA, LB, LC, LD, 0>(A, B, C, D, pb, f, g);
+  if constexpr (EriGradLane<LA, LB, LC, LD>::NPASS == 3) {
+    quartet_grad_lane_pass<LA, LB, LC, LD, 1>(A, B, C, D, pb, f, g);
+    quartet_grad_lane_pass<LA, LB, LC, LD, 2>(A, B, C, D, pb, f, g);
+  }
   for (int j = 0; j < 9; ++j) out[j] = g[j];
   return 0;
 }

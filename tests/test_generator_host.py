@@ -168,14 +168,18 @@ def test_gradient_program_matches_finite_differences(host_lib, cls):
         assert np.abs(-got[:, :, i].sum(axis=1) - fdD[i]).max() <= 1e-8 * scale
 
 
-LANE_GRAD_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0)]
+# one-pass classes, then the three-pass ones (one program per centre)
+LANE_GRAD_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0), (2, 1, 0, 0),
+                     (1, 1, 1, 0), (2, 0, 1, 0), (2, 0, 2, 0), (2, 1, 1, 0), (2, 2, 0, 0)]
 
 
 @pytest.mark.parametrize('cls', LANE_GRAD_CLASSES, ids=lambda c: '%d%d%d%d' % c)
 def test_lane_gradient_code_matches_program(host_lib, cls):
     """The per-lane gradient code (straight-line VRR, renamed HRR, contraction
     with D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc)) equals the gradient
-    program's derivative integrals contracted with the same D_k."""
+    program's derivative integrals contracted with the same D_k (the full
+    program, itself checked against finite differences above; for the
+    three-pass classes this checks the per-centre programs against it)."""
     host_lib.host_grad_lane.restype = ctypes.c_int
     rng = np.random.default_rng(sum(v * 10 ** i for i, v in enumerate(cls)) + 53)
     centers = rng.uniform(-1.2, 1.2, (4, 3))
