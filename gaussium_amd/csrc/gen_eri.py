@@ -640,6 +640,15 @@ def emit_grad_lane_pass(la, lb, lc, ld, X, centers):
             return 'F[%d]' % (slot - 1)
         return 'v%d' % slot
 
+    # each accumulation follows the definition of the value it adds, so that
+    # a VRR value lives only until its last use
+    wname = ['', 'w1 * ', 'w2 * ', 'w3 * ']
+    acc_of = {}
+    for t, e in enumerate(g['acc']):
+        acc_of.setdefault(e & 0xffff, []).append('    acc[%d] += %s%s;' % (t, wname[e >> 16], var(e & 0xffff)))
+    for slot in sorted(acc_of):
+        if slot <= L + 1:
+            lines += acc_of[slot]
     for (x, y, z, w) in g['vrr']:
         dest, kind, ax = x & 0xffff, (x >> 16) & 1, (x >> 17) & 3
         k1, k2 = (x >> 20) & 15, (x >> 24) & 15
@@ -653,9 +662,7 @@ def emit_grad_lane_pass(la, lb, lc, ld, X, centers):
         if k2:
             terms.append('R(%d) * o2pq * %s' % (k2, var(s4)))
         lines.append('    const R v%d = %s;' % (dest, ' + '.join(terms)))
-    wname = ['', 'w1 * ', 'w2 * ', 'w3 * ']
-    for t, e in enumerate(g['acc']):
-        lines.append('    acc[%d] += %s%s;' % (t, wname[e >> 16], var(e & 0xffff)))
+        lines += acc_of.get(dest, [])
     fin = []
     cur = {t: 'acc[%d]' % t for t in range(g['NACC'])}
     cur[g['ZSLOT']] = 'R(0)'
