@@ -139,12 +139,12 @@ def gen_vrr(la, lb, lc, ld):
             for lf in range(lc, lc + ld + 1):
                 for f in cart(lf):
                     layout.append((e, f))
-    body = []
+    # each accumulation right after the value it adds is first available
     for idx, (e, f) in enumerate(layout):
         v = node(e, f, 0)
-        body.append('    acc[%d] += %s;' % (idx, v))
+        em.lines.append('    acc[%d] += %s;' % (idx, v))
     assert max(m for (_, _, m) in memo) <= L
-    return em.lines + body, layout
+    return em.lines, layout
 
 
 def gen_hrr(la, lb, lc, ld, layout):
