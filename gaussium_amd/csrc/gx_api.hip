@@ -165,6 +165,11 @@ struct gx_ctx {
   std::vector<int> wg_per_cu_grad;
   int* d_atom = nullptr;
   int* d_item_ctr = nullptr;  // per class combination: [next item, workgroups done] of the J/K, then the gradient kernels
+  // side streams for the class-combination kernels of one build (created on
+  // first use, default priority), fork and join events
+  hipStream_t side[GX_NSIDE > 0 ? GX_NSIDE : 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[GX_NSIDE > 0 ? GX_NSIDE : 1] = {};
+  bool side_ready = false;
   // ket segments: per class, runs of pairs with the same primitive-count key
   std::vector<int> seg_lo, seg_hi;
   std::vector<int> cseg;  // [class][system] first segment, NCLS x (nsys + 1)
@@ -247,6 +252,14 @@ struct gx_ctx {
     dfree(&d_Psj);
     dfree(&d_Kt2);
     dfree(&d_item_ctr);
+    if (side_ready) {
+      for (int k = 0; k < GX_NSIDE; ++k) {
+        (void)hipStreamDestroy(side[k]);
+        (void)hipEventDestroy(ev_join[k]);
+      }
+      (void)hipEventDestroy(ev_fork);
+      side_ready = false;
+    }
     dfree(&d_nxyz);
     dfree(&d_nZ);
     dfree(&d_boys);
@@ -847,7 +860,8 @@ gx::OneArgs one_args(const gx_ctx* c) {
 // The J/K kernel of class combination (X|Y) over its current work items.
 // nden = 2: a two-density (unrestricted) build in FP64: J of Ps + Ps2 into
 // Jt, K of Ps into Kt and of Ps2 into Kt2.
-hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden = 1) {
+hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden = 1, hipStream_t st = nullptr) {
+  if (!st) st = c->stream;
   if ((size_t)combo < c->jk_fp32.size() && !c->jk_fp32[combo]) mixed = 0;  // this combination runs in FP64
   if (nden == 2) mixed = 0;
   gx::JKArgs a;
@@ -879,14 +893,14 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden 
     // the cooperative kernels digest one density: run them once per density
     // (J is linear, so both launches add their part of J(P + P2) to Jt)
     const size_t lds = jk_lds_bytes(la, lb, lc, ld, c->nmax, 0);
-    hipError_t e = launch_jk(la, lb, lc, ld, c->num_cu, lds, c->stream, a, per_cu);
+    hipError_t e = launch_jk(la, lb, lc, ld, c->num_cu, lds, st, a, per_cu);
     if (e != hipSuccess) return e;
     a.P = a.Pj = c->d_Ps2;
     a.Kt = c->d_Kt2;
-    return launch_jk(la, lb, lc, ld, c->num_cu, lds, c->stream, a, per_cu);
+    return launch_jk(la, lb, lc, ld, c->num_cu, lds, st, a, per_cu);
   }
-  return launch_jk(la, lb, lc, ld, c->num_cu, jk_lds_bytes(la, lb, lc, ld, c->nmax, mixed, nden), c->stream, a,
-                   per_cu, nden);
+  return launch_jk(la, lb, lc, ld, c->num_cu, jk_lds_bytes(la, lb, lc, ld, c->nmax, mixed, nden), st, a, per_cu,
+                   nden);
 }
 
 // One-electron gradient of the shell pairs of class (la, lb): one thread per
@@ -917,7 +931,8 @@ hipError_t launch_one_grad(gx_ctx* c, int la, int lb, const gx::OneArgs& oa, int
 }
 
 // Two-electron gradient launch of one class combination (items of c->gr).
-hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad) {
+hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad, hipStream_t st = nullptr) {
+  if (!st) st = c->stream;
   if (c->gr.n_items[combo] == 0) return hipSuccess;
   if (c->wg_per_cu_grad.empty()) c->wg_per_cu_grad.assign(combo_count(), 0);
   gx::GradArgs a;
@@ -936,8 +951,38 @@ hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad) {
   a.sys_atom = c->d_sys_atom;
   a.grad = grad;
   a.item_ctr = c->d_item_ctr + 2 * combo_count() + 2 * combo;
-  return launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, c->stream, a,
+  return launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, st, a,
                      &c->wg_per_cu_grad[combo], c->nmax);
+}
+
+// Fork the context stream into the side streams (returns how many to use:
+// 0 = launch everything on the context stream, as under graph capture).
+int fork_side(gx_ctx* c) {
+  if (GX_NSIDE <= 0) return 0;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+  if (!c->side_ready) {
+    for (int k = 0; k < GX_NSIDE; ++k) {
+      if (hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) != hipSuccess) return -1;
+      if (hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) != hipSuccess) return -1;
+    }
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) return -1;
+    c->side_ready = true;
+  }
+  if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return -1;
+  for (int k = 0; k < GX_NSIDE; ++k)
+    if (hipStreamWaitEvent(c->side[k], c->ev_fork, 0) != hipSuccess) return -1;
+  return GX_NSIDE;
+}
+
+// The context stream waits for the n side streams.
+hipError_t join_side(gx_ctx* c, int n) {
+  for (int k = 0; k < n; ++k) {
+    hipError_t e = hipEventRecord(c->ev_join[k], c->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->ev_join[k], 0);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace
@@ -1451,7 +1496,12 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
     c->ev_combo.push_back(combo_id);
     return hipSuccess;
   };
-  int combo = 0;
+  // independent class combinations run on the side streams (their tails and
+  // small grids overlap); timed builds keep one stream so that each launch's
+  // events bracket it alone
+  const int nside = timing ? 0 : fork_side(c);
+  if (nside < 0) return fail(GX_ERR_HIP, "gx_jk: side streams");
+  int combo = 0, launched = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
       if (c->jk.n_items[combo] == 0) continue;
@@ -1459,9 +1509,10 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
         GX_HIP(next_events(combo));
         GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
       }
-      GX_HIP(launch_combo(c, X, Y, combo, mixed));
+      GX_HIP(launch_combo(c, X, Y, combo, mixed, 1, nside ? c->side[launched++ % nside] : c->stream));
       if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
     }
+  GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, K);
   GX_HIP(hipGetLastError());
@@ -1503,12 +1554,15 @@ int gx_jk_uhf(gx_ctx* c, const double* Pa, const double* Pb, double* J, double* 
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt2, 0.0);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, 0.0, 0));
-  int combo = 0;
+  const int nside = fork_side(c);
+  if (nside < 0) return fail(GX_ERR_HIP, "gx_jk_uhf: side streams");
+  int combo = 0, launched = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
       if (c->jk.n_items[combo] == 0) continue;
-      GX_HIP(launch_combo(c, X, Y, combo, 0, 2));
+      GX_HIP(launch_combo(c, X, Y, combo, 0, 2, nside ? c->side[launched++ % nside] : c->stream));
     }
+  GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, Ka);
   hipLaunchKernelGGL(gx::k_symmetrize1, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
@@ -1627,9 +1681,15 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
                      c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, 0.0, 0, true));
-  int combo = 0;
+  const int nside = fork_side(c);
+  if (nside < 0) return fail(GX_ERR_HIP, "gx_grad: side streams");
+  int combo = 0, launched = 0;
   for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo) GX_HIP(launch_grad_combo(c, X, Y, combo, grad));
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      if (c->gr.n_items[combo] == 0) continue;
+      GX_HIP(launch_grad_combo(c, X, Y, combo, grad, nside ? c->side[launched++ % nside] : c->stream));
+    }
+  GX_HIP(join_side(c, nside));
   return GX_OK;
 }
 

@@ -88,6 +88,12 @@ __device__ __forceinline__ double gx_rcp(double x) {
   return fma(r, e, r);
 }
 
+// Side streams for the class-combination kernels of one build: with the
+// caller's stream, the four hardware queues a process gets by default.
+#ifndef GX_NSIDE
+#define GX_NSIDE 3
+#endif
+
 #ifndef GX_RSQ_NR
 #define GX_RSQ_NR 1  // Newton steps after the hardware estimate
 #endif
