@@ -176,3 +176,36 @@ def test_mixed_autotune_precision_choice(Engine):
             assert rows[combo][5] == 0, combo
     J2, K2 = eng.jk(P, cutoff=1e-11)
     assert (J2 - J).abs().max().item() < 1e-11 and (K2 - K).abs().max().item() < 1e-11
+
+
+def test_side_streams_join_back_to_caller_stream(Engine):
+    """Builds forked onto the library's side streams are ordered on the
+    caller's stream: back-to-back builds of two densities on a non-default
+    torch stream, their results consumed there without a host sync, equal the
+    one-stream timed builds (per-launch events keep every kernel on the
+    caller's stream) of the same densities; the gradient likewise."""
+    from gaussium_amd import workloads
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    n = len(basis)
+    rng = np.random.default_rng(8)
+    P1 = torch.from_numpy(_random_density(n, rng)).cuda()
+    P2 = torch.from_numpy(_random_density(n, rng)).cuda()
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        J1, K1 = eng.jk(P1, cutoff=1e-11)
+        J2, K2 = eng.jk(P2, cutoff=1e-11)
+        d1 = (J1 - K1).sum()   # consumed on the caller's stream
+        g1 = eng.grad(P1, P2, cutoff=1e-11)
+        g1c = g1.clone()
+    s.synchronize()
+    R1, L1 = eng.jk(P1, cutoff=1e-11, timing=True)
+    R2, L2 = eng.jk(P2, cutoff=1e-11, timing=True)
+    torch.cuda.synchronize()
+    eng.jk_launch_times()
+    assert (J1 - R1).abs().max().item() < 1e-11 and (K1 - L1).abs().max().item() < 1e-11
+    assert (J2 - R2).abs().max().item() < 1e-11 and (K2 - L2).abs().max().item() < 1e-11
+    assert abs(d1.item() - (R1 - L1).sum().item()) < 1e-9
+    g0 = eng.grad(P1, P2, cutoff=1e-11)
+    assert (g1c - g0).abs().max().item() < 1e-10
