@@ -34,6 +34,18 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(GX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// As GX_HIP, inside a fork onto the side streams: on failure the caller's
+// stream is joined to the side streams before returning (so that it still
+// orders after everything already launched).
+#define GX_HIP_FORKED(x, n)                                                              \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      (void)join_side(c, (n));                                                            \
+      return fail(GX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));           \
+    }                                                                                     \
+  } while (0)
+
 #define GX_TRY(x)            \
   do {                       \
     int r_ = (x);            \
@@ -1509,7 +1521,8 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
         GX_HIP(next_events(combo));
         GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
       }
-      GX_HIP(launch_combo(c, X, Y, combo, mixed, 1, nside ? c->side[launched++ % nside] : c->stream));
+      GX_HIP_FORKED(launch_combo(c, X, Y, combo, mixed, 1, nside ? c->side[launched++ % nside] : c->stream),
+                    nside);
       if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
     }
   GX_HIP(join_side(c, nside));
@@ -1560,7 +1573,8 @@ int gx_jk_uhf(gx_ctx* c, const double* Pa, const double* Pb, double* J, double* 
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
       if (c->jk.n_items[combo] == 0) continue;
-      GX_HIP(launch_combo(c, X, Y, combo, 0, 2, nside ? c->side[launched++ % nside] : c->stream));
+      GX_HIP_FORKED(launch_combo(c, X, Y, combo, 0, 2, nside ? c->side[launched++ % nside] : c->stream),
+                    nside);
     }
   GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
@@ -1687,7 +1701,8 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
       if (c->gr.n_items[combo] == 0) continue;
-      GX_HIP(launch_grad_combo(c, X, Y, combo, grad, nside ? c->side[launched++ % nside] : c->stream));
+      GX_HIP_FORKED(launch_grad_combo(c, X, Y, combo, grad, nside ? c->side[launched++ % nside] : c->stream),
+                    nside);
     }
   GX_HIP(join_side(c, nside));
   return GX_OK;
