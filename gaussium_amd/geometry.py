@@ -27,7 +27,8 @@ def optimize(basis_file, energy_object, nuclei, method=None, gtol=3e-5, etol=1e-
         raise NotImplementedError('geometry optimisation needs the RHF gradient (method %r)'
                                   % (energy_object.method,))
     basis = read_basis_set_file(basis_file, nuclei)
-    opts = {k: v for k, v in energy_object.options.items() if k in ('schwarz_cutoff', 'max_iter')}
+    opts = {k: v for k, v in energy_object.options.items()
+            if k in ('schwarz_cutoff', 'max_iter', 'stabilizer', 'precision', 'fp32_bound')}
     surface = RHFSurface(nuclei, basis, energy_object.electrons, **opts)
     x = surface.xyz0.ravel().copy()
     e, g = surface.energy_and_gradient(x)

@@ -21,7 +21,7 @@ class RHFSurface:
     """Closed-shell RHF energy as a function of the nuclear coordinates (bohr)."""
 
     def __init__(self, nuclei, basis, electrons, engine=None, schwarz_cutoff=0.0, threshold=1e-12,
-                 density_threshold=1e-9, max_iter=300):
+                 density_threshold=1e-9, max_iter=300, stabilizer=None, precision='fp64', fp32_bound=0.0):
         self.nuclei = list(nuclei)
         self.electrons = electrons
         self.xyz0 = np.array([n.coordinates for n in self.nuclei], dtype=np.float64)
@@ -30,19 +30,21 @@ class RHFSurface:
             raise ValueError('every basis function must sit on a nucleus')
         self.engine = engine if engine is not None else Engine()
         self.options = dict(schwarz_cutoff=schwarz_cutoff, threshold=threshold,
-                            density_threshold=density_threshold, max_iter=max_iter)
+                            density_threshold=density_threshold, max_iter=max_iter, stabilizer=stabilizer,
+                            precision=precision, fp32_bound=fp32_bound)
         self.evaluations = 0
         self.last = None
 
     def _nuclei_at(self, xyz):
         return [Nuclei(n.element, n.charge, n.mass, tuple(x)) for n, x in zip(self.nuclei, xyz)]
 
-    def _hf(self, xyz):
+    def _hf(self, xyz, initial_density=None):
         xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
         nuc = self._nuclei_at(xyz)
         self.engine.set_basis(self.shells0.moved(self.xyz0, xyz))
         self.engine.set_nuclei(nuc)
-        hf = RestrictedHF(nuc, None, self.electrons, engine=self.engine, **self.options)
+        hf = RestrictedHF(nuc, None, self.electrons, engine=self.engine, initial_density=initial_density,
+                          **self.options)
         e = hf.energies()[0] + nuclear_repulsion(nuc)
         if not hf.scf_method.converged:
             raise RuntimeError('SCF did not converge in %d iterations (|dE| < %g, max |dP| < %g); the '
@@ -53,11 +55,15 @@ class RHFSurface:
         self.last = hf
         return hf, e
 
-    def energy(self, xyz):
-        return self._hf(xyz)[1]
+    def energy(self, xyz, initial_density=None):
+        """Total energy at ``xyz``.  ``initial_density`` (e.g. the converged
+        density of a nearby geometry, ``last.scf_method.density``) starts the
+        SCF from the orbitals of its Fock matrix instead of the core-H guess,
+        so that displaced geometries stay on the same SCF solution."""
+        return self._hf(xyz, initial_density)[1]
 
-    def energy_and_gradient(self, xyz):
-        hf, e = self._hf(xyz)
+    def energy_and_gradient(self, xyz, initial_density=None):
+        hf, e = self._hf(xyz, initial_density)
         return e, hf.gradient().cpu().numpy()
 
 

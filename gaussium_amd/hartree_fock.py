@@ -16,8 +16,11 @@
   on the 2N spin-blocked basis with DIIS).
 
 Options the reference does not have (``schwarz_cutoff``, ``fp32_bound``,
-``precision``, ``max_iter``, ``threshold``) default to the reference's
-behaviour: no screening, all FP64, no iteration cap, |dE| < 1e-12.
+``precision``, ``max_iter``, ``threshold``, ``stabilizer``,
+``initial_density``) default to the reference's behaviour: no screening, all
+FP64, no iteration cap, |dE| < 1e-12, the reference's DIIS, the core-H
+guess.  ``stabilizer='second_order'`` (RHF) replaces the DIIS loop by
+``scf.SecondOrderSCF`` (optimal damping, L-BFGS, trust-region Newton).
 """
 import time
 
@@ -141,7 +144,7 @@ class RestrictedHF:
 
     def __init__(self, nuclei_array, basis_set_array, electrons, symmetry=None, processes=1,
                  schwarz_cutoff=0.0, fp32_bound=0.0, precision='fp64', max_iter=None, engine=None,
-                 verbose=False, threshold=1e-12, density_threshold=None):
+                 verbose=False, threshold=1e-12, density_threshold=None, stabilizer=None, initial_density=None):
         self.nuclei_array = nuclei_array
         self.basis_set_array = basis_set_array
         self.electrons = electrons
@@ -163,7 +166,8 @@ class RestrictedHF:
         self.schwarz_cutoff = schwarz_cutoff
         self.scf_method = RestrictedSCF(self.linear_algebra, electrons, S, self.fock_builder,
                                         threshold=threshold, max_iter=max_iter,
-                                        density_threshold=density_threshold)
+                                        density_threshold=density_threshold, stabilizer=stabilizer)
+        self.initial_density = initial_density
         self.setup_time = time.perf_counter() - t0
         self._repulsion = None
 
@@ -175,7 +179,17 @@ class RestrictedHF:
         return self._repulsion
 
     def initial_guess(self):
+        """Core-Hamiltonian guess (hartree_fock.py:62-73); with ``initial_density``
+        (e.g. the converged density of a nearby geometry) the orbitals of
+        F(initial_density) instead."""
+        if self.initial_density is not None:
+            p0 = torch.as_tensor(self.initial_density, dtype=torch.float64, device=self.core_hamiltonian.device)
+            return self.linear_algebra.diagonalize(self.fock_builder.create(p0))
         return self.linear_algebra.diagonalize(self.core_hamiltonian)
+
+    @property
+    def smallest_overlap_eigenvalue(self):
+        return self.linear_algebra.smallest_overlap_eigenvalue
 
     def begin_scf(self):
         e0, c0 = self.initial_guess()

@@ -16,7 +16,10 @@ Mirrors the reference's classes and their numerical conventions:
                                  a plain solve returns arbitrary coefficients.
 * ``RestrictedSCF``              core-H guess, stop at |dE| < 1e-12
                                  (hartreefock/scf_procedure.py:17-42); ``max_iter``
-                                 adds the iteration cap the reference lacks.
+                                 adds the iteration cap the reference lacks, and
+                                 ``stabilizer='second_order'`` the opt-in
+                                 ``SecondOrderSCF`` (damping, L-BFGS, trust-region
+                                 Newton) in place of the DIIS loop.
 * ``PopleNesbetBerthier``        unrestricted SCF (scf_procedure.py:45-89)
 * ``BlockedUnrestrictedSCF``     unrestricted SCF on the spin-blocked 2N basis with
                                  ``BlockedLinearAlgebra`` and DIIS (scf_procedure.py:92-120,
@@ -175,23 +178,284 @@ class DIIS:
             return self.create_b_matrix()
 
 
+def _rotate(c, kappa, nocc):
+    """C exp(K) with K antisymmetric, K[vir, occ] = kappa, K[occ, vir] = -kappa^T."""
+    n = c.shape[0]
+    k = torch.zeros(n, n, dtype=c.dtype, device=c.device)
+    k[nocc:, :nocc] = kappa
+    k[:nocc, nocc:] = -kappa.T
+    return c @ torch.linalg.matrix_exp(k)
+
+
+def _dot(a, b):
+    return float(torch.sum(a * b))
+
+
+class SecondOrderSCF:
+    """Opt-in convergence aid for the closed-shell SCF; the reference has none
+    (its loop, hartreefock/scf_procedure.py:24-42, with the DIIS of
+    diismethod/diis.py:30-112, cycles on dense random clusters whose core-H
+    guess is far from any solution).
+
+    It minimises the reference's energy functional E(P) = 1/2 tr P (H + F(P)),
+    P = 2 C_occ C_occ^T, in three stages; every stage evaluates F through the
+    caller's Fock builder, so screening and precision are the builder's.
+
+    1. Optimal damping (Cancès and Le Bris, Int. J. Quantum Chem. 79, 82
+       (2000)) from the core-H guess: a relaxed density Pt and Ft = F(Pt)
+       (exact, F is affine in P) move towards each aufbau density by the
+       lam in [0, 1] that minimises the quadratic
+       E(Pt + lam D) = E(Pt) + lam tr(Ft D) + lam^2/2 tr((F - Ft) D), D = P - Pt;
+       Ft is diagonalised next.  Until max |e| < ``switch`` for the
+       orthonormalised commutator e = X^T (S P F - F P S) X of the aufbau
+       density, or lam = 0.
+    2. Direct minimisation over orbital rotations C exp(K),
+       K[vir, occ] = kappa: gradient dE/dkappa_ai = 4 (C_vir^T F C_occ)_ai,
+       L-BFGS (``history`` pairs) preconditioned by
+       max(4 (e_a - e_i), ``precondition_floor``), steps capped at
+       ``max_rotation`` in any element, backtracking line search on the
+       energy.  After each step the orbitals are made canonical inside the
+       occupied and the virtual block (the L-BFGS pairs are rotated with
+       them), so the preconditioner is the diagonal of the orbital Hessian.
+       Until max |dE/dkappa| < ``newton_switch``.
+    3. Trust-region Newton (Steihaug conjugate gradients, at most ``max_cg``
+       Hessian products per step, relative residual min(0.5, |g|^1/2)) with
+       the exact orbital Hessian of the exponential parametrisation:
+       (H kappa)_ai = 4 (e_a - e_i) kappa_ai + 4 (C_vir^T G(D) C_occ)_ai,
+       G(D) = F(D) - H (J - K/2 of D),
+       D = 2 (C_vir kappa C_occ^T + C_occ kappa^T C_vir^T).
+
+    Converged when |dE| < the SCF threshold between accepted iterates and
+    max |dE/dkappa| < ``gradient_threshold`` (and the SCF's density test, if
+    any).  The solution is a stationary point of the same functional as the
+    reference's fixed point (F commutes with P) and a local minimum over
+    orbital rotations; the orbitals it returns are canonical in the occupied
+    and virtual blocks.
+    """
+
+    def __init__(self, switch=0.1, newton_switch=1e-4, gradient_threshold=1e-7, history=20, max_rotation=0.3,
+                 precondition_floor=0.2, trust_radius=0.5, max_cg=40):
+        self.switch = switch
+        self.newton_switch = newton_switch
+        self.gradient_threshold = gradient_threshold
+        self.history = history
+        self.max_rotation = max_rotation
+        self.precondition_floor = precondition_floor
+        self.trust_radius = trust_radius
+        self.max_cg = max_cg
+        self.stage_iterations = {'damping': 0, 'lbfgs': 0, 'newton': 0}
+        self.hessian_products = 0
+
+    # -- pieces ---------------------------------------------------------------
+    def _evaluate(self, scf, c):
+        p = density_matrix_restricted(c, scf.electrons)
+        f = scf.hamiltonian_matrix_factory.create(p)
+        e = scf.calculate.restricted(p, f)
+        scf.iterations += 1
+        scf.energy_history.append(e)
+        return p, f, e
+
+    def _canonical(self, c, f, nocc):
+        fmo = c.T @ f @ c
+        eo, uo = torch.linalg.eigh(fmo[:nocc, :nocc])
+        ev, uv = torch.linalg.eigh(fmo[nocc:, nocc:])
+        return torch.cat([c[:, :nocc] @ uo, c[:, nocc:] @ uv], dim=1), eo, ev, uo, uv
+
+    @staticmethod
+    def _gradient(c, f, nocc):
+        return 4.0 * (c[:, nocc:].T @ f @ c[:, :nocc])
+
+    # -- driver -----------------------------------------------------------------
+    def run(self, scf, orbital_coefficients):
+        nocc = scf.electrons // 2
+        la = scf.linear_algebra
+        s = scf.overlap
+        cap = scf.max_iter
+
+        def out_of_budget():
+            return cap is not None and scf.iterations >= cap
+
+        # stage 1: optimal damping
+        c = orbital_coefficients
+        relaxed = None
+        while True:
+            p, f, e = self._evaluate(scf, c)
+            self.stage_iterations['damping'] += 1
+            lam = 1.0
+            if relaxed is None:
+                relaxed = (p, f)
+            else:
+                pt, ft = relaxed
+                d = p - pt
+                slope = _dot(ft, d)
+                curv = _dot(f - ft, d)
+                lam = 1.0 if curv <= -slope else -slope / curv
+                lam = min(max(lam, 0.0), 1.0)
+                relaxed = (pt + lam * d, ft + lam * (f - ft))
+            err = float(la.orthonormalize(s @ p @ f - f @ p @ s).abs().max())
+            if err < self.switch or lam == 0.0 or out_of_budget():
+                break
+            _, c = la.diagonalize(relaxed[1])
+        relaxed = None
+
+        c, eo, ev, _, _ = self._canonical(c, f, nocc)
+        g = self._gradient(c, f, nocc)
+        pairs = []
+        stage = 'lbfgs'
+        converged = False
+        delta = self.trust_radius
+        while not out_of_budget():
+            gmax = float(g.abs().max())
+            if stage == 'lbfgs' and gmax < self.newton_switch:
+                stage = 'newton'
+            diag = 4.0 * (ev[:, None] - eo[None, :])
+            prec = torch.clamp(diag, min=self.precondition_floor)
+            if stage == 'lbfgs':
+                cn, pn, fn, en, step = self._lbfgs_step(scf, c, e, g, prec, pairs, nocc)
+                self.stage_iterations['lbfgs'] += 1
+                accepted = True
+            else:
+                cn, pn, fn, en, accepted, delta = self._newton_step(scf, c, e, g, diag, prec, delta, nocc)
+                self.stage_iterations['newton'] += 1
+            if not accepted:
+                continue
+            gn = self._gradient(cn, fn, nocc)
+            if stage == 'lbfgs':
+                y = gn - g
+                sy = _dot(step, y)
+                if sy > 1e-14:
+                    pairs.append((step, y, 1.0 / sy))
+                    if len(pairs) > self.history:
+                        pairs.pop(0)
+            de = e - en
+            dp = float((pn - p).abs().max())
+            c, eo, ev, uo, uv = self._canonical(cn, fn, nocc)
+            g = uv.T @ gn @ uo
+            pairs = [(uv.T @ a @ uo, uv.T @ b @ uo, r) for a, b, r in pairs]
+            p, f, e = pn, fn, en
+            if abs(de) < scf.threshold and float(g.abs().max()) < self.gradient_threshold and \
+                    (scf.density_threshold is None or dp < scf.density_threshold):
+                converged = True
+                break
+        return converged, p, f, e, torch.cat([eo, ev]), c
+
+    def _lbfgs_step(self, scf, c, e, g, prec, pairs, nocc):
+        q = g.clone()
+        alphas = []
+        for sv, yv, rho in reversed(pairs):
+            a = rho * _dot(sv, q)
+            q = q - a * yv
+            alphas.append(a)
+        r = q / prec
+        for (sv, yv, rho), a in zip(pairs, reversed(alphas)):
+            b = rho * _dot(yv, r)
+            r = r + (a - b) * sv
+        d = -r
+        gd = _dot(g, d)
+        if gd >= 0.0:  # not a descent direction: forget the pairs
+            pairs.clear()
+            d = -g / prec
+            gd = _dot(g, d)
+        m = float(d.abs().max())
+        if m > self.max_rotation:
+            d = d * (self.max_rotation / m)
+            gd *= self.max_rotation / m
+        t = 1.0
+        for _ in range(12):
+            cn = _rotate(c, t * d, nocc)
+            pn, fn, en = self._evaluate(scf, cn)
+            if en <= e + 1e-4 * t * gd + 1e-11:
+                break
+            tq = -gd * t * t / (2.0 * (en - e - gd * t))
+            t = min(0.5 * t, max(0.1 * t, tq))
+        return cn, pn, fn, en, t * d
+
+    def _newton_step(self, scf, c, e, g, diag, prec, delta, nocc):
+        co, cv = c[:, :nocc], c[:, nocc:]
+        h = scf.hamiltonian_matrix_factory.core_hamiltonian
+
+        def hess(k):
+            d = 2.0 * (cv @ k @ co.T)
+            self.hessian_products += 1
+            return diag * k + 4.0 * (cv.T @ (scf.hamiltonian_matrix_factory.create(d + d.T) - h) @ co)
+
+        def to_boundary(x, d):
+            xx, xd, dd = _dot(x, x), _dot(x, d), _dot(d, d)
+            return x + ((-xd + (xd * xd + dd * (delta * delta - xx)) ** 0.5) / dd) * d
+
+        gnorm = float(g.norm())
+        eta = min(0.5, gnorm ** 0.5)
+        x = torch.zeros_like(g)
+        r = g.clone()
+        z = r / prec
+        d = -z
+        rz = _dot(r, z)
+        hit = False
+        for _ in range(self.max_cg):
+            hd = hess(d)
+            dhd = _dot(d, hd)
+            if dhd <= 0.0:
+                x, hit = to_boundary(x, d), True
+                break
+            alpha = rz / dhd
+            xn = x + alpha * d
+            if float(xn.norm()) >= delta:
+                x, hit = to_boundary(x, d), True
+                break
+            x = xn
+            r = r + alpha * hd
+            if float(r.norm()) < eta * gnorm:
+                break
+            z = r / prec
+            rzn = _dot(r, z)
+            d = -z + (rzn / rz) * d
+            rz = rzn
+        predicted = _dot(g, x) + 0.5 * _dot(x, hess(x))
+        cn = _rotate(c, x, nocc)
+        pn, fn, en = self._evaluate(scf, cn)
+        de = en - e
+        rho = de / predicted if predicted < -1e-11 else 1.0
+        if rho < 0.25:
+            delta *= 0.25
+        elif rho > 0.75 and hit:
+            delta = min(2.0 * delta, 2.0)
+        return cn, pn, fn, en, de < 1e-11, delta
+
+
 class RestrictedSCF:
     def __init__(self, linear_algebra, electrons, overlap, hamiltonian_matrix_factory, threshold=1e-12,
-                 max_iter=None, use_diis=True, density_threshold=None):
+                 max_iter=None, use_diis=True, density_threshold=None, stabilizer=None):
         self.linear_algebra = linear_algebra
         self.electrons = electrons
+        self.overlap = overlap
         self.hamiltonian_matrix_factory = hamiltonian_matrix_factory
         self.threshold = threshold
         self.max_iter = max_iter
         self.calculate = TotalEnergy(hamiltonian_matrix_factory.core_hamiltonian)
-        self.diis = DIIS(overlap, linear_algebra) if use_diis else None
+        # ``stabilizer``: None keeps the reference's DIIS loop; 'second_order'
+        # (or a SecondOrderSCF) replaces it
+        if stabilizer == 'second_order':
+            stabilizer = SecondOrderSCF()
+        elif stabilizer is not None and not isinstance(stabilizer, SecondOrderSCF):
+            raise ValueError('unknown SCF stabilizer %r (None or "second_order")' % (stabilizer,))
+        self.stabilizer = stabilizer
+        self.diis = DIIS(overlap, linear_algebra) if use_diis and stabilizer is None else None
         self.iterations = 0
         self.energy_history = []
+        self.converged = False
         # optional second stop test, max |P_n - P_{n-1}| below this (gradients
         # are first order in the density error, energies second order)
         self.density_threshold = density_threshold
 
     def begin_iterations(self, orbital_energies, orbital_coefficients):
+        if self.stabilizer is not None:
+            self.converged, density, fock, total, orbital_energies, orbital_coefficients = \
+                self.stabilizer.run(self, orbital_coefficients)
+            self.density = density
+            self.fock = fock
+            self.orbital_energies = orbital_energies
+            self.orbital_coefficients = orbital_coefficients
+            return total, orbital_energies, orbital_coefficients
         previous = 0.0
         previous_density = None
         while True:
@@ -220,7 +484,6 @@ class RestrictedSCF:
         self.orbital_energies = orbital_energies
         self.orbital_coefficients = orbital_coefficients
         return total, orbital_energies, orbital_coefficients
-
 
 class PopleNesbetBerthier:
     """Unrestricted SCF (hartreefock/scf_procedure.py:45-89): n_a = (n + m - 1) / 2,

@@ -127,15 +127,216 @@ class DIIS:
             return self._coefficients()
 
 
-def rhf(h, s, electrons, fock_builder, threshold=1e-12, max_iter=None, diis_cond_limit=None):
+def _rotate(c, kappa, nocc):
+    """C exp(K), K antisymmetric with K[vir, occ] = kappa."""
+    from scipy.linalg import expm
+    n = c.shape[0]
+    k = np.zeros((n, n))
+    k[nocc:, :nocc] = kappa
+    k[:nocc, nocc:] = -kappa.T
+    return c @ expm(k)
+
+
+class SecondOrderSCF:
+    """Opt-in SCF convergence aid; the reference has none (its loop is
+    scf_procedure.py:24-42 with the DIIS of diis.py:30-112).  Minimises
+    E(P) = 1/2 tr P (H + F(P)), P = 2 C_occ C_occ^T, in three stages:
+
+    1. optimal damping (Cancès and Le Bris, Int. J. Quantum Chem. 79, 82
+       (2000)) from the core-H guess until max |X^T (S P F - F P S) X| <
+       ``switch`` or lam = 0;
+    2. L-BFGS over rotations C exp(K) (gradient 4 C_vir^T F C_occ,
+       preconditioner max(4 (e_a - e_i), floor), max-element step cap,
+       backtracking line search, canonical orbitals per block after each
+       step) until max |g| < ``newton_switch``;
+    3. trust-region Newton with Steihaug CG and the exact orbital Hessian
+       4 (e_a - e_i) kappa + 4 C_vir^T G(D) C_occ, G(D) = F(D) - H,
+       D = 2 (C_vir kappa C_occ^T + transpose).
+    Converged when |dE| < threshold and max |g| < ``gradient_threshold``.
+    """
+
+    def __init__(self, switch=0.1, newton_switch=1e-4, gradient_threshold=1e-7, history=20, max_rotation=0.3,
+                 precondition_floor=0.2, trust_radius=0.5, max_cg=40):
+        self.switch = switch
+        self.newton_switch = newton_switch
+        self.gradient_threshold = gradient_threshold
+        self.history = history
+        self.max_rotation = max_rotation
+        self.floor = precondition_floor
+        self.trust_radius = trust_radius
+        self.max_cg = max_cg
+        self.evaluations = 0
+
+    def run(self, h, s, la, electrons, fock_builder, c, threshold, max_iter):
+        nocc = electrons // 2
+
+        def evaluate(c):
+            p = density_matrix_restricted(c, electrons)
+            f = fock_builder(p)
+            self.evaluations += 1
+            return p, f, total_energy_restricted(h, p, f)
+
+        def canonical(c, f):
+            fmo = c.T @ f @ c
+            eo, uo = np.linalg.eigh(fmo[:nocc, :nocc])
+            ev, uv = np.linalg.eigh(fmo[nocc:, nocc:])
+            return np.hstack([c[:, :nocc] @ uo, c[:, nocc:] @ uv]), eo, ev, uo, uv
+
+        def grad(c, f):
+            return 4.0 * (c[:, nocc:].T @ f @ c[:, :nocc])
+
+        def budget():
+            return max_iter is not None and self.evaluations >= max_iter
+
+        relaxed = None
+        while True:
+            p, f, e = evaluate(c)
+            lam = 1.0
+            if relaxed is None:
+                relaxed = (p, f)
+            else:
+                pt, ft = relaxed
+                d = p - pt
+                slope = float(np.sum(ft * d))
+                curv = float(np.sum((f - ft) * d))
+                lam = 1.0 if curv <= -slope else -slope / curv
+                lam = min(max(lam, 0.0), 1.0)
+                relaxed = (pt + lam * d, ft + lam * (f - ft))
+            err = np.abs(la.orthonormalize(s @ p @ f - f @ p @ s)).max()
+            if err < self.switch or lam == 0.0 or budget():
+                break
+            _, c = la.diagonalize(relaxed[1])
+
+        c, eo, ev, _, _ = canonical(c, f)
+        g = grad(c, f)
+        pairs = []
+        stage = 'lbfgs'
+        delta = self.trust_radius
+        converged = False
+        while not budget():
+            if stage == 'lbfgs' and np.abs(g).max() < self.newton_switch:
+                stage = 'newton'
+            diag = 4.0 * (ev[:, None] - eo[None, :])
+            prec = np.maximum(diag, self.floor)
+            if stage == 'lbfgs':
+                q = g.copy()
+                alphas = []
+                for sv, yv, rho in reversed(pairs):
+                    a = rho * np.sum(sv * q)
+                    q = q - a * yv
+                    alphas.append(a)
+                r = q / prec
+                for (sv, yv, rho), a in zip(pairs, reversed(alphas)):
+                    r = r + (a - rho * np.sum(yv * r)) * sv
+                d = -r
+                gd = np.sum(g * d)
+                if gd >= 0.0:
+                    pairs = []
+                    d = -g / prec
+                    gd = np.sum(g * d)
+                m = np.abs(d).max()
+                if m > self.max_rotation:
+                    d = d * (self.max_rotation / m)
+                    gd *= self.max_rotation / m
+                t = 1.0
+                for _ in range(12):
+                    cn = _rotate(c, t * d, nocc)
+                    pn, fn, en = evaluate(cn)
+                    if en <= e + 1e-4 * t * gd + 1e-11:
+                        break
+                    tq = -gd * t * t / (2.0 * (en - e - gd * t))
+                    t = min(0.5 * t, max(0.1 * t, tq))
+                step = t * d
+            else:
+                co, cv = c[:, :nocc], c[:, nocc:]
+
+                def hess(k):
+                    dd = 2.0 * (cv @ k @ co.T)
+                    return diag * k + 4.0 * (cv.T @ (fock_builder(dd + dd.T) - h) @ co)
+
+                def to_boundary(x, d):
+                    xx, xd, dd_ = np.sum(x * x), np.sum(x * d), np.sum(d * d)
+                    return x + ((-xd + np.sqrt(xd * xd + dd_ * (delta * delta - xx))) / dd_) * d
+
+                gnorm = np.linalg.norm(g)
+                eta = min(0.5, np.sqrt(gnorm))
+                x = np.zeros_like(g)
+                r = g.copy()
+                z = r / prec
+                d = -z
+                rz = np.sum(r * z)
+                hit = False
+                for _ in range(self.max_cg):
+                    hd = hess(d)
+                    dhd = np.sum(d * hd)
+                    if dhd <= 0.0:
+                        x, hit = to_boundary(x, d), True
+                        break
+                    alpha = rz / dhd
+                    xn = x + alpha * d
+                    if np.linalg.norm(xn) >= delta:
+                        x, hit = to_boundary(x, d), True
+                        break
+                    x = xn
+                    r = r + alpha * hd
+                    if np.linalg.norm(r) < eta * gnorm:
+                        break
+                    z = r / prec
+                    rzn = np.sum(r * z)
+                    d = -z + (rzn / rz) * d
+                    rz = rzn
+                predicted = np.sum(g * x) + 0.5 * np.sum(x * hess(x))
+                cn = _rotate(c, x, nocc)
+                pn, fn, en = evaluate(cn)
+                rho = (en - e) / predicted if predicted < -1e-11 else 1.0
+                if rho < 0.25:
+                    delta *= 0.25
+                elif rho > 0.75 and hit:
+                    delta = min(2.0 * delta, 2.0)
+                if not en - e < 1e-11:
+                    continue
+            gn = grad(cn, fn)
+            if stage == 'lbfgs':
+                y = gn - g
+                sy = np.sum(step * y)
+                if sy > 1e-14:
+                    pairs.append((step, y, 1.0 / sy))
+                    if len(pairs) > self.history:
+                        pairs.pop(0)
+            de = e - en
+            c, eo, ev, uo, uv = canonical(cn, fn)
+            g = uv.T @ gn @ uo
+            pairs = [(uv.T @ a @ uo, uv.T @ b @ uo, rr) for a, b, rr in pairs]
+            p, f, e = pn, fn, en
+            if abs(de) < threshold and np.abs(g).max() < self.gradient_threshold:
+                converged = True
+                break
+        return converged, e, np.concatenate([eo, ev]), c
+
+
+def rhf(h, s, electrons, fock_builder, threshold=1e-12, max_iter=None, diis_cond_limit=None, stabilizer=None,
+        initial_density=None):
     """Restricted SCF (scf_procedure.py:24-42, hartree_fock.py:45-73).
 
     ``fock_builder(P) -> F``.  Returns (E_elec, orbital energies, C, iterations).
     ``max_iter`` is None for the reference behaviour (no cap).
+    ``stabilizer='second_order'`` runs ``SecondOrderSCF`` instead of the
+    reference's DIIS loop (an option the reference does not have; the
+    iteration count is then its number of energy evaluations);
+    ``initial_density`` starts from the orbitals of F(initial_density)
+    instead of the core-H guess.
     """
     la = LinearAlgebra(s)
+    eps, c = la.diagonalize(h if initial_density is None else fock_builder(initial_density))
+    if stabilizer == 'second_order':
+        so = SecondOrderSCF()
+        converged, e, eps, c = so.run(h, s, la, electrons, fock_builder, c, threshold, max_iter)
+        if not converged:
+            raise RuntimeError('second-order SCF did not converge in %d evaluations' % so.evaluations)
+        return e, eps, c, so.evaluations
+    if stabilizer is not None:
+        raise ValueError(stabilizer)
     diis = DIIS(s, la, cond_limit=diis_cond_limit)
-    eps, c = la.diagonalize(h)
     prev = 0.0
     it = 0
     while True:

@@ -42,7 +42,8 @@ def load_system(mol, gbs):
     return nuc, basis, ne
 
 
-def oracle_rhf(nuclei, basis, electrons, eri=None, max_iter=None, exact_boys=True, diis_cond_limit=1e12):
+def oracle_rhf(nuclei, basis, electrons, eri=None, max_iter=None, exact_boys=True, diis_cond_limit=1e12,
+               stabilizer=None):
     """Oracle RHF total energy from the C-oracle dense tensor and the oracle S/T/V.
 
     ``exact_boys`` evaluates the Boys function exactly instead of with the
@@ -69,7 +70,8 @@ def oracle_rhf(nuclei, basis, electrons, eri=None, max_iter=None, exact_boys=Tru
         J, K = jk_from_dense(eri, P)
         return H + J - 0.5 * K
 
-    e, eps, c, it = rhf(H, S, electrons, fb, max_iter=max_iter, diis_cond_limit=diis_cond_limit)
+    e, eps, c, it = rhf(H, S, electrons, fb, max_iter=max_iter, diis_cond_limit=diis_cond_limit,
+                        stabilizer=stabilizer)
     return e + nuclear_repulsion(onuc), it
 
 

@@ -9,8 +9,9 @@
 * The 66-function cluster (all 36 coordinates) against the same difference
   built from the GPU energy kernels; bar 1e-7.
 * The converged gradient against central differences of GPU SCF energies
-  (h = 1e-4) on C2H4/3-21G; bar 1e-6 Eh/bohr.  (The seeded random clusters
-  are not used here: the reference's DIIS does not converge on them.)
+  (h = 1e-4) on C2H4/3-21G with the reference's DIIS loop; bar 1e-6 Eh/bohr.
+  (The seeded random clusters, on which that loop cycles, are checked with
+  the second-order SCF in test_gpu_scf.py.)
 * Translational and rotational invariance; torch.autograd through RHFEnergy.
 """
 import numpy as np

@@ -183,3 +183,91 @@ def test_unrestricted_scf_host_logic_matches_oracle():
     e_ref, it = uhf(H, S, ne, mult, fb)
     assert abs(e - e_ref) < 1e-10, (e, e_ref)
     np.testing.assert_approx_equal(e + e_nuc, -147.634028141, 6)
+
+
+def _c2h4_oracle_matrices():
+    from oracle import eri_c
+    from oracle.basis import Nuclei, nuclear_repulsion
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    from tests.helpers import load_system
+    nuc, basis, ne = load_system('C2H4.mol', '3-21G.gbs')
+    nuc = [Nuclei(n.element, n.charge, n.mass, n.coordinates) for n in nuc]
+    with eri_c.exact_boys():
+        eri = eri_c.eri_dense(basis)
+    S = overlap_matrix(basis)
+    H = kinetic_matrix(basis) + nuclear_matrix(basis, nuc)
+    return ne, eri, S, H, nuclear_repulsion(nuc)
+
+
+class _OracleFock:
+    """The product's Fock-builder interface over the oracle's dense sums."""
+
+    def __init__(self, eri, H):
+        import torch
+        self.eri = eri
+        self.H = H
+        self.core_hamiltonian = torch.from_numpy(H)
+        self.builds = 0
+
+    def create(self, p):
+        import torch
+        from oracle.scf import jk_from_dense
+        self.builds += 1
+        J, K = jk_from_dense(self.eri, p.numpy())
+        return torch.from_numpy(self.H + J - 0.5 * K)
+
+
+def test_second_order_scf_host_logic_matches_oracle():
+    """The device-side SecondOrderSCF (torch; CPU tensors here) driven by the
+    oracle's dense Fock sums reaches the oracle's second-order result and the
+    reference's published C2H4/3-21G energy."""
+    import torch
+    from gaussium_amd.scf import LinearAlgebra, RestrictedSCF
+    from oracle.scf import jk_from_dense, rhf
+    ne, eri, S, H, e_nuc = _c2h4_oracle_matrices()
+    fock = _OracleFock(eri, H)
+    la = LinearAlgebra(torch.from_numpy(S))
+    scf = RestrictedSCF(la, ne, torch.from_numpy(S), fock, stabilizer='second_order')
+    e, eps, c = scf.begin_iterations(*la.diagonalize(fock.core_hamiltonian))
+    assert scf.converged
+    e_ref, _, _, it = rhf(H, S, ne, lambda p: H + jk_from_dense(eri, p)[0] - 0.5 * jk_from_dense(eri, p)[1],
+                          stabilizer='second_order')
+    assert abs(e - e_ref) < 1e-10, (e, e_ref)
+    np.testing.assert_approx_equal(e + e_nuc, -77.600460844, 6)
+    # W from the returned canonical orbitals equals P F P / 2 at the solution
+    no = ne // 2
+    co = c[:, :no]
+    W = 2.0 * (co * eps[:no]) @ co.T
+    P = scf.density
+    assert torch.allclose(W, 0.5 * P @ scf.fock @ P, atol=1e-7)
+
+
+def test_second_order_hessian_matches_gradient_differences():
+    """The Newton stage's orbital Hessian-vector product equals the central
+    difference of the orbital gradient 4 C_vir^T F C_occ along C exp(t K)."""
+    import torch
+    from gaussium_amd.scf import LinearAlgebra, RestrictedSCF, SecondOrderSCF, _rotate
+    ne, eri, S, H, e_nuc = _c2h4_oracle_matrices()
+    fock = _OracleFock(eri, H)
+    la = LinearAlgebra(torch.from_numpy(S))
+    scf = RestrictedSCF(la, ne, torch.from_numpy(S), fock, max_iter=6)
+    scf.begin_iterations(*la.diagonalize(fock.core_hamiltonian))  # a non-stationary point
+    no = ne // 2
+    so = SecondOrderSCF()
+    c, eo, ev, _, _ = so._canonical(scf.orbital_coefficients, scf.fock, no)
+    co, cv = c[:, :no], c[:, no:]
+    diag = 4.0 * (ev[:, None] - eo[None, :])
+    k = torch.from_numpy(np.random.default_rng(0).standard_normal(diag.shape))
+    d = 2.0 * (cv @ k @ co.T)
+    hk = diag * k + 4.0 * (cv.T @ (fock.create(d + d.T) - fock.core_hamiltonian) @ co)
+
+    def grad_at(t):
+        # the gradient in the frame of the rotated orbitals, whose t-derivative is H k
+        ct = _rotate(c, t * k, no)
+        return so._gradient(ct, fock.create(2.0 * ct[:, :no] @ ct[:, :no].T), no)
+
+    h = 1e-4
+    fd = (grad_at(h) - grad_at(-h)) / (2 * h)
+    err = (fd - hk).abs().max()
+    print('Hessian-vector product vs difference of gradients: %.2e (|Hk| %.2e)' % (float(err), float(hk.abs().max())))
+    assert float(err) < 1e-6 * max(1.0, float(hk.abs().max()))

@@ -431,3 +431,73 @@ def test_oracle_mo_transform_matches_einsum():
     c = rng.standard_normal((n, n))
     ref = np.einsum('mp,nq,lr,st,mnls->pqrt', c, c, c, c, eri)
     assert np.abs(molecular_orbitals(eri, c) - ref).max() < 1e-10
+
+
+@pytest.mark.parametrize('system,published', [  # tests/testslow/start/test_start.py:9-25
+    (('HeH+.mol', '6-311+GPP.gbs'), -2.92922773384),
+    (('C2H4.mol', '3-21G.gbs'), -77.600460844),
+])
+def test_oracle_second_order_scf_reaches_published_energies(system, published):
+    """Where the reference's own DIIS loop converges, the opt-in second-order
+    SCF lands on the same solution: the published energy and the DIIS
+    oracle's value within 1e-9 Eh."""
+    from tests.helpers import oracle_rhf
+    nuc, basis, ne = load_system(*system)
+    e, _ = oracle_rhf(nuc, basis, ne, stabilizer='second_order')
+    np.testing.assert_approx_equal(e, published, 6)
+    e_diis, _ = oracle_rhf(nuc, basis, ne)
+    assert abs(e - e_diis) < 1e-9
+
+
+def _cluster_66bf_oracle():
+    from gaussium_amd import workloads
+    from oracle import boys as oboys
+    from oracle import eri_c
+    from oracle.basis import Nuclei, nuclear_repulsion
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    onuc = [Nuclei(n.element, n.charge, n.mass, n.coordinates) for n in nuc]
+    with eri_c.exact_boys():
+        eri = eri_c.eri_dense(basis)
+    S = overlap_matrix(basis)
+    with oboys.exact(True):
+        H = kinetic_matrix(basis) + nuclear_matrix(basis, onuc)
+    return eri, S, H, ne, nuclear_repulsion(onuc)
+
+
+@pytest.mark.slow
+def test_oracle_second_order_scf_cluster_66bf_is_a_stable_solution():
+    """On the 66-function cluster, where the reference's DIIS cycles, the
+    second-order SCF ends at an aufbau solution of the Roothaan equations
+    (F, rebuilt from the final density by the dense contraction, commutes
+    with P; the occupied orbitals are the lowest) that is a minimum over
+    orbital rotations: the explicit orbital Hessian
+    4 (e_a - e_i) d_ab d_ij + 4 [4 (ai|bj) - (ab|ij) - (aj|bi)] in the MO
+    basis is positive definite."""
+    from oracle.scf import LinearAlgebra, density_matrix_restricted, jk_from_dense, rhf
+    eri, S, H, ne, e_nuc = _cluster_66bf_oracle()
+
+    def fb(P):
+        J, K = jk_from_dense(eri, P)
+        return H + J - 0.5 * K
+
+    e, eps, c, it = rhf(H, S, ne, fb, stabilizer='second_order', max_iter=3000)
+    no = ne // 2
+    P = density_matrix_restricted(c, ne)
+    F = fb(P)
+    comm = LinearAlgebra(S).orthonormalize(S @ P @ F - F @ P @ S)
+    assert np.abs(comm).max() < 1e-6
+    fmo = c.T @ F @ c
+    e_all = np.linalg.eigvalsh(fmo)
+    assert np.allclose(np.sort(np.diag(fmo)[:no]), e_all[:no], atol=1e-6)  # aufbau
+    assert e_all[no] - e_all[no - 1] > 1e-2
+    mo = np.einsum('pi,qj,rk,sl,pqrs->ijkl', c, c, c, c, eri, optimize=True)
+    o, v = slice(0, no), slice(no, c.shape[0])
+    nv = c.shape[0] - no
+    hess = 4.0 * (4.0 * np.einsum('aibj->aibj', mo[v, o, v, o]) - np.einsum('abij->aibj', mo[v, v, o, o])
+                  - np.einsum('ajbi->aibj', mo[v, o, v, o]))
+    diag = 4.0 * (np.diag(fmo)[no:, None] - np.diag(fmo)[None, :no])
+    hess = hess.reshape(nv * no, nv * no) + np.diag(diag.ravel())
+    assert np.linalg.eigvalsh(hess)[0] > 0.0
+    print('66 bf: E %.10f, %d evaluations, HOMO-LUMO %.4f, lowest Hessian eigenvalue %.3e' % (
+        e + e_nuc, it, e_all[no] - e_all[no - 1], np.linalg.eigvalsh(hess)[0]))
