@@ -213,6 +213,9 @@ struct gx_ctx {
   // pairs (ev[2i], ev[2i+1]) around the launch of class combo ev_combo[i]
   std::vector<hipEvent_t> ev;
   std::vector<int> ev_combo;
+  // the same for gx_grad_ex calls made with GX_JK_TIMING
+  std::vector<hipEvent_t> gev;
+  std::vector<int> gev_combo;
 
   gx::ShellView shells() const { return gx::ShellView{d_l, d_bf, d_ctr, d_lbf, d_ssys}; }
   gx::SysView systems() const { return gx::SysView{d_sn, d_smb}; }
@@ -264,14 +267,7 @@ struct gx_ctx {
     dfree(&d_Psj);
     dfree(&d_Kt2);
     dfree(&d_item_ctr);
-    if (side_ready) {
-      for (int k = 0; k < GX_NSIDE; ++k) {
-        (void)hipStreamDestroy(side[k]);
-        (void)hipEventDestroy(ev_join[k]);
-      }
-      (void)hipEventDestroy(ev_fork);
-      side_ready = false;
-    }
+    release_side();
     dfree(&d_nxyz);
     dfree(&d_nZ);
     dfree(&d_boys);
@@ -280,6 +276,22 @@ struct gx_ctx {
     for (auto& e : ev) (void)hipEventDestroy(e);
     ev.clear();
     ev_combo.clear();
+    for (auto& e : gev) (void)hipEventDestroy(e);
+    gev.clear();
+    gev_combo.clear();
+  }
+  // Destroys whichever side streams and events exist (also after a
+  // set-up that failed part way).
+  void release_side() {
+    for (int k = 0; k < (GX_NSIDE > 0 ? GX_NSIDE : 1); ++k) {
+      if (side[k]) (void)hipStreamDestroy(side[k]);
+      if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+      side[k] = nullptr;
+      ev_join[k] = nullptr;
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    ev_fork = nullptr;
+    side_ready = false;
   }
 };
 
@@ -437,12 +449,14 @@ size_t grad_lds_bytes(int la, int lb, int lc, int ld, int N) {
   return 0;
 }
 
+// mixed = 1 (per-lane classes only): kets below the FP32 bound run their
+// primitive loops in FP32 (k_grad_lane<..., true>).
 hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s, const gx::GradArgs& a,
-                       int* per_cu, int N) {
+                       int* per_cu, int N, int mixed) {
   const size_t lds = grad_lds_bytes(la, lb, lc, ld, N);
-#define GX_CASE(A, B, C, D)                                                                   \
-  if (la == A && lb == B && lc == C && ld == D) {                                             \
-    const void* fn = (const void*)gx::k_grad_lane<A, B, C, D>;                               \
+#define GX_LANE(A, B, C, D, M)                                                                \
+  {                                                                                           \
+    const void* fn = (const void*)gx::k_grad_lane<A, B, C, D, M>;                            \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
     if (*per_cu <= 0) {                                                                       \
@@ -451,11 +465,16 @@ hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s
       if (*per_cu < 1) *per_cu = 1;                                                           \
     }                                                                                         \
     const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
-    hipLaunchKernelGGL((gx::k_grad_lane<A, B, C, D>), dim3(grid), dim3(JK_TEAM), lds, s, a); \
+    hipLaunchKernelGGL((gx::k_grad_lane<A, B, C, D, M>), dim3(grid), dim3(JK_TEAM), lds, s, a); \
     return hipGetLastError();                                                                 \
+  }
+#define GX_CASE(A, B, C, D)                                   \
+  if (la == A && lb == B && lc == C && ld == D) {             \
+    if (mixed) GX_LANE(A, B, C, D, true) else GX_LANE(A, B, C, D, false) \
   }
   GX_FOR_EACH_GRAD_LANE_CLASS(GX_CASE)
 #undef GX_CASE
+#undef GX_LANE
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
     const void* fn = (const void*)gx::k_grad_coop<A, B, C, D>;                               \
@@ -805,7 +824,7 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
 
 int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = false) {
   gx_ctx::WorkSet& ws = grad ? c->gr : c->jk;
-  if (grad) fp32 = 0.0, mixed = 0;
+  if (!mixed) fp32 = 0.0;
   if (cutoff == ws.cutoff && fp32 == ws.fp32 && mixed == ws.mixed && !ws.d_items.empty()) return GX_OK;
   ws.release();
   ws.d_items.assign(combo_count(), nullptr);
@@ -815,8 +834,13 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = f
   if ((int)c->jk_fp32.size() != combo_count()) c->jk_fp32.assign(combo_count(), 1);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo)
-      GX_TRY(build_combo(c, ws, X, Y, combo, cutoff, fp32, mixed, grad, ws.ipw[combo + mixed * combo_count()]));
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      // gradients: the cooperative classes stay FP64; every mode uses the
+      // item size tuned by gx_grad_autotune
+      const int m = grad ? (mixed && gx_grad_is_lane(class_la(X), class_lb(X), class_la(Y), class_lb(Y))) : mixed;
+      GX_TRY(build_combo(c, ws, X, Y, combo, cutoff, m ? fp32 : 0.0, m, grad,
+                         ws.ipw[combo + (grad ? 0 : mixed) * combo_count()]));
+    }
   ws.cutoff = cutoff;
   ws.fp32 = fp32;
   ws.mixed = mixed;
@@ -946,7 +970,8 @@ hipError_t launch_one_grad(gx_ctx* c, int la, int lb, const gx::OneArgs& oa, int
 hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad, hipStream_t st = nullptr) {
   if (!st) st = c->stream;
   if (c->gr.n_items[combo] == 0) return hipSuccess;
-  if (c->wg_per_cu_grad.empty()) c->wg_per_cu_grad.assign(combo_count(), 0);
+  if (c->wg_per_cu_grad.empty()) c->wg_per_cu_grad.assign(2 * combo_count(), 0);
+  const int mixed = (c->gr.mixed && gx_grad_is_lane(class_la(X), class_lb(X), class_la(Y), class_lb(Y))) ? 1 : 0;
   gx::GradArgs a;
   a.sh = c->shells();
   a.pr = c->pairs();
@@ -958,13 +983,14 @@ hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad, h
   a.nitem = c->gr.n_items[combo];
   a.rng = c->gr.d_rng[combo];
   a.tab64 = c->d_tab64;
+  a.tab32 = c->d_tab32;
   a.P = c->d_Ps;
   a.atom = c->d_atom;
   a.sys_atom = c->d_sys_atom;
   a.grad = grad;
   a.item_ctr = c->d_item_ctr + 2 * combo_count() + 2 * combo;
   return launch_grad(class_la(X), class_lb(X), class_la(Y), class_lb(Y), c->num_cu, st, a,
-                     &c->wg_per_cu_grad[combo], c->nmax);
+                     &c->wg_per_cu_grad[combo + mixed * combo_count()], c->nmax, mixed);
 }
 
 // Fork the context stream into the side streams (returns how many to use:
@@ -974,11 +1000,15 @@ int fork_side(gx_ctx* c) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
   if (!c->side_ready) {
-    for (int k = 0; k < GX_NSIDE; ++k) {
-      if (hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) != hipSuccess) return -1;
-      if (hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) != hipSuccess) return -1;
+    bool ok = true;
+    for (int k = 0; k < GX_NSIDE && ok; ++k)
+      ok = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {  // nothing half made stays behind; the next call tries again
+      c->release_side();
+      return -1;
     }
-    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) return -1;
     c->side_ready = true;
   }
   if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return -1;
@@ -1012,8 +1042,9 @@ int gx_create(gx_ctx** out, int device) {
   GX_HIP(hipSetDevice(device));
   gx_ctx* c = new gx_ctx();
   c->device = device;
-  // the null stream until gx_set_stream: a context creates no stream (and
-  // so no hardware queue with its own scratch) of its own
+  // the null stream until gx_set_stream; the GX_NSIDE side streams of
+  // gx_jk / gx_jk_uhf / gx_grad are created on their first use (each context
+  // holds its own set until gx_destroy)
   c->own = nullptr;
   c->stream = nullptr;
   int lds = 0, ncu = 0;
@@ -1652,10 +1683,17 @@ int gx_jk_autotune(gx_ctx* c, const double* P, double cutoff, double fp32_bound,
 }
 
 int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cutoff) {
+  return gx_grad_ex(c, P, W, grad, cutoff, 0.0, 0);
+}
+
+int gx_grad_ex(gx_ctx* c, const double* P, const double* W, double* grad, double cutoff, double fp32_bound,
+               int flags) {
   if (!c || !P || !W || !grad) return fail(GX_ERR_ARG, "gx_grad: null argument");
   if (c->npair == 0) return fail(GX_ERR_STATE, "gx_grad: no basis set");
   if (c->natom == 0) return fail(GX_ERR_STATE, "gx_grad: no nuclei set");
-  if (!(cutoff >= 0.0)) return fail(GX_ERR_ARG, "gx_grad: negative threshold");
+  if (!(cutoff >= 0.0) || !(fp32_bound >= 0.0)) return fail(GX_ERR_ARG, "gx_grad: negative threshold");
+  const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
+  const bool timing = (flags & GX_JK_TIMING) != 0;
   for (int sh = 0; sh < c->nshell; ++sh)
     if (c->atom[sh] < 0 || c->atom[sh] >= c->natom)
       return fail(GX_ERR_STATE, "gx_grad: shell " + std::to_string(sh) + " has no nucleus (atom index " +
@@ -1694,18 +1732,59 @@ int gx_grad(gx_ctx* c, const double* P, const double* W, double* grad, double cu
   hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_atom_sys,
                      c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
   GX_HIP(hipGetLastError());
-  GX_TRY(build_orders(c, cutoff, 0.0, 0, true));
-  const int nside = fork_side(c);
+  GX_TRY(build_orders(c, cutoff, fp32_bound, mixed, true));
+  // timed calls keep every kernel on the context stream, bracketed by its events
+  const int nside = timing ? 0 : fork_side(c);
   if (nside < 0) return fail(GX_ERR_HIP, "gx_grad: side streams");
   int combo = 0, launched = 0;
   for (int X = 0; X < NCLS; ++X)
     for (int Y = 0; Y <= X; ++Y, ++combo) {
       if (c->gr.n_items[combo] == 0) continue;
+      if (timing) {
+        const size_t i = c->gev_combo.size();
+        while (c->gev.size() < 2 * i + 2) {
+          hipEvent_t e;
+          GX_HIP(hipEventCreate(&e));
+          c->gev.push_back(e);
+        }
+        c->gev_combo.push_back(combo);
+        GX_HIP(hipEventRecord(c->gev[2 * i], c->stream));
+      }
       GX_HIP_FORKED(launch_grad_combo(c, X, Y, combo, grad, nside ? c->side[launched++ % nside] : c->stream),
                     nside);
+      if (timing) GX_HIP(hipEventRecord(c->gev[2 * (c->gev_combo.size() - 1) + 1], c->stream));
     }
   GX_HIP(join_side(c, nside));
   return GX_OK;
+}
+
+int gx_grad_launch_times(gx_ctx* c, double* out, int cap) {
+  if (!c || !out) return fail(GX_ERR_ARG, "gx_grad_launch_times: null argument");
+  GX_HIP(hipSetDevice(c->device));
+  const int nc = combo_count();
+  std::vector<double> ms(nc, 0.0), cnt(nc, 0.0);
+  if (!c->gev_combo.empty()) GX_HIP(hipEventSynchronize(c->gev[2 * c->gev_combo.size() - 1]));
+  for (size_t i = 0; i < c->gev_combo.size(); ++i) {
+    float t = 0.f;
+    GX_HIP(hipEventElapsedTime(&t, c->gev[2 * i], c->gev[2 * i + 1]));
+    ms[c->gev_combo[i]] += t;
+    cnt[c->gev_combo[i]] += 1.0;
+  }
+  c->gev_combo.clear();
+  int row = 0, combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      if (cnt[combo] == 0.0) continue;
+      if (row >= cap) return fail(GX_ERR_ARG, "gx_grad_launch_times: output too small");
+      double* r = out + 6 * row++;
+      r[0] = class_la(X);
+      r[1] = class_lb(X);
+      r[2] = class_la(Y);
+      r[3] = class_lb(Y);
+      r[4] = ms[combo];
+      r[5] = cnt[combo];
+    }
+  return row;
 }
 
 int gx_grad_autotune(gx_ctx* c, const double* P, const double* W, double cutoff, const double* cand, int ncand,

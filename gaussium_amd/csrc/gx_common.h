@@ -46,7 +46,7 @@ struct __align__(16) PrimPair64 {
 };
 
 struct __align__(16) PrimPair32 {
-  float p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, pad0, pad1, pad2;
+  float p, ip, Px, Py, Pz, Kip, PAx, PAy, PAz, ea, pad1, pad2;
 };
 
 // Device-side views passed to kernels by value.

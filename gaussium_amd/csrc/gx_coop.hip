@@ -361,6 +361,7 @@ struct GradArgs {
   int nitem;
   const int* rng;
   const double* tab64;
+  const float* tab32;  // FP32 Taylor tables (mixed-precision per-lane kernels)
   const double* P;     // packed densities, pre-scaled by s_k s_l
   const int* atom;     // shell -> atom (over all systems)
   const int* sys_atom; // per system: first nucleus (nsys + 1)

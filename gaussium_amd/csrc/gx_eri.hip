@@ -57,7 +57,8 @@ __global__ void k_primpairs(int npp, const int* __restrict__ pair_of, const int*
   s.PAx = (float)r.PAx;
   s.PAy = (float)r.PAy;
   s.PAz = (float)r.PAz;
-  s.pad0 = s.pad1 = s.pad2 = 0.0f;
+  s.ea = (float)a;
+  s.pad1 = s.pad2 = 0.0f;
   out32[t] = s;
 }
 

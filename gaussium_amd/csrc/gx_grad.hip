@@ -7,44 +7,61 @@
 // Classes whose sets fit the registers at once run one pass; those whose
 // per-centre sets fit run three passes over the primitives (EriGradLane::NPASS).
 // The larger classes use k_grad_coop (gx_coop.hip).
+//
+// MIXED = true: the kets whose Schwarz product is below the FP32 bound (the
+// second range of every ket segment, as in k_jk) run their primitive loop in
+// FP32 (FP32 pair records and Taylor table); the weighted sets are then
+// converted and the HRR and contraction run in FP64.
 
 namespace gx {
 
+template <int L>
+__device__ __forceinline__ void boys_nb(double T, double scale, const double* __restrict__ tab, double (&F)[L + 1]) {
+  boys64_nb<L>(T, scale, tab, F);
+}
+template <int L>
+__device__ __forceinline__ void boys_nb(float T, float scale, const float* __restrict__ tab, float (&F)[L + 1]) {
+  boys32_nb<L>(T, scale, tab, F);
+}
+
 // One pass of the per-lane gradient: the weighted sets of pass X over the
-// bra x ket primitive pairs, then its HRR and contraction into g.  The
-// density blocks are read (from the LDS rows of the bra's shells and P_cd
-// from HBM) only after the primitive loop, so they hold no registers there.
-template <int LA, int LB, int LC, int LD, int X>
-__device__ __forceinline__ void grad_lane_pass(const PrimPair64* __restrict__ bra, int nbp,
-                                               const PrimPair64* __restrict__ kp, int nk,
-                                               const double* __restrict__ tab, double ABx, double ABy,
-                                               double ABz, double CDx, double CDy, double CDz,
+// bra x ket primitive pairs in precision R (records PP, table tab), then its
+// HRR and contraction into g in FP64.  The density blocks are read (from the
+// LDS rows of the bra's shells and P_cd from HBM) only after the primitive
+// loop, so they hold no registers there.
+template <int LA, int LB, int LC, int LD, int X, typename R, typename PP>
+__device__ __forceinline__ void grad_lane_pass(const PP* __restrict__ bra, int nbp, const PP* __restrict__ kp, int nk,
+                                               const R* __restrict__ tab, double ABx, double ABy, double ABz,
+                                               double CDx, double CDy, double CDz,
                                                const double* __restrict__ prow, const double* __restrict__ Pm,
                                                int N, int rowB, int bo, int co, int dof, double f, double (&g)[9]) {
   using P = EriGradLanePass<LA, LB, LC, LD, X>;
   using C = EriGradLane<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
-  double acc[P::NACC];
+  R acc[P::NACC];
 #pragma unroll
-  for (int i = 0; i < P::NACC; ++i) acc[i] = 0.0;
+  for (int i = 0; i < P::NACC; ++i) acc[i] = R(0);
   for (int ik = 0; ik < nk; ++ik) {
-    const PrimPair64 k = kp[ik];
-    const double w3 = 2.0 * k.ea;
+    const PP k = kp[ik];
+    const R w3 = R(2) * k.ea;
     for (int ib = 0; ib < nbp; ++ib) {
-      const PrimPair64 b = bra[ib];
-      const double rs = gx_rsqrt(b.p + k.p);
-      const double is = rs * rs;
-      const double rho = b.p * k.p * is;
-      const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
-      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      double F[P::L + 1];
-      boys64_nb<P::L>(T, b.Kip * k.Kip * rs, tab, F);
-      const double rho_p = rho * b.ip, rho_q = rho * k.ip;
-      P::template prim<double>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz, k.PAx,
-                               k.PAy, k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, 0.5 * b.ip, 0.5 * k.ip,
-                               0.5 * is, rho_p, rho_q, 2.0 * b.ea, 2.0 * (b.p - b.ea), w3);
+      const PP b = bra[ib];
+      const R rs = gx_rsqrt(b.p + k.p);
+      const R is = rs * rs;
+      const R rho = b.p * k.p * is;
+      const R PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+      const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      R F[P::L + 1];
+      boys_nb<P::L>(T, b.Kip * k.Kip * rs, tab, F);
+      const R rho_p = rho * b.ip, rho_q = rho * k.ip;
+      P::template prim<R>(acc, F, b.PAx, b.PAy, b.PAz, -rho_p * PQx, -rho_p * PQy, -rho_p * PQz, k.PAx, k.PAy,
+                          k.PAz, rho_q * PQx, rho_q * PQy, rho_q * PQz, R(0.5) * b.ip, R(0.5) * k.ip, R(0.5) * is,
+                          rho_p, rho_q, R(2) * b.ea, R(2) * (b.p - b.ea), w3);
     }
   }
+  double accd[P::NACC];
+#pragma unroll
+  for (int i = 0; i < P::NACC; ++i) accd[i] = (double)acc[i];
   double Pab[NA * NB], Pcd[NC * ND], Pac[NA * NC], Pad[NA * ND], Pbc[NB * NC], Pbd[NB * ND];
 #pragma unroll
   for (int ia = 0; ia < NA; ++ia) {
@@ -66,15 +83,32 @@ __device__ __forceinline__ void grad_lane_pass(const PrimPair64* __restrict__ br
   for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
     for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pm[(long)(co + ic) * N + dof + id];
-  P::template finish<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, Pab, Pcd, Pac, Pad, Pbc, Pbd, f, g);
+  P::template finish<double>(accd, ABx, ABy, ABz, CDx, CDy, CDz, Pab, Pcd, Pac, Pad, Pbc, Pbd, f, g);
 }
 
-template <int LA, int LB, int LC, int LD>
+// All passes of one quartet in precision R.
+template <int LA, int LB, int LC, int LD, typename R, typename PP>
+__device__ __forceinline__ void grad_lane_quartet(const PP* __restrict__ bra, int nbp, const PP* __restrict__ kp,
+                                                  int nk, const R* __restrict__ tab, double ABx, double ABy,
+                                                  double ABz, double CDx, double CDy, double CDz,
+                                                  const double* __restrict__ prow, const double* __restrict__ Pm,
+                                                  int N, int rowB, int bo, int co, int dof, double f, double (&g)[9]) {
+  grad_lane_pass<LA, LB, LC, LD, 0, R, PP>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N, rowB,
+                                           bo, co, dof, f, g);
+  if constexpr (EriGradLane<LA, LB, LC, LD>::NPASS == 3) {
+    grad_lane_pass<LA, LB, LC, LD, 1, R, PP>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N,
+                                             rowB, bo, co, dof, f, g);
+    grad_lane_pass<LA, LB, LC, LD, 2, R, PP>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N,
+                                             rowB, bo, co, dof, f, g);
+  }
+}
+
+template <int LA, int LB, int LC, int LD, bool MIXED>
 __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
   using C = EriGradLane<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
   extern __shared__ double prow[];  // P rows of the bra's shells
-  __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
+  __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1], r_len[2 * JK_MAXSEG];
   __shared__ int next_chunk;
   __shared__ double gsh[3 * GX_GRAD_MAXATOM];
   const int tid = threadIdx.x;
@@ -82,6 +116,7 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
   const int wave = tid / GX_WAVE;
   constexpr int nwave = JK_TEAM / GX_WAVE;
   const double* __restrict__ tab = a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
+  const float* __restrict__ tab32 = a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
   // nuclear sums of the current system, kept in LDS across the items of
   // the workgroup and flushed when the system changes and at the end
   int cur = -1, cur3 = 0;
@@ -119,17 +154,34 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
       cur = at0;
       cur3 = nat3;
     }
+    // ranges r < nseg: FP64 part of segment r; r >= nseg (MIXED): FP32 part
+    // of segment r - nseg, starting at the next multiple of 64 so that no
+    // chunk mixes precisions (the layout of the host's work items)
+    const int nr = MIXED ? 2 * nseg : nseg;
     if (tid < nseg) {
+      const int* rr = a.rng + (long)it.x * 2 * JK_MAXSEG;
       r_start[tid] = a.seg_lo[s0 + tid];
-      r_pre[tid + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + tid];  // gradients are FP64 only
+      r_pre[tid + 1] = rr[tid];
+      if (MIXED) {
+        r_start[nseg + tid] = a.seg_lo[s0 + tid] + rr[tid];
+        r_pre[nseg + tid + 1] = rr[JK_MAXSEG + tid];
+      }
     }
     __syncthreads();
     if (tid == 0) {
-      r_pre[0] = 0;
-      for (int r = 0; r < nseg; ++r) r_pre[r + 1] += r_pre[r];
+      int acc = 0;
+      for (int r = 0; r < nr; ++r) {
+        const int len = r_pre[r + 1];
+        if (r == nseg) acc = (acc + GX_WAVE - 1) & ~(GX_WAVE - 1);
+        r_pre[r + 1] = acc + len;
+        r_len[r] = len;
+        acc += len;
+      }
+      for (int r = 0; r < nr; ++r) r_pre[r] = r_pre[r + 1] - r_len[r];
+      r_pre[nr] = acc;
     }
     __syncthreads();
-    const int total = min(r_pre[nseg], it.z);
+    const int total = min(r_pre[nr], it.z);
     const double fA = (A == B) ? 0.5 : 1.0;
     double gab[6];
 #pragma unroll
@@ -138,8 +190,8 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
       const int v = v0 + lane;
       int r = 0;
       if (v < total)
-        while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
-      const bool act = v < total;
+        while (r + 1 < nr && v >= r_pre[r + 1]) ++r;
+      const bool act = v < total && v < r_pre[r] + r_len[r];
       if (act) {
         const int jp = r_start[r] + (v - r_pre[r]);
         const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
@@ -153,13 +205,13 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
         double g[9];
 #pragma unroll
         for (int j = 0; j < 9; ++j) g[j] = 0.0;
-        grad_lane_pass<LA, LB, LC, LD, 0>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N, rowB,
-                                          bo, co, dof, f, g);
-        if constexpr (C::NPASS == 3) {
-          grad_lane_pass<LA, LB, LC, LD, 1>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N,
-                                            rowB, bo, co, dof, f, g);
-          grad_lane_pass<LA, LB, LC, LD, 2>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm, N,
-                                            rowB, bo, co, dof, f, g);
+        if (!MIXED || r < nseg) {
+          grad_lane_quartet<LA, LB, LC, LD, double, PrimPair64>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz,
+                                                                prow, Pm, N, rowB, bo, co, dof, f, g);
+        } else if constexpr (MIXED) {
+          grad_lane_quartet<LA, LB, LC, LD, float, PrimPair32>(a.pr.pp32 + bpoff, nbp, a.pr.pp32 + a.pr.poff[jp],
+                                                               nk, tab32, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm,
+                                                               N, rowB, bo, co, dof, f, g);
         }
 #pragma unroll
         for (int j = 0; j < 6; ++j) gab[j] += g[j];

@@ -272,9 +272,12 @@ class Engine:
             _lib.check(n, 'gx_jk_launch_times')
         return {tuple(int(x) for x in r[:4]): (float(r[4]), int(r[5])) for r in out[:n]}
 
-    def grad(self, P, W, cutoff=0.0):
+    def grad(self, P, W, cutoff=0.0, fp32_bound=0.0, mixed=False, timing=False):
         """Nuclear gradient dE/dR [natom, 3] of the closed-shell RHF energy from
-        the density ``P`` and the energy-weighted density ``W`` (device tensors)."""
+        the density ``P`` and the energy-weighted density ``W`` (device tensors).
+        ``mixed``: quartets with Schwarz product below ``fp32_bound`` run their
+        primitive loops in FP32 (gx_grad_ex); ``timing`` records HIP events
+        around each two-electron kernel (``grad_launch_times``)."""
         if self.nuclei is None:
             raise _lib.GxError('grad needs nuclei (set_nuclei)')
         P, W = self._dev(P), self._dev(W)
@@ -282,8 +285,19 @@ class Engine:
         self._matrix_shape(W, 'W')
         g = self._empty(len(self.nuclei), 3)
         self._bind_stream()
-        _lib.check(self.lib.gx_grad(self.ctx, _p(P), _p(W), _p(g), float(cutoff)), 'gx_grad')
+        flags = (_lib.GX_JK_MIXED if mixed else 0) | (_lib.GX_JK_TIMING if timing else 0)
+        _lib.check(self.lib.gx_grad_ex(self.ctx, _p(P), _p(W), _p(g), float(cutoff), float(fp32_bound), flags),
+                   'gx_grad_ex')
         return g
+
+    def grad_launch_times(self):
+        """{(la, lb, lc, ld): (total ms, launches)} of the two-electron gradient
+        kernels over the ``grad(..., timing=True)`` calls since the previous read."""
+        out = np.zeros((64, 6))
+        n = self.lib.gx_grad_launch_times(self.ctx, _p(out), 64)
+        if n < 0:
+            _lib.check(n, 'gx_grad_launch_times')
+        return {tuple(int(x) for x in r[:4]): (float(r[4]), int(r[5])) for r in out[:n]}
 
     def jk_stats(self, cutoff=0.0, fp32_bound=0.0, mixed=False):
         st = _lib.GxStats()

@@ -207,13 +207,15 @@ class RestrictedHF:
         """Analytic nuclear gradient dE_total/dR [natom, 3] (device tensor) at the
         converged SCF: the density and the energy-weighted density
         W = 2 sum_occ e_i C_i C_i^T of the last iteration (runs the SCF first if
-        it has not run)."""
+        it has not run), with the Fock builder's screening and precision."""
         if not hasattr(self.scf_method, 'density'):
             self.begin_scf()
         eps = self.scf_method.orbital_energies
         c = self.scf_method.orbital_coefficients[:, :self.electrons // 2]
         W = 2.0 * (c * eps[:self.electrons // 2]) @ c.T
-        return self.engine.grad(self.scf_method.density, W, cutoff=self.schwarz_cutoff)
+        fb = self.fock_builder
+        return self.engine.grad(self.scf_method.density, W, cutoff=self.schwarz_cutoff, fp32_bound=fb.fp32_bound,
+                                mixed=fb.mixed)
 
 
 class UnrestrictedHF:

@@ -27,7 +27,7 @@
  *                     matrixelements/orbital_overlap_matrix.py:15-27,
  *                     matrixelements/kinetic_energy_matrix.py:16-33,
  *                     matrixelements/nuclear_attraction_matrix.py:16-29
- *   gx_grad_*         no counterpart (the reference has no gradients; they
+ *   gx_grad*          no counterpart (the reference has no gradients; they
  *                     replace the function-count-driven Nelder-Mead search,
  *                     geometryoptimization/nelder_mead.py:28-144)
  */
@@ -189,6 +189,23 @@ int gx_jk_autotune(gx_ctx* ctx, const double* P_dev, double cutoff, double fp32_
  * analytic gradient: it replaces the numerical differentiation of energies
  * by its optimiser (geometryoptimization/nelder_mead.py:126-144). */
 int gx_grad(gx_ctx* ctx, const double* P_dev, const double* W_dev, double* grad_dev, double cutoff);
+
+/* gx_grad with precision and timing flags.  GX_JK_MIXED: in the per-lane
+ * classes (up to (pp|pp) and the smaller d classes) the shell quartets with
+ * Q_ab Q_cd < fp32_bound run their primitive loops in FP32 (FP32 pair
+ * records and Boys table), the HRR, the contraction with the densities and
+ * the accumulation stay FP64; the wave-per-quartet classes stay FP64.
+ * GX_JK_TIMING: every class-combination kernel runs on the context stream
+ * between two HIP events (read with gx_grad_launch_times).  gx_grad is
+ * gx_grad_ex(..., 0.0, 0).  No counterpart in the reference (see gx_grad). */
+int gx_grad_ex(gx_ctx* ctx, const double* P_dev, const double* W_dev, double* grad_dev, double cutoff,
+               double fp32_bound, int flags);
+
+/* Kernel durations of the two-electron gradient kernels of the gx_grad_ex
+ * calls made with GX_JK_TIMING since the previous call of this function, as
+ * gx_jk_launch_times: rows of la, lb, lc, ld, total milliseconds, launches.
+ * No counterpart in the reference (measurement). */
+int gx_grad_launch_times(gx_ctx* ctx, double* out, int cap);
 
 /* Autotuner of the gradient's work-item size per class combination, as
  * gx_jk_autotune (rows of la, lb, lc, ld, chosen value, milliseconds); the

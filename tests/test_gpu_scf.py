@@ -98,3 +98,49 @@ def test_cluster_255bf_energy_and_gradient(surface_255):
     sm = _surface(workloads.cluster_255bf, schwarz_cutoff=1e-11, precision='mixed', fp32_bound=1e-4)
     em = sm.energy(s.xyz0, initial_density=p0)
     assert abs(em - e0) < 1e-6
+
+
+def test_cluster_66bf_mixed_gradient(surface_66):
+    """Mixed-precision gradient (gx_grad_ex with GX_JK_MIXED: quartets with
+    Schwarz product below 1e-4 in FP32) at the converged density of the
+    66-function cluster: within 1e-6 Eh/bohr of the FP64 gradient."""
+    s = surface_66
+    s.energy_and_gradient(s.xyz0)
+    hf = s.last
+    eng = s.engine
+    no = s.electrons // 2
+    c = hf.scf_method.orbital_coefficients[:, :no]
+    W = 2.0 * (c * hf.scf_method.orbital_energies[:no]) @ c.T
+    P = hf.scf_method.density
+    g64 = eng.grad(P, W, cutoff=1e-11).cpu().numpy()
+    gmx = eng.grad(P, W, cutoff=1e-11, fp32_bound=1e-4, mixed=True).cpu().numpy()
+    rows = eng.jk_class_counts(cutoff=1e-11, fp32_bound=1e-4, mixed=True)
+    assert sum(r[5] for r in rows) > 0  # FP32 quartets exist
+    err = np.abs(gmx - g64).max()
+    print('66 bf: max |g_mixed - g_fp64| = %.2e Eh/bohr' % err)
+    assert err < 1e-6
+    assert np.abs(gmx.sum(axis=0)).max() < 1e-8
+
+
+def test_mixed_gradient_all_fp32_path():
+    """Every quartet of the 7-function triatomic in FP32 (bound 1e30): the FP32
+    primitive loop (pair records with the exponent of A, FP32 Boys table)
+    agrees with FP64 to single precision, at a random symmetric P and W."""
+    from gaussium_amd import workloads
+    from gaussium_amd.engine import Engine
+    import torch
+    nuc, basis, ne, info = workloads.triatomic_7bf()
+    n = len(basis)
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, n))
+    P = torch.from_numpy((a + a.T) / n).cuda()
+    W = torch.from_numpy((b + b.T) / n).cuda()
+    eng = Engine(basis, nuc)
+    g64 = eng.grad(P, W).cpu().numpy()
+    g32 = eng.grad(P, W, fp32_bound=1e30, mixed=True).cpu().numpy()
+    scale = np.abs(g64).max()
+    err = np.abs(g32 - g64).max()
+    print('all-FP32 gradient: max |g32 - g64| = %.2e (max |g| %.2e)' % (err, scale))
+    assert err < 1e-4 * max(scale, 1.0)
+    assert err > 0.0  # the FP32 path ran
