@@ -673,10 +673,13 @@ int host_count(const gx_ctx* c, int lo, int hi, double QA, double thr) {
 
 // For every class combo and bra: [lo, hi, n64, n32] per ket segment.
 template <class F>
-void for_each_bra_segment(const gx_ctx* c, int X, int Y, double cutoff, double fp32, int mixed, F&& f) {
-  if (combo_is_coop(X, Y)) mixed = 0;  // cooperative classes are always FP64
+void for_each_bra_segment(const gx_ctx* c, int X, int Y, double cutoff, double fp32, int mixed, F&& f,
+                          bool jk_rules = true) {
+  // J/K: the cooperative classes are always FP64, and so are the
+  // combinations gx_jk_autotune put in FP64; the gradient's caller decides
+  if (jk_rules && combo_is_coop(X, Y)) mixed = 0;
   const size_t combo = (size_t)X * (X + 1) / 2 + Y;
-  if (combo < c->jk_fp32.size() && !c->jk_fp32[combo]) mixed = 0;  // tuned to FP64
+  if (jk_rules && combo < c->jk_fp32.size() && !c->jk_fp32[combo]) mixed = 0;
   const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
   for (int i = 0; i < nb; ++i) {
     const int ip = b0 + i;
@@ -731,7 +734,7 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
     while (c->seg_lo[s0 + sg] != lo) ++sg;
     rng[(size_t)i * 2 * JK_MAXSEG + sg] = n64;
     rng[(size_t)i * 2 * JK_MAXSEG + JK_MAXSEG + sg] = n32;
-  });
+  }, !grad);
   // per bra: virtual layout and per-chunk cost (widest ket of the chunk).
   // Per-lane classes: chunks of 64 kets, four waves per item; cooperative
   // classes: one ket per chunk, one wave per item.
@@ -1930,6 +1933,14 @@ int gx_jk_launch_times(gx_ctx* c, double* out, int cap) {
       r[5] = cnt[combo];
     }
   return row;
+}
+
+int gx_kernel_kind(int la, int lb, int lc, int ld, int gradient) {
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > GX_ERI_LMAX || lb > la || lc > GX_ERI_LMAX || ld > lc ||
+      pair_class(lc, ld) > pair_class(la, lb))
+    return fail(GX_ERR_ARG, "gx_kernel_kind: not a canonical class combination");
+  if (gradient) return gx_grad_is_lane(la, lb, lc, ld) ? 0 : 1;
+  return gx_class_is_coop(la, lb, lc, ld) ? 1 : 0;
 }
 
 int gx_one_electron(gx_ctx* c, double* S, double* T, double* V) {

@@ -15,6 +15,18 @@ Digestion: 12 per unique basis-function quartet.
 
 The counts are FMA = 2 flops and do not depend on how a kernel schedules
 the work; roofline fractions use them against the FP64 (or FP32) peak.
+
+Gradient (one derivative shell quartet, d/dA, d/dB, d/dC; d/dD by
+translational invariance), per primitive quartet: setup 25 + 3 (the
+exponent weights); Boys 20 + 4 (L + 1); one VRR over 0 <= e <= la+lb+1,
+0 <= f <= lc+ld+1 without the corner (e, f) = (la+lb+1, lc+ld+1) (no
+centre pair is raised together), costed as above with L + 1 in place of L;
+contraction of four weighted sets (unweighted for the lowered terms, 2a, 2b,
+2c): 2 x 4 per [e0|f0] element with la <= e <= la+lb+1, lc <= f <= lc+ld+1
+(corner excluded).  Per contracted shell quartet: the HRR of the three raised
+classes (hrr_flops with l_X + 1) and of the unraised class (for the lowered
+terms).  Per basis-function quartet: the density factor D (8 flops) and nine
+derivatives, each 2 a (a+1|..) - a (a-1|..) contracted with D (3 + 2 flops).
 """
 from functools import lru_cache
 
@@ -73,6 +85,55 @@ def hrr_flops(la, lb, lc, ld):
             ket += ncart(c) * ncart(d)
     f += 2 * ket * ncart(la) * ncart(lb)
     return f
+
+
+@lru_cache(maxsize=None)
+def grad_prim_flops(la, lb, lc, ld):
+    """Model flops of one derivative primitive quartet (see the module text)."""
+    L = la + lb + lc + ld + 1
+    eb, fb = la + lb + 1, lc + ld + 1
+    f = 25 + 3 + 20 + 4 * L
+    for e in range(eb + 1):
+        for g in range(fb + 1):
+            if e == eb and g == fb:
+                continue
+            if e == 0 and g == 0:
+                f += L + 1
+                continue
+            nm = L - e - g + 1
+            for a in cart(e):
+                for c in cart(g):
+                    if e > 0:
+                        i = next(k for k in range(3) if a[k] > 0)
+                        terms = 2 + (2 if a[i] - 1 > 0 else 0) + (1 if c[i] > 0 else 0)
+                    else:
+                        i = next(k for k in range(3) if c[k] > 0)
+                        terms = 2 + (2 if c[i] - 1 > 0 else 0) + (1 if a[i] > 0 else 0)
+                    f += nm * 2 * terms
+    nel = 0
+    for e in range(la, eb + 1):
+        for g in range(lc, fb + 1):
+            if e == eb and g == fb:
+                continue
+            nel += ncart(e) * ncart(g)
+    f += 2 * 4 * nel
+    return f
+
+
+def grad_hrr_flops(la, lb, lc, ld):
+    return (hrr_flops(la + 1, lb, lc, ld) + hrr_flops(la, lb + 1, lc, ld) + hrr_flops(la, lb, lc + 1, ld)
+            + hrr_flops(la, lb, lc, ld))
+
+
+def grad_model_flops(rows):
+    """Model flops of the two-electron gradient (FP64 part, FP32 part) from
+    ``gx_jk_class_counts`` rows taken with the gradient's thresholds."""
+    f64 = f32 = 0.0
+    for la, lb, lc, ld, s64, s32, p64, p32, b64, b32 in rows:
+        pf, hf = grad_prim_flops(la, lb, lc, ld), grad_hrr_flops(la, lb, lc, ld)
+        f64 += p64 * pf + s64 * hf + (8.0 + 9.0 * 5.0) * b64
+        f32 += p32 * pf + s32 * hf + (8.0 + 9.0 * 5.0) * b32
+    return f64, f32
 
 
 def jk_model_flops(rows):

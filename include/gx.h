@@ -235,6 +235,13 @@ int gx_jk_class_counts(gx_ctx* ctx, double cutoff, double fp32_bound, int flags,
  * the reference (measurement). */
 int gx_jk_launch_times(gx_ctx* ctx, double* out, int cap);
 
+/* Kernel family of class combination (la lb|lc ld) (canonical: la >= lb,
+ * lc >= ld, (lc, ld) not after (la, lb)): 0 = one shell quartet per lane
+ * (k_jk, k_grad_lane), 1 = one quartet per wave (k_jk_coop, k_grad_coop),
+ * for the J/K (gradient = 0) or gradient (1) kernels; negative on bad
+ * arguments.  No counterpart in the reference (measurement). */
+int gx_kernel_kind(int la, int lb, int lc, int ld, int gradient);
+
 /* One-electron matrices S, T, V (nbf x nbf FP64 device buffers); V needs
  * gx_set_nuclei first. */
 int gx_one_electron(gx_ctx* ctx, double* S_dev, double* T_dev, double* V_dev);
