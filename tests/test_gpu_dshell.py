@@ -130,3 +130,33 @@ def test_jk_255bf_rows_match_oracle():
     Jm, Km = eng.jk(Pd, cutoff=1e-11, fp32_bound=1e-4, mixed=True)
     assert np.abs(Jm.cpu().numpy() - J).max() < 1e-6
     assert np.abs(Km.cpu().numpy() - K).max() < 1e-6
+
+
+def _one_electron_parity(basis, nuc):
+    from gaussium_amd.engine import Engine
+    from oracle.basis import Nuclei as ONuc
+    from oracle.boys import exact
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    eng = Engine(basis, nuc)
+    S, T, V = (x.cpu().numpy() for x in eng.one_electron())
+    onuc = [ONuc(n.element, n.charge, n.mass, n.coordinates) for n in nuc]
+    errs = (np.abs(S - overlap_matrix(basis)).max(), np.abs(T - kinetic_matrix(basis)).max())
+    with exact():
+        ev = np.abs(V - nuclear_matrix(basis, onuc)).max()
+    print('S %.1e T %.1e V %.1e' % (errs[0], errs[1], ev))
+    assert errs[0] < 1e-10 and errs[1] < 1e-10 and ev < 1e-10
+
+
+def test_one_electron_with_d_shells_matches_oracle():
+    """S, T, V of contracted S/L/P/D shells on three centres (d_xx .. d_yz,
+    one- and two-primitive D) against the oracle, 1e-10."""
+    from gaussium_amd.basis import Nuclei
+    nuc = [Nuclei('A', 3.0, 6.0, (0.0, 0.0, 0.0)), Nuclei('B', 2.0, 4.0, (1.3, -0.4, 0.6)),
+           Nuclei('C', 1.0, 1.0, (-0.5, 1.1, -0.9))]
+    _one_electron_parity(_d_basis(), nuc)
+
+
+def test_one_electron_polarised_cluster_matches_oracle():
+    from gaussium_amd import workloads
+    nuc, basis, ne, info = workloads.cluster(3, 2, 5.0, polarised=True, seed=9)
+    _one_electron_parity(basis, nuc)

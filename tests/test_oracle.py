@@ -501,3 +501,59 @@ def test_oracle_second_order_scf_cluster_66bf_is_a_stable_solution():
     assert np.linalg.eigvalsh(hess)[0] > 0.0
     print('66 bf: E %.10f, %d evaluations, HOMO-LUMO %.4f, lowest Hessian eigenvalue %.3e' % (
         e + e_nuc, it, e_all[no] - e_all[no - 1], np.linalg.eigvalsh(hess)[0]))
+
+
+def test_one_electron_d_branch_closed_forms():
+    """The oracle's d branch against closed forms for one normalised
+    primitive x^l y^m z^n exp(-a r^2) with a nucleus of charge Z at its centre:
+      <d_xx|d_xx> = 1, <d_xx|d_yy> = 1/3;
+      <T> = sum over axes of a (4 l - 1) / (2 (2 l - 1)) (l > 0), a / 2 (l = 0):
+            7a/2 for d_xy, 13a/6 for d_xx;
+      <1/r> = sqrt(2a) Gamma(L + 1) / Gamma(L + 3/2) for every component of
+            total order L (from 1/r = 2/sqrt(pi) int_0^inf exp(-t^2 r^2) dt),
+            and 1/3 of it between d_xx and d_yy;
+    and one displaced nucleus against direct 3-D quadrature of the density."""
+    from math import gamma, sqrt
+    from oracle import boys as oboys
+    from oracle.basis import Nuclei, expand_basis_set
+    from oracle.one_electron import kinetic_matrix, nuclear_matrix, overlap_matrix
+    a, z = 0.8, 3.0
+    basis = expand_basis_set([['D', [1.0, a]]], (0.2, -0.1, 0.3))
+    nuc = [Nuclei('X', z, 1.0, (0.2, -0.1, 0.3))]
+    S, T = overlap_matrix(basis), kinetic_matrix(basis)
+    with oboys.exact(True):
+        V = nuclear_matrix(basis, nuc)
+    xx, yy, xy = 0, 1, 3
+    assert abs(S[xx, xx] - 1.0) < 1e-13 and abs(S[xy, xy] - 1.0) < 1e-13
+    assert abs(S[xx, yy] - 1.0 / 3.0) < 1e-13
+    assert abs(T[xy, xy] - 3.5 * a) < 1e-12
+    assert abs(T[xx, xx] - 13.0 * a / 6.0) < 1e-12
+    r1 = sqrt(2 * a) * gamma(3) / gamma(3.5)
+    assert abs(V[xx, xx] + z * r1) < 1e-12 and abs(V[xy, xy] + z * r1) < 1e-12
+    assert abs(V[xx, yy] + z * r1 / 3.0) < 1e-12
+    # a nucleus 0.7 bohr away along (1, 1, 1)/sqrt(3): -Z int |phi_xy|^2 / |r - C| d^3r by
+    # Gauss-Hermite x Gauss-Hermite x Gauss-Hermite quadrature of the Coulomb kernel
+    # 1/|r - C| = 2/sqrt(pi) int_0^inf exp(-t^2 |r - C|^2) dt, done in closed form per axis:
+    # int x^2 exp(-2a x^2 - t^2 (x - c)^2) dx, then a 1-D quadrature over t
+    c = 0.7 / sqrt(3.0)
+    nuc2 = [Nuclei('X', z, 1.0, (0.2 + c, -0.1 + c, 0.3 + c))]
+    with oboys.exact(True):
+        V2 = nuclear_matrix(basis, nuc2)
+    from scipy.integrate import quad
+    from math import pi, exp
+
+    def axis(t, l):
+        # int x^(2l) exp(-2a x^2 - t^2 (x - c)^2) dx for l = 0, 1 (x measured from the centre)
+        g = 2 * a + t * t
+        mu = t * t * c / g
+        pref = sqrt(pi / g) * exp(-2 * a * t * t * c * c / g)
+        return pref if l == 0 else pref * (mu * mu + 1.0 / (2 * g))
+
+    norm2 = (4 * a) ** 2 / ((pi / (2 * a)) ** 1.5)  # squared primitive norm of d_xy
+
+    def integrand(t):
+        return axis(t, 1) * axis(t, 1) * axis(t, 0)
+
+    val, _ = quad(integrand, 0.0, float('inf'), epsabs=1e-14, epsrel=1e-13, limit=400)
+    v_ref = -z * norm2 * 2.0 / sqrt(pi) * val
+    assert abs(V2[xy, xy] - v_ref) < 1e-10, (V2[xy, xy], v_ref)
