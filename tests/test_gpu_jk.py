@@ -117,6 +117,19 @@ def test_start_rhf_energy(system, published):
     assert abs(e - e_ref) < 1e-8
 
 
+@pytest.mark.parametrize('basis_file,published', [  # examples/run.py:18-20 of the reference
+    ('3-21G.gbs', -2.83567987364),
+    ('6-311G.gbs', -2.85989542457),
+    ('cc-pVDZ.gbs', -2.85516047724192),
+])
+def test_start_rhf_helium(basis_file, published):
+    """start('He.mol', basis, 'RHF', 4) against the published energies, 1e-9 Eh."""
+    from gaussium_amd import start
+    from tests.helpers import golden
+    e = start(golden('He.mol'), golden(basis_file), 'RHF', 4)
+    assert abs(e - published) < 1e-9, (e, published)
+
+
 def test_mock_heh_electronic_energy():
     """tests/testsfast/hartreefock/test_hatree_fock.py:59-61 of the reference: -4.227529."""
     from gaussium_amd.basis import Nuclei

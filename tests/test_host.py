@@ -271,3 +271,31 @@ def test_second_order_hessian_matches_gradient_differences():
     err = (fd - hk).abs().max()
     print('Hessian-vector product vs difference of gradients: %.2e (|Hk| %.2e)' % (float(err), float(hk.abs().max())))
     assert float(err) < 1e-6 * max(1.0, float(hk.abs().max()))
+
+
+def test_bfgs_minimize_converges_and_never_climbs():
+    """The optimiser behind start(..., geometry_optimization=...): it reaches
+    the minimum of a Rosenbrock valley, and with a gradient that points uphill
+    (every line search fails) it stops at the start instead of accepting an
+    uphill point."""
+    from gaussium_amd.geometry import bfgs_minimize
+
+    def rosen(x):
+        e = (1 - x[0]) ** 2 + 10 * (x[1] - x[0] ** 2) ** 2
+        g = np.array([-2 * (1 - x[0]) - 40 * x[0] * (x[1] - x[0] ** 2), 20 * (x[1] - x[0] ** 2)])
+        return e, g
+
+    x, e, hist = bfgs_minimize(rosen, [-1.0, 1.5], gtol=1e-8, etol=0.0, max_steps=500)
+    assert np.abs(x - 1.0).max() < 1e-6 and e < 1e-12
+    assert all(b[0] <= a[0] for a, b in zip(hist, hist[1:]))
+    calls = []
+
+    def wrong_way(x):
+        e, g = rosen(x)
+        calls.append(e)
+        return e, -g
+
+    x0 = np.array([-1.0, 1.5])
+    x, e, hist = bfgs_minimize(wrong_way, x0, max_steps=50)
+    assert e == rosen(x0)[0] and np.array_equal(x, x0)
+    assert len(hist) == 1 and len(calls) > 1

@@ -330,6 +330,23 @@ def test_oracle_rhf_published_energies(system, published):
     assert abs(e - e2) < 1e-9
 
 
+@pytest.mark.parametrize('basis_file,published', [  # examples/run.py:18-20
+    ('3-21G.gbs', -2.83567987364),
+    ('6-311G.gbs', -2.85989542457),
+    ('cc-pVDZ.gbs', -2.85516047724192),
+])
+def test_oracle_rhf_published_helium_energies(basis_file, published):
+    """He with three basis sets (the published values carry 11-15 digits):
+    the oracle within 1e-9 Eh with the reference's Boys series and with the
+    exact one."""
+    from tests.helpers import oracle_rhf
+    nuc, basis, ne = load_system('He.mol', basis_file)
+    e, _ = oracle_rhf(nuc, basis, ne, exact_boys=False, diis_cond_limit=None)
+    assert abs(e - published) < 1e-9
+    e2, _ = oracle_rhf(nuc, basis, ne)
+    assert abs(e2 - published) < 1e-9
+
+
 def test_oracle_uhf_published_energy():
     """tests/testslow/start/test_start.py:33-37: UHF O2/STO-3G -147.634028141."""
     from tests.helpers import oracle_uhf
