@@ -48,6 +48,8 @@ SIGNATURES = {
     'gx_eri_dense': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     'gx_jk': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    'gx_jk_noalloc': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     'gx_jk_uhf': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
     'gx_jk_autotune': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -19,9 +20,37 @@
 #include "gx_grad.hip"
 #include "gx_one.hip"
 
+// Device scratch of gx_jk_noalloc: the scaled density, the GX_JT_COPIES
+// copies of Jt and Kt of one build, (2 + GX_JT_COPIES) * sum_k n_k^2 doubles
+// at most GX_NOALLOC_MAX_DOUBLES (5 MiB: one system of up to 256 functions).
+// Every device the library runs on holds its own copy until the process
+// ends; its address is taken once per device by gx_create.
+#define GX_NOALLOC_MAX_DOUBLES (10L * 256 * 256)
+static __device__ __align__(256) double scratch[GX_NOALLOC_MAX_DOUBLES];
+
 namespace {
 
 thread_local std::string g_err;
+
+// Per-device address of `scratch` (hipGetSymbolAddress with the device
+// current); host code uses these, never the array's name.
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;
+
+hipError_t scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[device] = static_cast<double*>(p);
+  return e;
+}
+
+double* scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -899,8 +928,13 @@ gx::OneArgs one_args(const gx_ctx* c) {
 // The J/K kernel of class combination (X|Y) over its current work items.
 // nden = 2: a two-density (unrestricted) build in FP64: J of Ps + Ps2 into
 // Jt, K of Ps into Kt and of Ps2 into Kt2.
-hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden = 1, hipStream_t st = nullptr) {
+// Ps, Jt, Kt: the scaled density and accumulators (nullptr: the context's).
+hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden = 1, hipStream_t st = nullptr,
+                        double* Ps = nullptr, double* Jt = nullptr, double* Kt = nullptr) {
   if (!st) st = c->stream;
+  if (!Ps) Ps = c->d_Ps;
+  if (!Jt) Jt = c->d_Jt;
+  if (!Kt) Kt = c->d_Kt;
   if ((size_t)combo < c->jk_fp32.size() && !c->jk_fp32[combo]) mixed = 0;  // this combination runs in FP64
   if (nden == 2) mixed = 0;
   gx::JKArgs a;
@@ -916,12 +950,12 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden 
   a.master = c->d_boys;
   a.tab64 = c->d_tab64;
   a.tab32 = c->d_tab32;
-  a.P = c->d_Ps;
-  a.P2 = nden == 2 ? c->d_Ps2 : c->d_Ps;
-  a.Pj = nden == 2 ? c->d_Psj : c->d_Ps;
-  a.Jt = c->d_Jt;
-  a.Kt = c->d_Kt;
-  a.Kt2 = nden == 2 ? c->d_Kt2 : c->d_Kt;
+  a.P = Ps;
+  a.P2 = nden == 2 ? c->d_Ps2 : Ps;
+  a.Pj = nden == 2 ? c->d_Psj : Ps;
+  a.Jt = Jt;
+  a.Kt = Kt;
+  a.Kt2 = nden == 2 ? c->d_Kt2 : Kt;
   a.item_ctr = c->d_item_ctr + 2 * combo;
   a.jstride = c->mtot;
   a.mixed = mixed;
@@ -1043,6 +1077,8 @@ int gx_create(gx_ctx** out, int device) {
   GX_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(GX_ERR_ARG, "gx_create: no such device");
   GX_HIP(hipSetDevice(device));
+  // the no-alloc entry point's scratch address on this device, once
+  GX_HIP(scratch_setup(device));
   gx_ctx* c = new gx_ctx();
   c->device = device;
   // the null stream until gx_set_stream; the GX_NSIDE side streams of
@@ -1562,6 +1598,62 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, K);
+  GX_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int gx_jk_noalloc(gx_ctx* c, const double* P, double* J, double* K, double cutoff, double fp32_bound,
+                  int flags) {
+  if (!c || !P || !J || !K) return fail(GX_ERR_ARG, "gx_jk_noalloc: null argument");
+  if (c->npair == 0) return fail(GX_ERR_STATE, "gx_jk_noalloc: no basis set");
+  if (!(cutoff >= 0.0) || !(fp32_bound >= 0.0)) return fail(GX_ERR_ARG, "gx_jk_noalloc: negative threshold");
+  if (flags & ~GX_JK_MIXED) return fail(GX_ERR_ARG, "gx_jk_noalloc: only GX_JK_MIXED is accepted");
+  const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
+  const long nn = c->mtot;
+  const long need = (2L + GX_JT_COPIES) * nn;
+  if (need > GX_NOALLOC_MAX_DOUBLES)
+    return fail(GX_ERR_ARG, "gx_jk_noalloc: the build needs " + std::to_string(need) +
+                                " doubles of scratch, at most " + std::to_string(GX_NOALLOC_MAX_DOUBLES) +
+                                " (sum of n_k^2 <= " + std::to_string(GX_NOALLOC_MAX_DOUBLES / (2 + GX_JT_COPIES)) +
+                                ")");
+  double* base = scratch_of(c->device);
+  if (!base) return fail(GX_ERR_STATE, "gx_jk_noalloc: no scratch address for this device");
+  // the work items are built (and uploaded) by gx_jk / gx_jk_autotune; this
+  // call only uses them
+  const double fp32 = mixed ? fp32_bound : 0.0;
+  if (c->jk.d_items.empty() || c->jk.cutoff != cutoff || c->jk.fp32 != fp32 || c->jk.mixed != mixed)
+    return fail(GX_ERR_STATE, "gx_jk_noalloc: no work items for these thresholds (call gx_jk with the same "
+                              "cutoff, fp32_bound and flags once first)");
+  const int N = c->nmax;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      if (c->cls_start[X + 1] == c->cls_start[X] || c->cls_start[Y + 1] == c->cls_start[Y]) continue;
+      const size_t lds = jk_lds_bytes(class_la(X), class_lb(X), class_la(Y), class_lb(Y), N, mixed);
+      if (lds > (size_t)c->max_lds)
+        return fail(GX_ERR_ARG, "gx_jk_noalloc: basis too large for the LDS row cache (nbf=" +
+                                    std::to_string(N) + ")");
+    }
+  GX_HIP(hipSetDevice(c->device));
+  double* Ps = base;
+  double* Kt = base + nn;
+  double* Jt = base + 2 * nn;
+  const unsigned g = (unsigned)((nn + 255) / 256);
+  // the scratch keeps the previous call's contents: the accumulators are
+  // cleared by kernels (no memset node, so a captured call stays replayable)
+  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, P, Ps);
+  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((nn * GX_JT_COPIES + 255) / 256)), dim3(256), 0, c->stream,
+                     nn * GX_JT_COPIES, Jt, 0.0);
+  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, Kt, 0.0);
+  GX_HIP(hipGetLastError());
+  int combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo) {
+      if (c->jk.n_items[combo] == 0) continue;
+      GX_HIP(launch_combo(c, X, Y, combo, mixed, 1, c->stream, Ps, Jt, Kt));
+    }
+  hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
+                     c->d_sbfb, c->d_scale, Jt, Kt, J, K);
   GX_HIP(hipGetLastError());
   return GX_OK;
 }

@@ -154,6 +154,21 @@ int gx_eri_dense(gx_ctx* ctx, double* out_dev);
 int gx_jk(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double cutoff,
           double fp32_bound, int flags);
 
+/* gx_jk that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  The scaled density and the J/K accumulators live in
+ * one file-scope device array of the library (5 MiB per device,
+ * (2 + 8) sum_k n_k^2 <= 655360 doubles, i.e. one system of at most 256
+ * functions); larger inputs are rejected with GX_ERR_ARG.  The work items
+ * must already exist for these thresholds (a gx_jk call with the same
+ * cutoff, fp32_bound and GX_JK_MIXED flag builds them; otherwise
+ * GX_ERR_STATE).  Only GX_JK_MIXED is accepted; every kernel runs on the
+ * context stream.  All calls on one device share that array: calls that
+ * overlap in time on one device (other streams, other contexts, other
+ * threads) race; consecutive calls do not (each clears what it uses). */
+int gx_jk_noalloc(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double cutoff,
+                  double fp32_bound, int flags);
+
 /* Unrestricted (two-density) direct build, FP64: J of P_a + P_b, K of P_a
  * and K of P_b from one evaluation of the integrals (the cooperative d-shell
  * classes evaluate theirs once per density), same thresholds and work items
