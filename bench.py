@@ -239,14 +239,18 @@ def converged_scf(eng, nuc, basis, ne, precision='fp64', initial_density=None):
 
 # --------------------------------------------------------------------------- modes
 def kernels_only(args):
-    """The J/K kernels of both modes on the core-Hamiltonian guess density (the
+    """The J/K kernels of both modes (220-function cluster, or the 255-function
+    one with --workload cluster_255bf_d) on the core-Hamiltonian guess density (the
     work per kernel depends on the Schwarz bounds only, not on P), plus one
     plain device copy of COPY_BYTES whose FETCH_SIZE / WRITE_SIZE calibrate the
     HBM counters."""
     import torch
     from gaussium_amd import workloads
     from gaussium_amd.engine import Engine
-    nuc, basis, ne, info = workloads.cluster_220bf(seed=args.seed)
+    if args.workload == 'cluster_255bf_d':
+        nuc, basis, ne, info = workloads.cluster_255bf()
+    else:
+        nuc, basis, ne, info = workloads.cluster_220bf(seed=args.seed)
     eng = Engine(basis, nuc)
     P = core_guess_density(eng, ne)
     out = {'workload': info['workload'], 'copy_bytes': COPY_BYTES}

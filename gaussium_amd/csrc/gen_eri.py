@@ -248,7 +248,11 @@ def emit_class(la, lb, lc, ld):
 #     slots 0..NACC-1 hold the contracted accumulators.
 #   out[k] = H[outslot[k]] in the reference component order.
 # ---------------------------------------------------------------------------
-COOP_NACC = 90  # classes with more contracted intermediates than this are cooperative
+COOP_NACC = 90  # classes with more contracted intermediates than this are cooperative ...
+COOP_NACC_KET_S = 200  # ... except those without ket HRR (ld = 0) up to this many (measured on
+                       # gfx950: per-lane code, even spilling, beats the wave-per-quartet kernel
+                       # there: (dp|ds) 0.38 -> 0.11 ms, (dd|ps) 0.29 -> 0.07, (dd|ds) 0.39 -> 0.24
+                       # at the 255-function cluster; (dp|pp), with a ket HRR, got slower)
 
 
 def compact_hrr(hentries, nacc, fixed, keep):
@@ -773,8 +777,8 @@ def emit_prog(la, lb, lc, ld):
 
 def is_coop(c):
     la, lb, lc, ld = c
-    return (sum(ncart(e) for e in range(la, la + lb + 1)) * sum(ncart(f) for f in range(lc, lc + ld + 1))
-            > COOP_NACC)
+    nacc = sum(ncart(e) for e in range(la, la + lb + 1)) * sum(ncart(f) for f in range(lc, lc + ld + 1))
+    return nacc > (COOP_NACC_KET_S if ld == 0 else COOP_NACC)
 
 
 def main():
