@@ -20,8 +20,9 @@
 #include "gx_grad.hip"
 #include "gx_one.hip"
 
-// Device scratch of gx_jk_noalloc: the scaled density, the GX_JT_COPIES
-// copies of Jt and Kt of one build, (2 + GX_JT_COPIES) * sum_k n_k^2 doubles
+// Device scratch of gx_jk_noalloc: the scaled density, Kt, Jt and the
+// GX_JT_COPIES copies of the pair-indexed J_cd accumulator of one build,
+// 3 sum_k n_k^2 + GX_JT_COPIES sum_pairs n_A n_B doubles
 // at most GX_NOALLOC_MAX_DOUBLES (5 MiB: one system of up to 256 functions).
 // Every device the library runs on holds its own copy until the process
 // ends; its address is taken once per device by gx_create.
@@ -178,22 +179,23 @@ struct gx_ctx {
   double* d_Q = nullptr;
   gx::PrimPair64* d_pp64 = nullptr;
   gx::PrimPair32* d_pp32 = nullptr;
-  // Work sets per class combo: work items (4 ints: bra position, virtual ket
-  // range [v0, v1), bra primitive-pair range p0 | p1 << 16), largest first;
-  // and per bra position the surviving ket counts of each ket segment (FP64
-  // parts, then FP32 parts).  `jk` for gx_jk, `gr` for gx_grad (every combo
-  // on the cooperative kernels).
+  gx::PairRec* d_prec = nullptr;  // per pair, for the J/K and gradient kernels
+  // Work sets per class combo: work items (4 ints: bra position, range
+  // [v0, v1) of the combo's ket list, bra primitive-pair range p0 | p1 << 16),
+  // largest first; and the ket list: per bra its surviving kets (FP64 ones,
+  // then FP32 ones from the next multiple of 64, GX_KET_* encoding).  `jk` for
+  // gx_jk, `gr` for gx_grad.
   struct WorkSet {
-    std::vector<int*> d_items, d_rng;
+    std::vector<int*> d_items, d_kets;
     std::vector<int> n_items;
     std::vector<double> ipw;  // items per resident workgroup, per combo and mode (FP64, mixed; autotuned)
     double cutoff = -1.0, fp32 = -1.0;
     int mixed = -1;
     void release() {
       for (auto& p : d_items) dfree(&p);
-      for (auto& p : d_rng) dfree(&p);
+      for (auto& p : d_kets) dfree(&p);
       d_items.clear();
-      d_rng.clear();
+      d_kets.clear();
       n_items.clear();
       cutoff = fp32 = -1.0;
       mixed = -1;
@@ -214,7 +216,6 @@ struct gx_ctx {
   // ket segments: per class, runs of pairs with the same primitive-count key
   std::vector<int> seg_lo, seg_hi;
   std::vector<int> cseg;  // [class][system] first segment, NCLS x (nsys + 1)
-  int *d_seg_lo = nullptr, *d_seg_hi = nullptr, *d_cseg = nullptr;
   // systems of a batch (one system unless gx_set_basis_batch): shells
   // [sys_shell[s], sys_shell[s + 1]), nuclei [sys_atom[s], sys_atom[s + 1]),
   // nbf sn[s], first function sbfb[s], packed-matrix offset smb[s]
@@ -230,8 +231,14 @@ struct gx_ctx {
   std::vector<int64_t> pre_pnp, pre_nab;     // prefix sums over pair ids
   std::vector<int> nab;                       // basis-function pairs per shell pair
   int num_cu = 256;
-  // scratch
+  // scratch: d_Jt holds Jt (mtot doubles) then GX_JT_COPIES copies of the
+  // pair-indexed J_cd accumulator (jkp_n doubles each; component k of pair p
+  // at jko + k * pairs of its class); jkp_of maps a packed matrix element to
+  // its Jkp entry (-1: none)
   double *d_Ps = nullptr, *d_Jt = nullptr, *d_Kt = nullptr;
+  long jkp_n = 0;
+  int* d_jkp_of = nullptr;
+  long jt_doubles() const { return mtot + GX_JT_COPIES * jkp_n; }
   // two-density (unrestricted) builds, allocated on first use
   double *d_Ps2 = nullptr, *d_Psj = nullptr, *d_Kt2 = nullptr;
   // nuclei
@@ -249,7 +256,7 @@ struct gx_ctx {
   gx::ShellView shells() const { return gx::ShellView{d_l, d_bf, d_ctr, d_lbf, d_ssys}; }
   gx::SysView systems() const { return gx::SysView{d_sn, d_smb}; }
   int cseg_at(int cls, int sys) const { return cseg[(size_t)cls * (nsys + 1) + sys]; }
-  gx::PairView pairs() const { return gx::PairView{d_pA, d_pB, d_ppoff, d_pnp, d_Q, d_pp64, d_pp32}; }
+  gx::PairView pairs() const { return gx::PairView{d_pA, d_pB, d_ppoff, d_pnp, d_Q, d_pp64, d_pp32, d_prec}; }
 
   void free_pairs() {
     dfree(&d_pA);
@@ -259,6 +266,7 @@ struct gx_ctx {
     dfree(&d_Q);
     dfree(&d_pp64);
     dfree(&d_pp32);
+    dfree(&d_prec);
     jk.release();
     gr.release();
     jk.ipw.clear();
@@ -266,9 +274,6 @@ struct gx_ctx {
     jk_fp32.clear();
     wg_per_cu.clear();
     wg_per_cu_grad.clear();
-    dfree(&d_seg_lo);
-    dfree(&d_seg_hi);
-    dfree(&d_cseg);
   }
   void free_all() {
     free_pairs();
@@ -291,6 +296,7 @@ struct gx_ctx {
     dfree(&d_scale);
     dfree(&d_Ps);
     dfree(&d_Jt);
+    dfree(&d_jkp_of);
     dfree(&d_Kt);
     dfree(&d_Ps2);
     dfree(&d_Psj);
@@ -633,7 +639,7 @@ int screen_primpairs(gx_ctx* c) {
     GX_TRY(upload(&d_voff, voff, c->stream));
     GX_TRY(upload(&d_vnp, vnp, c->stream));
     GX_TRY(dalloc(&d_S, (size_t)npp));
-    const gx::PairView pv{d_vA, d_vB, d_voff, d_vnp, nullptr, c->d_pp64, c->d_pp32};
+    const gx::PairView pv{d_vA, d_vB, d_voff, d_vnp, nullptr, c->d_pp64, c->d_pp32, nullptr};
     for (int k = 0; k < NCLS; ++k) {
       const int p0 = c->cls_start[k], p1 = c->cls_start[k + 1];
       if (p1 == p0) continue;
@@ -751,11 +757,12 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
   constexpr double kItemOverhead = GX_ITEM_OVERHEAD;  // setup + flush of one item
   constexpr double kQuartetCost = 8.0;                // digestion of one shell quartet per lane
   dfree(&ws.d_items[combo]);
-  dfree(&ws.d_rng[combo]);
+  dfree(&ws.d_kets[combo]);
   ws.n_items[combo] = 0;
   const int b0 = c->cls_start[X], nb = c->cls_start[X + 1] - b0;
   const int nk = c->cls_start[Y + 1] - c->cls_start[Y];
   if (nb == 0 || nk == 0) return GX_OK;
+  // surviving kets per bra and ket segment: FP64 part, then FP32 part
   std::vector<int> rng((size_t)nb * 2 * JK_MAXSEG, 0);
   for_each_bra_segment(c, X, Y, cutoff, fp32, mixed, [&](int i, int ip, int lo, int, int n64, int n32) {
     const int s0 = c->cseg_at(Y, c->ssys[c->pA[ip]]);
@@ -764,38 +771,44 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
     rng[(size_t)i * 2 * JK_MAXSEG + sg] = n64;
     rng[(size_t)i * 2 * JK_MAXSEG + JK_MAXSEG + sg] = n32;
   }, !grad);
-  // per bra: virtual layout and per-chunk cost (widest ket of the chunk).
-  // Per-lane classes: chunks of 64 kets, four waves per item; cooperative
-  // classes: one ket per chunk, one wave per item.
+  // per bra: its ket list (appended to `kets`, from kofs[i]) and per-chunk
+  // cost (widest ket of the chunk).  Per-lane classes: chunks of 64 kets,
+  // four waves per item; cooperative classes: one ket per chunk, one wave
+  // per item.
   const bool coop = grad ? !gx_grad_is_lane(class_la(X), class_lb(X), class_la(Y), class_lb(Y))
                          : combo_is_coop(X, Y);
   const int csz = coop ? 1 : GX_WAVE;
   const double waves = coop ? 1.0 : (double)(JK_TEAM / GX_WAVE);
   std::vector<std::vector<double>> chunk(nb);
+  std::vector<int> kets, kofs(nb + 1, 0);
   double total = 0.0;
-  std::vector<int> vk;  // virtual index -> ket pair id (-1 = padding)
+  std::vector<int> vk;
   for (int i = 0; i < nb; ++i) {
     const int* r = &rng[(size_t)i * 2 * JK_MAXSEG];
     const int sy = c->ssys[c->pA[b0 + i]];
     const int s0 = c->cseg_at(Y, sy), nseg = c->cseg_at(Y, sy + 1) - s0;
     vk.clear();
     for (int h = 0; h < 2; ++h) {
-      if (h == 1) vk.resize((vk.size() + GX_WAVE - 1) / GX_WAVE * GX_WAVE, -1);
+      // the FP32 kets start at the next multiple of 64, so no chunk mixes precisions
+      if (h == 1) vk.resize((vk.size() + GX_WAVE - 1) / GX_WAVE * GX_WAVE, GX_KET_NONE);
       for (int sg = 0; sg < nseg; ++sg) {
         const int start = c->seg_lo[s0 + sg] + (h ? r[sg] : 0);
-        for (int k = 0; k < r[h * JK_MAXSEG + sg]; ++k) vk.push_back(start + k);
+        for (int k = 0; k < r[h * JK_MAXSEG + sg]; ++k) vk.push_back((start + k) | (h ? GX_KET_FP32 : 0));
       }
     }
     while (!vk.empty() && vk.back() < 0) vk.pop_back();
+    kofs[i] = (int)kets.size();
+    kets.insert(kets.end(), vk.begin(), vk.end());
     const double nbp = c->pnp[b0 + i];
     for (size_t v0 = 0; v0 < vk.size(); v0 += csz) {
       int widest = 0;
       for (size_t v = v0; v < std::min(vk.size(), v0 + csz); ++v)
-        if (vk[v] >= 0) widest = std::max(widest, c->pnp[vk[v]]);
+        if (vk[v] >= 0) widest = std::max(widest, c->pnp[vk[v] & GX_KET_ID]);
       chunk[i].push_back(nbp * widest + kQuartetCost);
       total += chunk[i].back();
     }
   }
+  kofs[nb] = (int)kets.size();
   // item size: about `ipw` items per resident workgroup (four per CU
   // assumed), each spread over the waves of its workgroup, and not below the
   // per-item overhead scaled by GX_ITEMS_PER_WG / ipw (so that the autotuner's
@@ -809,6 +822,7 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
     const auto& ch = chunk[i];
     if (ch.empty()) continue;
     const int nbp = c->pnp[b0 + i];
+    const int len = kofs[i + 1] - kofs[i];
     double widest = 0.0;
     for (double x : ch) widest = std::max(widest, x);
     // primitive split so that one chunk of each part fits the target
@@ -825,8 +839,8 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
         if (last || (acc >= waves * target && k + 1 - k0 >= (size_t)waves)) {
           JKItem it;
           it.bpos = i;
-          it.v0 = (int)(k0 * csz);
-          it.v1 = (int)((k + 1) * csz);
+          it.v0 = kofs[i] + (int)(k0 * csz);
+          it.v1 = kofs[i] + std::min(len, (int)((k + 1) * csz));
           it.prims = p0 | (p1 << 16);
           items.push_back(it);
           icost.push_back(acc / waves + kItemOverhead);
@@ -848,8 +862,46 @@ int build_combo(gx_ctx* c, gx_ctx::WorkSet& ws, int X, int Y, int combo, double 
     flat[4 * k + 3] = it.prims;
   }
   ws.n_items[combo] = (int)items.size();
+  if (items.empty()) return GX_OK;
   GX_TRY(upload(&ws.d_items[combo], flat, c->stream));
-  GX_TRY(upload(&ws.d_rng[combo], rng, c->stream));
+  GX_TRY(upload(&ws.d_kets[combo], kets, c->stream));
+  GX_HIP(hipStreamSynchronize(c->stream));  // before the host vectors go out of scope
+  return GX_OK;
+}
+
+// Per-pair records (PairRec) of the current pair list on the device.
+int build_pairrec(gx_ctx* c) {
+  std::vector<gx::PairRec> rec(c->npair);
+  std::vector<int> jkp_of((size_t)c->mtot, -1);
+  std::vector<long> cbase(NCLS + 1, 0);
+  for (int k = 0; k < NCLS; ++k)
+    cbase[k + 1] = cbase[k] + (long)(c->cls_start[k + 1] - c->cls_start[k]) * ncart(class_la(k)) * ncart(class_lb(k));
+  c->jkp_n = cbase[NCLS];
+  for (int p = 0; p < c->npair; ++p) {
+    const int A = c->pA[p], B = c->pB[p];
+    gx::PairRec& r = rec[p];
+    r.ABx = c->ctr[3 * A] - c->ctr[3 * B];
+    r.ABy = c->ctr[3 * A + 1] - c->ctr[3 * B + 1];
+    r.ABz = c->ctr[3 * A + 2] - c->ctr[3 * B + 2];
+    r.fa = c->lbf[A];
+    r.fb = c->lbf[B];
+    r.poff = c->ppoff[p];
+    r.np = c->pnp[p];
+    r.A = A;
+    r.B = B;
+    r.sys = c->ssys[A];
+    // class-major, then component-major: component k of the pair sits at
+    // jko + k * (pairs of its class), so that consecutive pairs of a class
+    // are consecutive for every component
+    const int cls = c->pcls[p], ncls = c->cls_start[cls + 1] - c->cls_start[cls];
+    const int na = ncart(c->l[A]), nb = ncart(c->l[B]);
+    r.jko = (int)(cbase[cls] + (p - c->cls_start[cls]));
+    for (int ia = 0; ia < na; ++ia)
+      for (int ib = 0; ib < nb; ++ib)
+        jkp_of[c->smb[r.sys] + (long)(r.fa + ia) * c->sn[r.sys] + r.fb + ib] = r.jko + (ia * nb + ib) * ncls;
+  }
+  GX_TRY(upload(&c->d_prec, rec, c->stream));
+  GX_TRY(upload(&c->d_jkp_of, jkp_of, c->stream));
   GX_HIP(hipStreamSynchronize(c->stream));  // before the host vectors go out of scope
   return GX_OK;
 }
@@ -860,7 +912,7 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = f
   if (cutoff == ws.cutoff && fp32 == ws.fp32 && mixed == ws.mixed && !ws.d_items.empty()) return GX_OK;
   ws.release();
   ws.d_items.assign(combo_count(), nullptr);
-  ws.d_rng.assign(combo_count(), nullptr);
+  ws.d_kets.assign(combo_count(), nullptr);
   ws.n_items.assign(combo_count(), 0);
   if ((int)ws.ipw.size() != 2 * combo_count()) ws.ipw.assign(2 * combo_count(), GX_ITEMS_PER_WG);
   if ((int)c->jk_fp32.size() != combo_count()) c->jk_fp32.assign(combo_count(), 1);
@@ -942,11 +994,9 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden 
   a.pr = c->pairs();
   a.sy = c->systems();
   a.bra0 = c->cls_start[X];
-  a.seg_lo = c->d_seg_lo;
-  a.cseg = c->d_cseg + (size_t)Y * (c->nsys + 1);
   a.items = reinterpret_cast<const int4*>(c->jk.d_items[combo]);
   a.nitem = c->jk.n_items[combo];
-  a.rng = c->jk.d_rng[combo];
+  a.kets = c->jk.d_kets[combo];
   a.master = c->d_boys;
   a.tab64 = c->d_tab64;
   a.tab32 = c->d_tab32;
@@ -954,10 +1004,12 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden 
   a.P2 = nden == 2 ? c->d_Ps2 : Ps;
   a.Pj = nden == 2 ? c->d_Psj : Ps;
   a.Jt = Jt;
+  a.Jkp = Jt + c->mtot;
   a.Kt = Kt;
   a.Kt2 = nden == 2 ? c->d_Kt2 : Kt;
   a.item_ctr = c->d_item_ctr + 2 * combo;
-  a.jstride = c->mtot;
+  a.jstride = c->jkp_n;
+  a.jkcs = c->cls_start[Y + 1] - c->cls_start[Y];
   a.mixed = mixed;
   if (c->wg_per_cu.empty()) c->wg_per_cu.assign(3 * combo_count(), 0);
   const int la = class_la(X), lb = class_lb(X), lc = class_la(Y), ld = class_lb(Y);
@@ -1014,11 +1066,9 @@ hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad, h
   a.pr = c->pairs();
   a.sy = c->systems();
   a.bra0 = c->cls_start[X];
-  a.seg_lo = c->d_seg_lo;
-  a.cseg = c->d_cseg + (size_t)Y * (c->nsys + 1);
   a.items = reinterpret_cast<const int4*>(c->gr.d_items[combo]);
   a.nitem = c->gr.n_items[combo];
-  a.rng = c->gr.d_rng[combo];
+  a.kets = c->gr.d_kets[combo];
   a.tab64 = c->d_tab64;
   a.tab32 = c->d_tab32;
   a.P = c->d_Ps;
@@ -1230,7 +1280,6 @@ int gx_set_basis_batch(gx_ctx* c, int nsys, const int* sys_shell, const int* sys
   GX_TRY(upload(&c->d_sbfb, c->sbfb, c->stream));
   GX_TRY(upload(&c->d_smb, c->smb, c->stream));
   GX_TRY(dalloc(&c->d_Ps, (size_t)c->mtot));
-  GX_TRY(dalloc(&c->d_Jt, (size_t)c->mtot * GX_JT_COPIES));
   GX_TRY(dalloc(&c->d_Kt, (size_t)c->mtot));
   // two-density buffers: reallocated at the new size on their next use
   dfree(&c->d_Ps2);
@@ -1358,9 +1407,8 @@ int gx_set_basis_batch(gx_ctx* c, int nsys, const int* sys_shell, const int* sys
           return fail(GX_ERR_STATE, "gx_set_basis: too many primitive-count segments");
       }
   }
-  GX_TRY(upload(&c->d_seg_lo, c->seg_lo, c->stream));
-  GX_TRY(upload(&c->d_seg_hi, c->seg_hi, c->stream));
-  GX_TRY(upload(&c->d_cseg, c->cseg, c->stream));
+  GX_TRY(build_pairrec(c));
+  GX_TRY(dalloc(&c->d_Jt, (size_t)c->jt_doubles()));
   c->nab.assign(c->npair, 0);
   c->pre_pnp.assign(c->npair + 1, 0);
   c->pre_nab.assign(c->npair + 1, 0);
@@ -1559,8 +1607,8 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   const unsigned g = (unsigned)((nn + 255) / 256);
   hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, P, c->d_Ps);
-  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((nn * GX_JT_COPIES + 255) / 256)), dim3(256), 0, c->stream,
-                     nn * GX_JT_COPIES, c->d_Jt, 0.0);
+  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((c->jt_doubles() + 255) / 256)), dim3(256), 0, c->stream,
+                     c->jt_doubles(), c->d_Jt, 0.0);
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, fp32_bound, mixed));
@@ -1597,7 +1645,7 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
     }
   GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
-                     c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, K);
+                     c->d_sbfb, c->d_scale, c->d_Jt, c->d_Jt + nn, c->jkp_n, c->d_jkp_of, c->d_Kt, J, K);
   GX_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1610,12 +1658,11 @@ int gx_jk_noalloc(gx_ctx* c, const double* P, double* J, double* K, double cutof
   if (flags & ~GX_JK_MIXED) return fail(GX_ERR_ARG, "gx_jk_noalloc: only GX_JK_MIXED is accepted");
   const int mixed = (flags & GX_JK_MIXED) ? 1 : 0;
   const long nn = c->mtot;
-  const long need = (2L + GX_JT_COPIES) * nn;
+  const long need = 2L * nn + c->jt_doubles();
   if (need > GX_NOALLOC_MAX_DOUBLES)
     return fail(GX_ERR_ARG, "gx_jk_noalloc: the build needs " + std::to_string(need) +
-                                " doubles of scratch, at most " + std::to_string(GX_NOALLOC_MAX_DOUBLES) +
-                                " (sum of n_k^2 <= " + std::to_string(GX_NOALLOC_MAX_DOUBLES / (2 + GX_JT_COPIES)) +
-                                ")");
+                                " doubles of scratch (3 sum_k n_k^2 + " + std::to_string(GX_JT_COPIES) +
+                                " x the pair blocks), at most " + std::to_string(GX_NOALLOC_MAX_DOUBLES));
   double* base = scratch_of(c->device);
   if (!base) return fail(GX_ERR_STATE, "gx_jk_noalloc: no scratch address for this device");
   // the work items are built (and uploaded) by gx_jk / gx_jk_autotune; this
@@ -1642,8 +1689,8 @@ int gx_jk_noalloc(gx_ctx* c, const double* P, double* J, double* K, double cutof
   // cleared by kernels (no memset node, so a captured call stays replayable)
   hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, P, Ps);
-  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((nn * GX_JT_COPIES + 255) / 256)), dim3(256), 0, c->stream,
-                     nn * GX_JT_COPIES, Jt, 0.0);
+  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((c->jt_doubles() + 255) / 256)), dim3(256), 0, c->stream,
+                     c->jt_doubles(), Jt, 0.0);
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, Kt, 0.0);
   GX_HIP(hipGetLastError());
   int combo = 0;
@@ -1653,7 +1700,7 @@ int gx_jk_noalloc(gx_ctx* c, const double* P, double* J, double* K, double cutof
       GX_HIP(launch_combo(c, X, Y, combo, mixed, 1, c->stream, Ps, Jt, Kt));
     }
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
-                     c->d_sbfb, c->d_scale, Jt, Kt, J, K);
+                     c->d_sbfb, c->d_scale, Jt, Jt + nn, c->jkp_n, c->d_jkp_of, Kt, J, K);
   GX_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1686,9 +1733,9 @@ int gx_jk_uhf(gx_ctx* c, const double* Pa, const double* Pb, double* J, double* 
   hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, Pb, c->d_Ps2);
   hipLaunchKernelGGL(gx::k_add, dim3(g), dim3(256), 0, c->stream, nn, c->d_Ps, c->d_Ps2, c->d_Psj);
-  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((nn * GX_JT_COPIES + 255) / 256)), dim3(256), 0, c->stream,
-                     nn * GX_JT_COPIES, c->d_Jt, 0.0);
-  // Jt holds GX_JT_COPIES copies; Kt and Kt2 one each
+  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((c->jt_doubles() + 255) / 256)), dim3(256), 0, c->stream,
+                     c->jt_doubles(), c->d_Jt, 0.0);
+  // Jt and the Jkp copies; Kt and Kt2 one each
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
   hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt2, 0.0);
   GX_HIP(hipGetLastError());
@@ -1704,7 +1751,7 @@ int gx_jk_uhf(gx_ctx* c, const double* Pa, const double* Pb, double* J, double* 
     }
   GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
-                     c->d_sbfb, c->d_scale, c->d_Jt, c->d_Kt, J, Ka);
+                     c->d_sbfb, c->d_scale, c->d_Jt, c->d_Jt + nn, c->jkp_n, c->d_jkp_of, c->d_Kt, J, Ka);
   hipLaunchKernelGGL(gx::k_symmetrize1, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, c->d_Kt2, Kb);
   GX_HIP(hipGetLastError());

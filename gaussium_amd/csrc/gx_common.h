@@ -18,7 +18,7 @@
 
 #define GX_WAVE 64
 
-// Copies of the J accumulator (see jt_copy in gx_eri.hip).
+// Copies of the pair-indexed J_cd accumulator (see jkp_copy in gx_eri.hip).
 #ifndef GX_JT_COPIES
 #define GX_JT_COPIES 8
 #endif
@@ -65,6 +65,19 @@ struct SysView {
   const long* mb;       // offset of the system's matrix in the packed arrays
 };
 
+// One shell pair (A, B) in a single 64-byte record, so that a kernel gets
+// everything it needs about a ket (or bra) pair with one dependent load:
+// A - B, the first basis functions of A and B inside their system, the
+// primitive-pair range, the shells, the system and the pair's first entry
+// in the pair-indexed J accumulator.
+struct __align__(16) PairRec {
+  double ABx, ABy, ABz;
+  int fa, fb;    // first basis function of A and of B inside their system
+  int poff, np;  // primitive pairs [poff, poff + np)
+  int A, B;      // shells
+  int sys, jko;  // system; entry of component 0 in the pair-indexed J accumulator
+};
+
 struct PairView {
   const int* A;
   const int* B;
@@ -73,7 +86,15 @@ struct PairView {
   const double* Q;      // Schwarz bound (scaled)
   const PrimPair64* pp64;
   const PrimPair32* pp32;
+  const PairRec* rec;   // per pair (J/K and gradient kernels)
 };
+
+// Ket lists of the work items: entry = shell-pair id, GX_KET_FP32 set when the
+// quartet runs in FP32, GX_KET_NONE for the idle lanes between a bra's FP64
+// and FP32 kets.
+#define GX_KET_FP32 0x40000000
+#define GX_KET_ID 0x3fffffff
+#define GX_KET_NONE (-1)
 
 // Reciprocal and reciprocal square root from the hardware estimates (about
 // 2^-24 relative on gfx950, measured) plus Newton steps (quadratic

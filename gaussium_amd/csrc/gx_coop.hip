@@ -13,6 +13,10 @@
 // LDS per wave, in doubles: max(NACC + NT, NH) for the quartet; the J/K
 // kernel adds the density blocks and the partial J/K blocks of one quartet.
 
+#ifndef GX_COOP_U
+#define GX_COOP_U 4
+#endif
+
 namespace gx {
 
 template <class G>
@@ -49,6 +53,7 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
                                              const double* __restrict__ tab, double* __restrict__ H) {
   constexpr int W = GX_WAVE / NG;  // lanes per group
+  constexpr int U = GX_COOP_U;     // program entries per lane in flight
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int grp = lane / W, sub = lane - grp * W;
   double* __restrict__ Hg = H + grp * (G::NACC + G::NT);
@@ -82,38 +87,67 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
     for (int m = 0; m <= G::L; ++m)
       if (sub == m) T[1 + m] = F[m];
     __syncthreads();
+    // Entries of one level read only earlier levels, so each lane takes
+    // GX_COOP_U of them at a time: all their descriptor loads, then all their
+    // table reads, then the writes, which keeps U dependency chains in flight
+    // instead of one.
     for (int lev = 1; lev <= G::L; ++lev) {
-      const int e1 = vl[lev + 1];
-      for (int t = vl[lev] + sub; t < e1; t += W) {
-        const int4 en = vp[t];
-        const unsigned x = (unsigned)en.x, y = (unsigned)en.y, z = (unsigned)en.z;
-        const bool ket_step = (x >> 16) & 1u;
-        const unsigned ax = (x >> 17) & 3u;
-        const double k1 = (double)((x >> 20) & 15u), k2 = (double)((x >> 24) & 15u);
-        const double pa = ax == 0 ? b.PAx : (ax == 1 ? b.PAy : b.PAz);
-        const double qc = ax == 0 ? k.PAx : (ax == 1 ? k.PAy : k.PAz);
-        const double wp = ax == 0 ? WPx : (ax == 1 ? WPy : WPz);
-        const double wq = ax == 0 ? WQx : (ax == 1 ? WQy : WQz);
-        const double cA = ket_step ? qc : pa, cB = ket_step ? wq : wp;
-        const double o = ket_step ? o2q : o2p, r = ket_step ? rho_q : rho_p;
-        double v = cA * T[y & 0xffffu];
-        v = fma(cB, T[y >> 16], v);
-        v = fma(k1 * o, T[z & 0xffffu] - r * T[z >> 16], v);
-        v = fma(k2 * o2pq, T[(unsigned)en.w], v);
-        T[x & 0xffffu] = v;
+      const int e0 = vl[lev], e1 = vl[lev + 1];
+      for (int t0 = e0 + sub; t0 < e1; t0 += U * W) {
+        int4 en[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) en[u] = vp[t0 + u * W < e1 ? t0 + u * W : e0];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const unsigned x = (unsigned)en[u].x, y = (unsigned)en[u].y, z = (unsigned)en[u].z;
+          const bool ket_step = (x >> 16) & 1u;
+          const unsigned ax = (x >> 17) & 3u;
+          const double k1 = (double)((x >> 20) & 15u), k2 = (double)((x >> 24) & 15u);
+          const double pa = ax == 0 ? b.PAx : (ax == 1 ? b.PAy : b.PAz);
+          const double qc = ax == 0 ? k.PAx : (ax == 1 ? k.PAy : k.PAz);
+          const double wp = ax == 0 ? WPx : (ax == 1 ? WPy : WPz);
+          const double wq = ax == 0 ? WQx : (ax == 1 ? WQy : WQz);
+          const double cA = ket_step ? qc : pa, cB = ket_step ? wq : wp;
+          const double o = ket_step ? o2q : o2p, r = ket_step ? rho_q : rho_p;
+          double w = cA * T[y & 0xffffu];
+          w = fma(cB, T[y >> 16], w);
+          w = fma(k1 * o, T[z & 0xffffu] - r * T[z >> 16], w);
+          v[u] = fma(k2 * o2pq, T[(unsigned)en[u].w], w);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (t0 + u * W < e1) T[(unsigned)en[u].x & 0xffffu] = v[u];
       }
       __syncthreads();
     }
     if (act) {
       if constexpr (GRAD) {
         const double w1 = 2.0 * b.ea, w2 = 2.0 * (b.p - b.ea), w3 = 2.0 * k.ea;
-        for (int t = sub; t < G::NACC; t += W) {
-          const unsigned e = ap[t], st = e >> 16;
-          const double wt = st == 0 ? 1.0 : (st == 1 ? w1 : (st == 2 ? w2 : w3));
-          Hg[t] = fma(wt, T[e & 0xffffu], Hg[t]);
+        for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
+          unsigned e[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) e[u] = ap[t0 + u * W < G::NACC ? t0 + u * W : 0];
+          double v[U], wt[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const unsigned st = e[u] >> 16;
+            wt[u] = st == 0 ? 1.0 : (st == 1 ? w1 : (st == 2 ? w2 : w3));
+            v[u] = T[e[u] & 0xffffu];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (t0 + u * W < G::NACC) Hg[t0 + u * W] = fma(wt[u], v[u], Hg[t0 + u * W]);
         }
       } else {
-        for (int t = sub; t < G::NACC; t += W) Hg[t] += T[ap[t]];
+        for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
+          double v[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) v[u] = T[ap[t0 + u * W < G::NACC ? t0 + u * W : 0]];
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (t0 + u * W < G::NACC) Hg[t0 + u * W] += v[u];
+        }
       }
     }
     __syncthreads();  // T is rewritten by the next primitive quartet
@@ -131,15 +165,24 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
   const int2* __restrict__ hp = G::hrr();
   const int* __restrict__ hl = G::hlev();
   for (int lev = 0; lev < G::NHLEV; ++lev) {
-    const int e1 = hl[lev + 1];
-    for (int t = hl[lev] + lane; t < e1; t += GX_WAVE) {
-      const int2 en = hp[t];
-      const unsigned x = (unsigned)en.x, y = (unsigned)en.y;
-      const bool ket_step = (x >> 16) & 1u;
-      const unsigned ax = (x >> 17) & 3u;
-      const double ab = ax == 0 ? ABx : (ax == 1 ? ABy : ABz);
-      const double cd = ax == 0 ? CDx : (ax == 1 ? CDy : CDz);
-      H[x & 0xffffu] = fma(ket_step ? cd : ab, H[y >> 16], H[y & 0xffffu]);
+    const int e0 = hl[lev], e1 = hl[lev + 1];
+    for (int t0 = e0 + lane; t0 < e1; t0 += U * GX_WAVE) {
+      int2 en[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) en[u] = hp[t0 + u * GX_WAVE < e1 ? t0 + u * GX_WAVE : e0];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const unsigned x = (unsigned)en[u].x, y = (unsigned)en[u].y;
+        const bool ket_step = (x >> 16) & 1u;
+        const unsigned ax = (x >> 17) & 3u;
+        const double ab = ax == 0 ? ABx : (ax == 1 ? ABy : ABz);
+        const double cd = ax == 0 ? CDx : (ax == 1 ? CDy : CDz);
+        v[u] = fma(ket_step ? cd : ab, H[y >> 16], H[y & 0xffffu]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (t0 + u * GX_WAVE < e1) H[(unsigned)en[u].x & 0xffffu] = v[u];
     }
     __syncthreads();
   }
@@ -242,7 +285,6 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
   constexpr int NG = coop_groups<G>();
   double* __restrict__ pblk = smem + coop_lds_ng<G, NG>();  // P_ab P_cd P_ac P_ad P_bc P_bd
   double* __restrict__ jblk = pblk + NBLK;           // J_ab J_cd K_ac K_ad K_bc K_bd partials
-  __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
   const int lane = threadIdx.x;
   const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
   // static striding over the items (claiming them from a counter, as the
@@ -250,40 +292,22 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
   for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
-    const int A = a.pr.A[ip], B = a.pr.B[ip];
-    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
-    const int sys = a.sh.sys[A];
+    const PairRec br = a.pr.rec[ip];
+    const int ao = br.fa, bo = br.fb;
+    const int sys = br.sys;
     const int N = a.sy.n[sys];
     const long mb = a.sy.mb[sys];
-    const int s0 = a.cseg[sys];
-    const int nseg = a.cseg[sys + 1] - s0;
     const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
-    const PrimPair64* __restrict__ bra = a.pr.pp64 + a.pr.poff[ip] + p0;
-    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
-                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
-    const double fA = (A == B) ? 0.5 : 1.0;
-    __syncthreads();  // the previous item's readers of r_* are done
-    if (lane < nseg) {
-      r_start[lane] = a.seg_lo[s0 + lane];
-      r_pre[lane + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + lane];  // FP64 counts; cooperative classes have no FP32 part
-    }
-    __syncthreads();
-    if (lane == 0) {
-      r_pre[0] = 0;
-      for (int r = 0; r < nseg; ++r) r_pre[r + 1] += r_pre[r];
-    }
-    __syncthreads();
-    const int total = min(r_pre[nseg], it.z);
-    for (int v = it.y; v < total; ++v) {
-      int r = 0;
-      while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
-      const int jp = r_start[r] + (v - r_pre[r]);
-      const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
-      const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
-      coop_quartet<G, false, NG>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
-                      a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
-                      a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], tab, H);
-      double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
+    const PrimPair64* __restrict__ bra = a.pr.pp64 + br.poff + p0;
+    const double ABx = br.ABx, ABy = br.ABy, ABz = br.ABz;
+    const double fA = (br.A == br.B) ? 0.5 : 1.0;
+    for (int v = it.y; v < it.z; ++v) {
+      const int jp = __builtin_amdgcn_readfirstlane(a.kets[v]);  // cooperative classes: FP64 kets only
+      const PairRec kq = a.pr.rec[jp];
+      const int co = kq.fa, dof = kq.fb;
+      coop_quartet<G, false, NG>(bra, nbp, a.pr.pp64 + kq.poff, kq.np, ABx, ABy, ABz, kq.ABx, kq.ABy, kq.ABz,
+                                 tab, H);
+      double f = fA * ((kq.A == kq.B) ? 0.5 : 1.0);
       if (jp == ip) f *= 0.5;
       // density blocks of this quartet; partials cleared
       for (int t = lane; t < NBLK; t += GX_WAVE) {
@@ -330,8 +354,8 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
         int u = t;
         double* dst = a.Kt;
         double m = 1.0;
-        if (u < SAB) { r0 = ao; c0 = bo; s = NB; dst = jt_copy(a); m = 2.0; }
-        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; dst = jt_copy(a); m = 2.0; }
+        if (u < SAB) { r0 = ao; c0 = bo; s = NB; dst = a.Jt; m = 2.0; }
+        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; dst = a.Jt; m = 2.0; }
         else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; s = NC; }
         else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
         else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
@@ -355,11 +379,9 @@ struct GradArgs {
   PairView pr;
   SysView sy;
   int bra0;
-  const int* seg_lo;
-  const int* cseg;     // per system: first ket segment of the ket class
-  const int4* items;
+  const int4* items;   // as JKArgs::items
   int nitem;
-  const int* rng;
+  const int* kets;     // as JKArgs::kets
   const double* tab64;
   const float* tab32;  // FP32 Taylor tables (mixed-precision per-lane kernels)
   const double* P;     // packed densities, pre-scaled by s_k s_l
@@ -390,7 +412,6 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
   extern __shared__ double smem[];
   double* __restrict__ H = smem;
   double* __restrict__ pblk = smem + grad_lds<G>();
-  __shared__ int r_start[JK_MAXSEG], r_pre[JK_MAXSEG + 1];
   __shared__ double gsh[3 * GX_GRAD_MAXATOM];
   const int lane = threadIdx.x;
   const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
@@ -403,47 +424,32 @@ __global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
   for (int w = blockIdx.x; w < a.nitem;) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
-    const int A = a.pr.A[ip], B = a.pr.B[ip];
-    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
-    const int sys = a.sh.sys[A];
+    const PairRec br = a.pr.rec[ip];
+    const int A = br.A, B = br.B;
+    const int ao = br.fa, bo = br.fb;
+    const int sys = br.sys;
     const int N = a.sy.n[sys];
     const long mb = a.sy.mb[sys];
-    const int s0 = a.cseg[sys];
-    const int nseg = a.cseg[sys + 1] - s0;
     const int at0 = a.sys_atom[sys], nat3 = 3 * (a.sys_atom[sys + 1] - at0);
     const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
-    const PrimPair64* __restrict__ bra = a.pr.pp64 + a.pr.poff[ip] + p0;
-    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
-                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
+    const PrimPair64* __restrict__ bra = a.pr.pp64 + br.poff + p0;
+    const double ABx = br.ABx, ABy = br.ABy, ABz = br.ABz;
     const double fA = (A == B) ? 0.5 : 1.0;
-    __syncthreads();
-    if (lane < nseg) {
-      r_start[lane] = a.seg_lo[s0 + lane];
-      r_pre[lane + 1] = a.rng[(long)it.x * 2 * JK_MAXSEG + lane];  // gradients are FP64 only
-    }
     if (at0 != cur) {  // another system: flush the sums held for the previous one
+      __syncthreads();
       for (int t = lane; t < cur3; t += GX_WAVE)
         if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
       for (int t = lane; t < nat3; t += GX_WAVE) gsh[t] = 0.0;
       cur = at0;
       cur3 = nat3;
+      __syncthreads();
     }
-    __syncthreads();
-    if (lane == 0) {
-      r_pre[0] = 0;
-      for (int r = 0; r < nseg; ++r) r_pre[r + 1] += r_pre[r];
-    }
-    __syncthreads();
-    const int total = min(r_pre[nseg], it.z);
-    for (int v = it.y; v < total; ++v) {
-      int r = 0;
-      while (r + 1 < nseg && v >= r_pre[r + 1]) ++r;
-      const int jp = r_start[r] + (v - r_pre[r]);
-      const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
-      const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
-      coop_quartet<G, true>(bra, nbp, a.pr.pp64 + a.pr.poff[jp], a.pr.np[jp], ABx, ABy, ABz,
-                            a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
-                            a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2], tab, H);
+    for (int v = it.y; v < it.z; ++v) {
+      const int jp = __builtin_amdgcn_readfirstlane(a.kets[v]);  // gradients of these classes: FP64 only
+      const PairRec kq = a.pr.rec[jp];
+      const int Cs = kq.A, Ds = kq.B;
+      const int co = kq.fa, dof = kq.fb;
+      coop_quartet<G, true>(bra, nbp, a.pr.pp64 + kq.poff, kq.np, ABx, ABy, ABz, kq.ABx, kq.ABy, kq.ABz, tab, H);
       double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
       if (jp == ip) f *= 0.5;
       for (int t = lane; t < NBLK; t += GX_WAVE) {

@@ -339,21 +339,25 @@ __global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int 
 // ---------------------------------------------------------------------------
 // Fused direct J/K.
 //
-// Persistent workgroups stride over the bra shell pairs of one class (in
+// Persistent workgroups take work items of one class combination (in
 // decreasing-work order); all waves of a workgroup share one bra at a time.
 // For each bra the ket class is split into segments of equal primitive-pair
 // count, each sorted by decreasing Schwarz bound, so the kets surviving the
-// cutoff -- and, in mixed mode, those that stay in FP64 -- are prefixes
-// found by binary search.  The surviving kets form a compact index space the
-// waves sweep 64 at a time (one ket pair per lane).  The bra's primitive
-// pairs and the density rows of its shells sit in LDS.  Digestion:
+// cutoff -- and, in mixed mode, those that stay in FP64 -- are prefixes of
+// the segments.  The host writes each bra's surviving kets as one list (FP64
+// kets, then FP32 kets from the next multiple of 64); an item is a range of
+// that list, which the waves sweep 64 at a time (one ket pair per lane).  The
+// density rows of the bra's shells sit in LDS.  Digestion:
 //   Jt_ab += 2 f v P_cd      (register sums, wave-reduced, one atomic per element)
-//   Jt_cd += 2 f v P_ab      (global FP64 atomics)
+//   J_cd  += 2 f v P_ab      (global FP64 atomics into the pair-indexed Jkp)
 //   Kt_ac += f v P_bd, Kt_ad += f v P_bc, Kt_bc += f v P_ad, Kt_bd += f v P_ac
-//                            (LDS rows of A and B, flushed once per bra: the
+//                            (LDS rows of A and B, flushed once per item: the
 //                            whole wave would otherwise hit the same two rows)
 // with f the 8-fold degeneracy weight; J = Jt + Jt^T, K = Kt + Kt^T.
+// JK_MAXSEG bounds the segments per (class, system) (host bookkeeping).
+#ifndef JK_MAXSEG
 #define JK_MAXSEG 16
+#endif
 #ifndef JK_TEAM
 #define JK_TEAM 256  // threads working on one work item
 #endif
@@ -363,31 +367,36 @@ struct JKArgs {
   PairView pr;
   SysView sy;
   int bra0;
-  const int* seg_lo;     // ket segments: first global pair id of each
-  const int* cseg;       // per system: first ket segment of the ket class (nsys + 1)
-  const int4* items;     // work items: bra position, [v0, v1), p0 | p1 << 16; largest first
+  const int4* items;     // work items: bra position, ket-list range [v0, v1), p0 | p1 << 16; largest first
   int nitem;
-  const int* rng;        // per bra position, 2 JK_MAXSEG ints: surviving kets per segment, FP64 then FP32
+  const int* kets;       // ket lists of the bras (GX_KET_* encoding), indexed by the items' [v0, v1)
   const double* master;  // Boys master table
   const double* tab64;   // per-L FP64 Taylor tables (BOYS_NT x BOYS_ROW64 each)
   const float* tab32;    // per-L FP32 Taylor tables
   const double* P;       // packed densities, pre-scaled by s_k s_l (the K density)
   const double* P2;      // second K density (two-density builds), pre-scaled
   const double* Pj;      // the J density (P, or P + P2 in two-density builds), pre-scaled
-  double* Jt;            // packed accumulators: GX_JT_COPIES copies of jstride doubles for Jt
+  double* Jt;            // packed accumulator of the J_ab parts (one copy)
+  double* Jkp;           // pair-indexed J_cd accumulator: GX_JT_COPIES copies of jstride doubles
   double* Kt;
   double* Kt2;           // K accumulator of the second density (single copy)
   int* item_ctr;         // [next item, workgroups done]: zero before a launch, reset by its last workgroup
-  long jstride;
+  long jstride;          // doubles per copy of Jkp (sum over pairs of n_A n_B)
+  int jkcs;              // Jkp stride between the components of a ket pair (pairs of the ket class)
   int mixed;
 };
 
-// Workgroups are dispatched to the eight XCDs round-robin, so copy
-// blockIdx % GX_JT_COPIES of Jt collects the J atomics of (mostly) one XCD:
-// the same address is then not contended from every XCD at once.  The copies
-// are summed by k_symmetrize.
-__device__ __forceinline__ double* jt_copy(const JKArgs& a) {
-  return a.Jt + (long)(blockIdx.x % GX_JT_COPIES) * a.jstride;
+// J_cd = sum_ab (ab|cd) P_ab arrives from every bra at once, one value per
+// lane and ket pair.  It is accumulated per ket pair (component k of pair p
+// at jko + k * jkcs, jkcs = pairs of the ket class) rather than at (c, d) of
+// the matrix: the kets of a chunk are mostly consecutive pairs, so a wave's
+// atomics hit consecutive addresses (a few cache lines) instead of 64
+// scattered ones.  Workgroups are dispatched
+// to the eight XCDs round-robin, so copy blockIdx % GX_JT_COPIES collects the
+// atomics of (mostly) one XCD: the same address is then not contended from
+// every XCD at once.  k_symmetrize adds the copies into J.
+__device__ __forceinline__ double* jkp_copy(const JKArgs& a) {
+  return a.Jkp + (long)(blockIdx.x % GX_JT_COPIES) * a.jstride;
 }
 
 // MIXED = false: pure FP64 build, the FP32 path is compiled out and the FP64
@@ -403,7 +412,6 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
   extern __shared__ double smem[];
-  __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1], r_len[2 * JK_MAXSEG];
   __shared__ double jab_s[NA * NB];
   __shared__ int next_chunk;
   const int tid = threadIdx.x;
@@ -430,33 +438,30 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
   __shared__ int next_item;
   for (int w = blockIdx.x; w < a.nitem;) {
     const int4 it = a.items[w];
-    const int bpos = it.x;
-    const int ip = a.bra0 + bpos;
-    const int A = a.pr.A[ip], B = a.pr.B[ip];
-    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
+    const int ip = a.bra0 + it.x;
+    const PairRec br = a.pr.rec[ip];
+    const int A = br.A, B = br.B;
+    const int ao = br.fa, bo = br.fb;
     const int rowB = (A == B) ? 0 : NA;
-    const int sys = a.sh.sys[A];
+    const int sys = br.sys;
     const int N = a.sy.n[sys];
     const double* __restrict__ Pm = a.P + a.sy.mb[sys];
-    double* __restrict__ Jm = jt_copy(a) + a.sy.mb[sys];
+    double* __restrict__ Jm = a.Jt + a.sy.mb[sys];
+    double* __restrict__ Jkp = jkp_copy(a);
     double* __restrict__ Km = a.Kt + a.sy.mb[sys];
     double* __restrict__ Km2 = (NDEN == 2 ? a.Kt2 : a.Kt) + a.sy.mb[sys];
     const double* __restrict__ Pm2 = (NDEN == 2 ? a.P2 : a.P) + a.sy.mb[sys];
     const double* __restrict__ Pjm = a.Pj + a.sy.mb[sys];
     const int nrk = (NA + NB) * N;  // one density's rows
     double* krow = prow + NDEN * nrk;
-    const int s0 = a.cseg[sys];
-    const int nseg = a.cseg[sys + 1] - s0;
-    const int nr = 2 * nseg;  // range r < nseg: FP64 part of segment r; else FP32 part of r - nseg
     // wave-uniform: kept in scalar registers so that the bra loop reads go
     // through the scalar data path
     const int p0 = __builtin_amdgcn_readfirstlane(it.w & 0xffff);
     const int nbp = __builtin_amdgcn_readfirstlane((it.w >> 16) - (it.w & 0xffff));
-    const int bpoff = __builtin_amdgcn_readfirstlane(a.pr.poff[ip]) + p0;
+    const int bpoff = __builtin_amdgcn_readfirstlane(br.poff) + p0;
     const PrimPair64* __restrict__ bra64 = pp64 + bpoff;
     const PrimPair32* __restrict__ bra32 = pp32 + bpoff;
-    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
-                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
+    const double ABx = br.ABx, ABy = br.ABy, ABz = br.ABz;
     __syncthreads();  // the previous item's flush is complete
     for (int i = tid; i < nrk; i += JK_TEAM) {
       const int r = i / N, c = i - r * N;
@@ -470,34 +475,8 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
     }
     if (tid < NA * NB) jab_s[tid] = 0.0;
     if (tid == 0) next_chunk = nwave;
-    if (tid < nseg) {
-      const int* rr = a.rng + (long)bpos * 2 * JK_MAXSEG;
-      const int n64 = rr[tid], n32 = rr[JK_MAXSEG + tid];
-      // ranges: all FP64 parts first (r = sg), then all FP32 parts (r = nseg + sg),
-      // so that a bra has at most one precision boundary
-      r_start[tid] = a.seg_lo[s0 + tid];
-      r_start[nseg + tid] = a.seg_lo[s0 + tid] + n64;
-      r_pre[tid + 1] = n64;  // lengths for now
-      r_pre[nseg + tid + 1] = n32;
-    }
     __syncthreads();
-    if (tid == 0) {
-      // r_pre[r]: virtual start of range r.  The FP32 ranges start at the next
-      // multiple of 64, so no chunk of 64 mixes precisions (the lanes of the
-      // gap stay idle).
-      int acc = 0;
-      for (int r = 0; r < nr; ++r) {
-        const int len = r_pre[r + 1];
-        if (r == nseg) acc = (acc + GX_WAVE - 1) & ~(GX_WAVE - 1);
-        r_pre[r + 1] = acc + len;  // end of range r for now
-        r_len[r] = len;
-        acc += len;
-      }
-      for (int r = 0; r < nr; ++r) r_pre[r] = r_pre[r + 1] - r_len[r];  // starts
-      r_pre[nr] = acc;
-    }
-    __syncthreads();
-    const int total = min(r_pre[nr], it.z);
+    const int v1 = it.z;
     const double fA = (A == B) ? 0.5 : 1.0;
     double jab[NA * NB];
 #pragma unroll
@@ -505,45 +484,34 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
 
 #if GX_DYN_CHUNK
     // waves take the item's chunks of 64 kets in turn from an LDS counter
-    for (int v0 = it.y + wave * GX_WAVE; v0 < total;) {
+    for (int v0 = it.y + wave * GX_WAVE; v0 < v1;) {
 #else
-    for (int v0 = it.y + wave * GX_WAVE; v0 < total; v0 += nwave * GX_WAVE) {
+    for (int v0 = it.y + wave * GX_WAVE; v0 < v1; v0 += nwave * GX_WAVE) {
 #endif
       const int v = v0 + lane;
-      int r = 0;
-      if (v < total)
-        while (r + 1 < nr && v >= r_pre[r + 1]) ++r;
-      const bool act = v < total && v < r_pre[r] + r_len[r];
-      const int jp = act ? r_start[r] + (v - r_pre[r]) : 0;
-      const bool lo = act && r >= nseg;
-      const bool hi = act && r < nseg;
+      const int ke = v < v1 ? a.kets[v] : GX_KET_NONE;
+      const bool act = ke >= 0;
+      const int jp = act ? (ke & GX_KET_ID) : 0;
+      const bool lo = act && (ke & GX_KET_FP32);
+      const bool hi = act && !(ke & GX_KET_FP32);
+      const PairRec kq = a.pr.rec[jp];
       double val[C::NOUT];
-      int Cs = 0, Ds = 0;
-      double CDx = 0.0, CDy = 0.0, CDz = 0.0;
-      if (act) {
-        Cs = a.pr.A[jp];
-        Ds = a.pr.B[jp];
-        CDx = a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds];
-        CDy = a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1];
-        CDz = a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2];
-      }
+      const double CDx = kq.ABx, CDy = kq.ABy, CDz = kq.ABz;
       if (hi)
-        quartet64_jk<LA, LB, LC, LD, MIXED ? GX_JKGM : GX_JKG>(bra64, nbp, pp64 + a.pr.poff[jp],
-                                                              a.pr.np[jp], ABx, ABy, ABz, CDx, CDy, CDz,
-                                                              tab64, val);
+        quartet64_jk<LA, LB, LC, LD, MIXED ? GX_JKGM : GX_JKG>(bra64, nbp, pp64 + kq.poff, kq.np, ABx, ABy,
+                                                              ABz, CDx, CDy, CDz, tab64, val);
       if constexpr (MIXED) {
         if (lo) {
           float vf[C::NOUT];
-          quartet32_jk<LA, LB, LC, LD>(bra32, nbp, pp32 + a.pr.poff[jp], a.pr.np[jp], (float)ABx,
-                                       (float)ABy, (float)ABz, (float)CDx, (float)CDy, (float)CDz, tab32,
-                                       vf);
+          quartet32_jk<LA, LB, LC, LD>(bra32, nbp, pp32 + kq.poff, kq.np, (float)ABx, (float)ABy, (float)ABz,
+                                       (float)CDx, (float)CDy, (float)CDz, tab32, vf);
 #pragma unroll
           for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
         }
       }
       if (act) {
-        const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
-        double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
+        const int co = kq.fa, dof = kq.fb;
+        double f = fA * ((kq.A == kq.B) ? 0.5 : 1.0);
         if (jp == ip) f *= 0.5;
 #pragma unroll
         for (int k = 0; k < C::NOUT; ++k) val[k] *= f;
@@ -574,7 +542,7 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
         for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
           for (int id = 0; id < ND; ++id)
-            atomicAdd(&Jm[(long)(co + ic) * N + dof + id], 2.0 * jcd[ic * ND + id]);
+            atomicAdd(&Jkp[kq.jko + (long)(ic * ND + id) * a.jkcs], 2.0 * jcd[ic * ND + id]);
 #pragma unroll
         for (int d = 0; d < NDEN; ++d) {
           const double* __restrict__ pr = prow + d * nrk;
@@ -688,10 +656,13 @@ __device__ __forceinline__ void packed_index(long t, int nsys, const long* __res
 }
 
 // J = s (Jt + Jt^T) s, K = s (Kt + Kt^T) s per system (scale by global
-// index); Jt is the sum of its GX_JT_COPIES copies (stride `total`).
+// index), where Jt also takes the pair-indexed J_cd sums: element t of the
+// packed matrices is entry jkp_of[t] of every Jkp copy (-1: t lies in the
+// transposed half of a shell-pair block, whose sums sit at the transpose).
 __global__ void k_symmetrize(long total, int nsys, const long* __restrict__ mb, const int* __restrict__ n,
                              const int* __restrict__ bfb, const double* __restrict__ scale,
-                             const double* __restrict__ Jt, const double* __restrict__ Kt,
+                             const double* __restrict__ Jt, const double* __restrict__ Jkp, long jstride,
+                             const int* __restrict__ jkp_of, const double* __restrict__ Kt,
                              double* __restrict__ J, double* __restrict__ K) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -699,9 +670,16 @@ __global__ void k_symmetrize(long total, int nsys, const long* __restrict__ mb, 
   packed_index(t, nsys, mb, n, s, i, j);
   const double f = scale[bfb[s] + i] * scale[bfb[s] + j];
   const long tt = mb[s] + (long)j * n[s] + i;
-  double x = 0.0;
+  double x = Jt[t] + Jt[tt];
+  const int e1 = jkp_of[t], e2 = jkp_of[tt];
+  if (e1 >= 0) {
 #pragma unroll
-  for (int k = 0; k < GX_JT_COPIES; ++k) x += Jt[k * total + t] + Jt[k * total + tt];
+    for (int k = 0; k < GX_JT_COPIES; ++k) x += Jkp[k * jstride + e1];
+  }
+  if (e2 >= 0) {
+#pragma unroll
+    for (int k = 0; k < GX_JT_COPIES; ++k) x += Jkp[k * jstride + e2];
+  }
   J[t] = f * x;
   K[t] = f * (Kt[t] + Kt[tt]);
 }

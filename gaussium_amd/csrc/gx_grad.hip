@@ -108,7 +108,6 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
   using C = EriGradLane<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
   extern __shared__ double prow[];  // P rows of the bra's shells
-  __shared__ int r_start[2 * JK_MAXSEG], r_pre[2 * JK_MAXSEG + 1], r_len[2 * JK_MAXSEG];
   __shared__ int next_chunk;
   __shared__ double gsh[3 * GX_GRAD_MAXATOM];
   const int tid = threadIdx.x;
@@ -126,22 +125,20 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
   for (int w = blockIdx.x; w < a.nitem;) {
     const int4 it = a.items[w];
     const int ip = a.bra0 + it.x;
-    const int A = a.pr.A[ip], B = a.pr.B[ip];
-    const int ao = a.sh.lbf[A], bo = a.sh.lbf[B];
+    const PairRec br = a.pr.rec[ip];
+    const int A = br.A, B = br.B;
+    const int ao = br.fa, bo = br.fb;
     const int rowB = (A == B) ? 0 : NA;
-    const int sys = a.sh.sys[A];
+    const int sys = br.sys;
     const int N = a.sy.n[sys];
     const double* __restrict__ Pm = a.P + a.sy.mb[sys];
-    const int s0 = a.cseg[sys];
-    const int nseg = a.cseg[sys + 1] - s0;
     const int at0 = a.sys_atom[sys], nat3 = 3 * (a.sys_atom[sys + 1] - at0);
     const int p0 = __builtin_amdgcn_readfirstlane(it.w & 0xffff);
     const int nbp = __builtin_amdgcn_readfirstlane((it.w >> 16) - (it.w & 0xffff));
-    const int bpoff = __builtin_amdgcn_readfirstlane(a.pr.poff[ip]) + p0;
+    const int bpoff = __builtin_amdgcn_readfirstlane(br.poff) + p0;
     const PrimPair64* __restrict__ bra = a.pr.pp64 + bpoff;
-    const double ABx = a.sh.ctr[3 * A] - a.sh.ctr[3 * B], ABy = a.sh.ctr[3 * A + 1] - a.sh.ctr[3 * B + 1],
-                 ABz = a.sh.ctr[3 * A + 2] - a.sh.ctr[3 * B + 2];
-    __syncthreads();  // the previous item's readers of prow / r_* are done
+    const double ABx = br.ABx, ABy = br.ABy, ABz = br.ABz;
+    __syncthreads();  // the previous item's readers of prow are done
     for (int i = tid; i < (NA + NB) * N; i += JK_TEAM) {
       const int r = i / N, c = i - r * N;
       prow[i] = Pm[(long)(r < NA ? ao + r : bo + (r - NA)) * N + c];
@@ -154,62 +151,33 @@ __global__ __launch_bounds__(JK_TEAM) void k_grad_lane(GradArgs a) {
       cur = at0;
       cur3 = nat3;
     }
-    // ranges r < nseg: FP64 part of segment r; r >= nseg (MIXED): FP32 part
-    // of segment r - nseg, starting at the next multiple of 64 so that no
-    // chunk mixes precisions (the layout of the host's work items)
-    const int nr = MIXED ? 2 * nseg : nseg;
-    if (tid < nseg) {
-      const int* rr = a.rng + (long)it.x * 2 * JK_MAXSEG;
-      r_start[tid] = a.seg_lo[s0 + tid];
-      r_pre[tid + 1] = rr[tid];
-      if (MIXED) {
-        r_start[nseg + tid] = a.seg_lo[s0 + tid] + rr[tid];
-        r_pre[nseg + tid + 1] = rr[JK_MAXSEG + tid];
-      }
-    }
     __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int r = 0; r < nr; ++r) {
-        const int len = r_pre[r + 1];
-        if (r == nseg) acc = (acc + GX_WAVE - 1) & ~(GX_WAVE - 1);
-        r_pre[r + 1] = acc + len;
-        r_len[r] = len;
-        acc += len;
-      }
-      for (int r = 0; r < nr; ++r) r_pre[r] = r_pre[r + 1] - r_len[r];
-      r_pre[nr] = acc;
-    }
-    __syncthreads();
-    const int total = min(r_pre[nr], it.z);
+    const int v1 = it.z;
     const double fA = (A == B) ? 0.5 : 1.0;
     double gab[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) gab[j] = 0.0;
-    for (int v0 = it.y + wave * GX_WAVE; v0 < total;) {
+    for (int v0 = it.y + wave * GX_WAVE; v0 < v1;) {
       const int v = v0 + lane;
-      int r = 0;
-      if (v < total)
-        while (r + 1 < nr && v >= r_pre[r + 1]) ++r;
-      const bool act = v < total && v < r_pre[r] + r_len[r];
-      if (act) {
-        const int jp = r_start[r] + (v - r_pre[r]);
-        const int Cs = a.pr.A[jp], Ds = a.pr.B[jp];
-        const int co = a.sh.lbf[Cs], dof = a.sh.lbf[Ds];
-        const PrimPair64* __restrict__ kp = a.pr.pp64 + a.pr.poff[jp];
-        const int nk = a.pr.np[jp];
+      const int ke = v < v1 ? a.kets[v] : GX_KET_NONE;
+      if (ke >= 0) {
+        const int jp = ke & GX_KET_ID;
+        const PairRec kq = a.pr.rec[jp];
+        const int Cs = kq.A, Ds = kq.B;
+        const int co = kq.fa, dof = kq.fb;
+        const PrimPair64* __restrict__ kp = a.pr.pp64 + kq.poff;
+        const int nk = kq.np;
         double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
         if (jp == ip) f *= 0.5;
-        const double CDx = a.sh.ctr[3 * Cs] - a.sh.ctr[3 * Ds], CDy = a.sh.ctr[3 * Cs + 1] - a.sh.ctr[3 * Ds + 1],
-                     CDz = a.sh.ctr[3 * Cs + 2] - a.sh.ctr[3 * Ds + 2];
+        const double CDx = kq.ABx, CDy = kq.ABy, CDz = kq.ABz;
         double g[9];
 #pragma unroll
         for (int j = 0; j < 9; ++j) g[j] = 0.0;
-        if (!MIXED || r < nseg) {
+        if (!MIXED || !(ke & GX_KET_FP32)) {
           grad_lane_quartet<LA, LB, LC, LD, double, PrimPair64>(bra, nbp, kp, nk, tab, ABx, ABy, ABz, CDx, CDy, CDz,
                                                                 prow, Pm, N, rowB, bo, co, dof, f, g);
         } else if constexpr (MIXED) {
-          grad_lane_quartet<LA, LB, LC, LD, float, PrimPair32>(a.pr.pp32 + bpoff, nbp, a.pr.pp32 + a.pr.poff[jp],
+          grad_lane_quartet<LA, LB, LC, LD, float, PrimPair32>(a.pr.pp32 + bpoff, nbp, a.pr.pp32 + kq.poff,
                                                                nk, tab32, ABx, ABy, ABz, CDx, CDy, CDz, prow, Pm,
                                                                N, rowB, bo, co, dof, f, g);
         }

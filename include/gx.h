@@ -157,9 +157,10 @@ int gx_jk(gx_ctx* ctx, const double* P_dev, double* J_dev, double* K_dev, double
 /* gx_jk that allocates nothing.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops).  The scaled density and the J/K accumulators live in
- * one file-scope device array of the library (5 MiB per device,
- * (2 + 8) sum_k n_k^2 <= 655360 doubles, i.e. one system of at most 256
- * functions); larger inputs are rejected with GX_ERR_ARG.  The work items
+ * one file-scope device array of the library (5 MiB per device: at most
+ * 655360 doubles for the scaled density, Kt and Jt, 3 sum_k n_k^2, and 8
+ * copies of the pair-indexed J accumulator, about 8 n^2 / 2 -- one system
+ * of up to 256 functions fits); larger inputs are rejected with GX_ERR_ARG.  The work items
  * must already exist for these thresholds (a gx_jk call with the same
  * cutoff, fp32_bound and GX_JK_MIXED flag builds them; otherwise
  * GX_ERR_STATE).  Only GX_JK_MIXED is accepted; every kernel runs on the

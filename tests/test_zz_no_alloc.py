@@ -57,7 +57,7 @@ def test_noalloc_errors():
     rc, _, _ = _call(eng, P, 1e-11, 0.0, _lib.GX_JK_TIMING)
     assert rc == -1
     eng.close()
-    # 330 functions: (2 + 8) * 330^2 doubles exceed the scratch
+    # 330 functions: 3 * 330^2 + 8 * (about 330^2 / 2) doubles exceed the scratch
     nuc, basis, ne, info = workloads.cluster(30, 30, 14.0, seed=7)
     assert len(basis) > 256
     eng = Engine(basis, nuc)
