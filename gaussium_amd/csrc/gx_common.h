@@ -224,19 +224,23 @@ __device__ __forceinline__ void boys32(float T, float scale, const float* __rest
 // Branch-free variants: both the table and the asymptotic forms are
 // evaluated (index clamped to the table) and one is selected, so that a loop
 // over several primitive quartets stays a single basic block and the table
-// gathers of all of them can be in flight together.
+// gathers of all of them can be in flight together.  The nearest grid point
+// comes from adding 2^52 (2^23 in FP32): the sum's low mantissa bits are
+// rint(T / BOYS_DT), with no float-integer conversions; the row offset is a
+// 24-bit multiply.
 template <int L>
 __device__ __forceinline__ void boys64_nb(double T, double scale, const double* __restrict__ tab,
                                           double (&F)[L + 1]) {
   const bool small = T < BOYS_TMAX;
-  const double Tc = small ? T : 0.0;
-  const int n = (int)(Tc * BOYS_INV_DT + 0.5);
-  const double x = n * BOYS_DT - Tc;
-  const double* row = tab + n * BOYS_ROW64;
+  const double Tc = fmin(T, BOYS_TMAX);  // row BOYS_NT - 1 for T >= BOYS_TMAX (value unused)
+  const double tn = fma(Tc, BOYS_INV_DT, 4503599627370496.0);  // 2^52 + rint(Tc / dT)
+  const int n = __double2loint(tn);
+  const double x = fma(tn - 4503599627370496.0, BOYS_DT, -Tc);  // -(T - T_n)
+  const double* row = tab + __mul24(n, BOYS_ROW64);
   double f = row[BOYS_TAYLOR64 - 1];
 #pragma unroll
   for (int k = BOYS_TAYLOR64 - 2; k >= 0; --k) f = fma(f, x, row[k]);
-  const double rt = gx_rsqrt(small ? 1.0 : T);
+  const double rt = gx_rsqrt(fmax(T, BOYS_TMAX));  // used only for T >= BOYS_TMAX
   const double it = rt * rt;
   double g = 0.88622692545275801365 * rt;  // asymptotic F_0
   if (L == 0) {
@@ -274,14 +278,15 @@ template <int L>
 __device__ __forceinline__ void boys32_nb(float T, float scale, const float* __restrict__ tab,
                                           float (&F)[L + 1]) {
   const bool small = T < (float)BOYS_TMAX;
-  const float Tc = small ? T : 0.0f;
-  const int n = (int)(Tc * (float)BOYS_INV_DT + 0.5f);
-  const float x = n * (float)BOYS_DT - Tc;
-  const float* row = tab + n * BOYS_ROW32;
+  const float Tc = fminf(T, (float)BOYS_TMAX);
+  const float tn = fmaf(Tc, (float)BOYS_INV_DT, 8388608.0f);  // 2^23 + rint(Tc / dT)
+  const int n = __float_as_int(tn) - 0x4b000000;
+  const float x = fmaf(tn - 8388608.0f, (float)BOYS_DT, -Tc);
+  const float* row = tab + __mul24(n, BOYS_ROW32);
   float f = row[BOYS_TAYLOR32 - 1];
 #pragma unroll
   for (int k = BOYS_TAYLOR32 - 2; k >= 0; --k) f = fmaf(f, x, row[k]);
-  const float rt = gx_rsqrt(small ? 1.0f : T);
+  const float rt = gx_rsqrt(fmaxf(T, (float)BOYS_TMAX));
   const float it = rt * rt;
   float g = 0.886226925f * rt;
   if (L == 0) {
