@@ -16,6 +16,9 @@
 #ifndef GX_COOP_U
 #define GX_COOP_U 4
 #endif
+#ifndef GX_COOP_U_MIN_NT
+#define GX_COOP_U_MIN_NT 800
+#endif
 
 namespace gx {
 
@@ -53,7 +56,9 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
                                              const double* __restrict__ tab, double* __restrict__ H) {
   constexpr int W = GX_WAVE / NG;  // lanes per group
-  constexpr int U = GX_COOP_U;     // program entries per lane in flight
+  // program entries per lane in flight: measured to pay off only for the
+  // large programs (VRR tables of 800 doubles and more)
+  constexpr int U = G::NT >= GX_COOP_U_MIN_NT ? GX_COOP_U : 1;
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int grp = lane / W, sub = lane - grp * W;
   double* __restrict__ Hg = H + grp * (G::NACC + G::NT);
