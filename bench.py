@@ -65,7 +65,7 @@ def kernel_name(combo, gradient=False, mixed=False):
             return 'k_grad_coop<%d, %d, %d, %d>' % combo
         return 'k_grad_lane<%d, %d, %d, %d, %s>' % (la, lb, lc, ld, 'true' if mixed else 'false')
     if coop:
-        return 'k_jk_coop<%d, %d, %d, %d>' % combo
+        return 'k_jk_rys<%d, %d, %d, %d>' % combo
     return 'k_jk<%d, %d, %d, %d, %s, 1>' % (la, lb, lc, ld, 'true' if mixed else 'false')
 
 

@@ -1,8 +1,9 @@
 """Build the gx HIP library (libgx.so) for gfx950, in-tree.
 
 ``python -m gaussium_amd.build`` or ``gaussium_amd.build.build()``.
-The code generator writes csrc/_gen/eri_gen.h; hipcc compiles the single
-translation unit csrc/gx_api.hip into gaussium_amd/libgx.so.
+The code generators write csrc/_gen/eri_gen.h and csrc/_gen/rys_tab.h; hipcc
+compiles the single translation unit csrc/gx_api.hip into
+gaussium_amd/libgx.so.
 """
 import os
 import subprocess
@@ -26,13 +27,26 @@ HIPCC_FLAGS = [
 ]
 
 
+def generate_rys(out_dir, lmax=ERI_LMAX):
+    """Rys tables and 2D-integral code (gen_rys.py, about 30 s of 60-digit
+    arithmetic); kept while newer than the generator."""
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, 'rys_tab.h')
+    gen = os.path.join(CSRC, 'gen_rys.py')
+    if not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(gen):
+        subprocess.run([sys.executable, gen, out + '.tmp', str(lmax)], check=True)
+        os.replace(out + '.tmp', out)
+    return out
+
+
 def build(verbose=False):
     os.makedirs(GEN, exist_ok=True)
     subprocess.run([sys.executable, os.path.join(CSRC, 'gen_eri.py'),
                     os.path.join(GEN, 'eri_gen.h'), str(ERI_LMAX)], check=True)
+    generate_rys(GEN)
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     tmp = LIB + '.tmp'
-    cmd = [hipcc] + HIPCC_FLAGS + ['-I', os.path.join(ROOT, 'include'), '-I', CSRC,
+    cmd = [hipcc] + HIPCC_FLAGS + ['-I', os.path.join(ROOT, 'include'), '-I', CSRC, '-I', GEN,
                                    os.path.join(CSRC, 'gx_api.hip'), '-o', tmp]
     if verbose:
         print(' '.join(cmd), flush=True)

@@ -17,6 +17,7 @@
 // Single translation unit: the kernels are templates instantiated below.
 #include "gx_eri.hip"
 #include "gx_coop.hip"
+#include "gx_rys.hip"
 #include "gx_grad.hip"
 #include "gx_one.hip"
 
@@ -453,7 +454,7 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
 #undef GX_CASE
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
-    const void* fn = (const void*)gx::k_jk_coop<A, B, C, D>;                                  \
+    const void* fn = (const void*)gx::k_jk_rys<A, B, C, D>;                                   \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
     if (*per_cu <= 0) {                                                                       \
@@ -462,7 +463,7 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
       if (*per_cu < 1) *per_cu = 1;                                                           \
     }                                                                                         \
     const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
-    hipLaunchKernelGGL((gx::k_jk_coop<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a);   \
+    hipLaunchKernelGGL((gx::k_jk_rys<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a);    \
     return hipGetLastError();                                                                 \
   }
   GX_FOR_EACH_COOP_CLASS(GX_CASE)
@@ -529,18 +530,18 @@ hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s
   return hipErrorInvalidValue;
 }
 
-// Whether a class combination runs on the cooperative kernels (one quartet
-// per wave; always FP64).
+// Whether a class combination runs one quartet per wave (the Rys J/K kernel,
+// the cooperative gradient, dense and Schwarz kernels; always FP64).
 bool combo_is_coop(int X, int Y) { return gx_class_is_coop(class_la(X), class_lb(X), class_la(Y), class_lb(Y)); }
 
 // Must match the layout at the top of gx::k_jk (per-lane classes) or
-// gx::k_jk_coop (cooperative classes).
+// gx::k_jk_rys (the classes evaluated one quartet per wave).
 size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed, int nden = 1) {
 #define GX_CASE(A, B, C, D)                                                                     \
   if (la == A && lb == B && lc == C && ld == D) {                                               \
     using G = EriProg<A, B, C, D>;                                                              \
     const int nblk = (G::NA + G::NB) * (G::NC + G::ND) + G::NA * G::NB + G::NC * G::ND;          \
-    return (size_t)(gx::coop_lds_ng<G, gx::coop_groups<G>()>() + 2 * nblk) * sizeof(double);   \
+    return (size_t)(gx::rys_lds<A, B, C, D>() + 2 * nblk) * sizeof(double);                    \
   }
   GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE

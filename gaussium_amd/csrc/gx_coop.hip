@@ -1,17 +1,18 @@
 // Wave-cooperative ERI classes for gfx950.
 //
 // Classes whose contracted [e0|f0] intermediates do not fit in one lane's
-// registers -- (dp|pp), (dp|ds), (dp|dp) and the dd classes -- are evaluated
-// by a whole wave on one shell quartet.  The VRR table [e0|f0]^(m), the
+// registers -- (dp|pp), (dp|dp) and the dd classes with a ket HRR -- are
+// evaluated by a whole wave on one shell quartet: for the Schwarz bounds,
+// the dense and listed ERIs and the nuclear gradient (the J/K build of these
+// classes uses the Rys kernel of gx_rys.hip).  The VRR table [e0|f0]^(m), the
 // contracted accumulators and the HRR table sit in LDS, and the 64 lanes
 // share the entries of each recursion level.  The recursions are the same as
 // the per-lane class code (obara_saika_scheme.py:131-146 of the reference, in
 // the VRR + HRR arrangement of head_gordon_pople.py:111-181); gen_eri.py
-// writes them out as level-ordered programs (EriProg<...>).
+// writes them out as level-ordered programs (EriProg<...>, EriGradProg<...>).
 //
 // One workgroup is one wave, so __syncthreads() is the level barrier.
-// LDS per wave, in doubles: max(NACC + NT, NH) for the quartet; the J/K
-// kernel adds the density blocks and the partial J/K blocks of one quartet.
+// LDS per wave, in doubles: max(NACC + NT, NH) for the quartet.
 
 #ifndef GX_COOP_U
 #define GX_COOP_U 4
@@ -27,41 +28,22 @@ constexpr int coop_lds() {
   return G::NACC + G::NT > G::NH ? G::NACC + G::NT : G::NH;
 }
 
-// With NG lane groups (see coop_quartet): NG accumulator + VRR tables.
-template <class G, int NG>
-constexpr int coop_lds_ng() {
-  return NG * (G::NACC + G::NT) > G::NH ? NG * (G::NACC + G::NT) : G::NH;
-}
-
-// Lane groups of the J/K kernel: two for the smaller programs, whose VRR
-// levels (15-100 entries) leave half of a 64-lane pass idle; one for the
-// larger ones, where the doubled LDS costs more than the fuller passes gain
-// (measured on the d-shell classes: (dp|ds) -25%, (dd|ds) +25% with two).
-template <class G>
-constexpr int coop_groups() {
-  return G::NACC + G::NT <= 512 ? 2 : 1;
-}
-
 // Contracted shell quartet into H: on return H[G::out()[k]] is component k
 // (reference order), unscaled.  Every lane of the wave calls it.  With
 // GRAD, G is a gradient program (EriGradProg): accumulator entries carry a
 // weight set (1, 2 alpha_a, 2 alpha_b, 2 alpha_c) and slot NACC stays zero.
-// NG > 1: the wave splits into NG groups of 64 / NG lanes, each running the
-// VRR of every NG-th primitive quartet on its own table (H + g (NACC + NT))
-// into its own accumulators; the group sums go to H before the HRR.  Small
-// recursion levels then keep more lanes busy.
-template <class G, bool GRAD = false, int NG = 1>
+template <class G, bool GRAD = false>
 __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra, int nb,
                                              const PrimPair64* __restrict__ ket, int nk, double ABx,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
                                              const double* __restrict__ tab, double* __restrict__ H) {
-  constexpr int W = GX_WAVE / NG;  // lanes per group
+  constexpr int W = GX_WAVE;
   // program entries per lane in flight: measured to pay off only for the
   // large programs (VRR tables of 800 doubles and more)
   constexpr int U = G::NT >= GX_COOP_U_MIN_NT ? GX_COOP_U : 1;
-  const int lane = threadIdx.x & (GX_WAVE - 1);
-  const int grp = lane / W, sub = lane - grp * W;
-  double* __restrict__ Hg = H + grp * (G::NACC + G::NT);
+  const int sub = threadIdx.x & (GX_WAVE - 1);
+  const int lane = sub;
+  double* __restrict__ Hg = H;
   double* __restrict__ T = Hg + G::NACC;
   const int4* __restrict__ vp = G::vrr();
   const int* __restrict__ vl = G::vlev();
@@ -70,10 +52,8 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
   for (int t = sub; t < G::NACC; t += W) Hg[t] = 0.0;
   if (sub == 0) T[0] = 0.0;
   const int nq = nk * nb;
-  for (int q0 = 0; q0 < nq; q0 += NG) {
-    const int q = q0 + grp;
-    const bool act = q < nq;  // this group has a primitive quartet this round
-    const int ik = act ? q / nb : 0, ib = act ? q - (q / nb) * nb : 0;
+  for (int q = 0; q < nq; ++q) {
+    const int ik = q / nb, ib = q - ik * nb;
     const PrimPair64 k = ket[ik];
     const PrimPair64 b = bra[ib];
     const double rs = gx_rsqrt(b.p + k.p);
@@ -126,46 +106,34 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
       }
       __syncthreads();
     }
-    if (act) {
-      if constexpr (GRAD) {
-        const double w1 = 2.0 * b.ea, w2 = 2.0 * (b.p - b.ea), w3 = 2.0 * k.ea;
-        for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
-          unsigned e[U];
+    if constexpr (GRAD) {
+      const double w1 = 2.0 * b.ea, w2 = 2.0 * (b.p - b.ea), w3 = 2.0 * k.ea;
+      for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
+        unsigned e[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) e[u] = ap[t0 + u * W < G::NACC ? t0 + u * W : 0];
-          double v[U], wt[U];
+        for (int u = 0; u < U; ++u) e[u] = ap[t0 + u * W < G::NACC ? t0 + u * W : 0];
+        double v[U], wt[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const unsigned st = e[u] >> 16;
-            wt[u] = st == 0 ? 1.0 : (st == 1 ? w1 : (st == 2 ? w2 : w3));
-            v[u] = T[e[u] & 0xffffu];
-          }
-#pragma unroll
-          for (int u = 0; u < U; ++u)
-            if (t0 + u * W < G::NACC) Hg[t0 + u * W] = fma(wt[u], v[u], Hg[t0 + u * W]);
+        for (int u = 0; u < U; ++u) {
+          const unsigned st = e[u] >> 16;
+          wt[u] = st == 0 ? 1.0 : (st == 1 ? w1 : (st == 2 ? w2 : w3));
+          v[u] = T[e[u] & 0xffffu];
         }
-      } else {
-        for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
-          double v[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) v[u] = T[ap[t0 + u * W < G::NACC ? t0 + u * W : 0]];
+        for (int u = 0; u < U; ++u)
+          if (t0 + u * W < G::NACC) Hg[t0 + u * W] = fma(wt[u], v[u], Hg[t0 + u * W]);
+      }
+    } else {
+      for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
+        double v[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u)
-            if (t0 + u * W < G::NACC) Hg[t0 + u * W] += v[u];
-        }
+        for (int u = 0; u < U; ++u) v[u] = T[ap[t0 + u * W < G::NACC ? t0 + u * W : 0]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (t0 + u * W < G::NACC) Hg[t0 + u * W] += v[u];
       }
     }
     __syncthreads();  // T is rewritten by the next primitive quartet
-  }
-  if constexpr (NG > 1) {
-    // group sums into the first group's accumulators
-    for (int t = lane; t < G::NACC; t += GX_WAVE) {
-      double v = H[t];
-#pragma unroll
-      for (int g = 1; g < NG; ++g) v += H[g * (G::NACC + G::NT) + t];
-      H[t] = v;
-    }
-    __syncthreads();
   }
   const int2* __restrict__ hp = G::hrr();
   const int* __restrict__ hl = G::hlev();
@@ -273,102 +241,6 @@ __global__ __launch_bounds__(GX_WAVE) void k_quartets_coop(ShellView sh, PairVie
   const int t = blockIdx.x;
   coop_quartet_pairs<G>(sh, pr, bra[t], ket[t], tab + G::L * BOYS_NT * BOYS_ROW64, H);
   if (threadIdx.x == 0) out[pos[t]] = H[G::out()[comp[t]]] * sc[t];
-}
-
-// Fused direct J/K for a cooperative class: persistent one-wave workgroups
-// over work items (bra, range of surviving kets); each ket is one quartet.
-// Digestion as in k_jk (same Jt/Kt accumulators and degeneracy factors), via
-// per-quartet LDS blocks flushed with device atomics.  Always FP64.
-template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(GX_WAVE) void k_jk_coop(JKArgs a) {
-  using G = EriProg<LA, LB, LC, LD>;
-  constexpr int NA = G::NA, NB = G::NB, NC = G::NC, ND = G::ND;
-  constexpr int SAB = NA * NB, SCD = NC * ND, SAC = NA * NC, SAD = NA * ND, SBC = NB * NC, SBD = NB * ND;
-  constexpr int NBLK = SAB + SCD + SAC + SAD + SBC + SBD;
-  extern __shared__ double smem[];
-  double* __restrict__ H = smem;
-  constexpr int NG = coop_groups<G>();
-  double* __restrict__ pblk = smem + coop_lds_ng<G, NG>();  // P_ab P_cd P_ac P_ad P_bc P_bd
-  double* __restrict__ jblk = pblk + NBLK;           // J_ab J_cd K_ac K_ad K_bc K_bd partials
-  const int lane = threadIdx.x;
-  const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
-  // static striding over the items (claiming them from a counter, as the
-  // gradient kernels do, measured slower here)
-  for (int w = blockIdx.x; w < a.nitem; w += gridDim.x) {
-    const int4 it = a.items[w];
-    const int ip = a.bra0 + it.x;
-    const PairRec br = a.pr.rec[ip];
-    const int ao = br.fa, bo = br.fb;
-    const int sys = br.sys;
-    const int N = a.sy.n[sys];
-    const long mb = a.sy.mb[sys];
-    const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
-    const PrimPair64* __restrict__ bra = a.pr.pp64 + br.poff + p0;
-    const double ABx = br.ABx, ABy = br.ABy, ABz = br.ABz;
-    const double fA = (br.A == br.B) ? 0.5 : 1.0;
-    for (int v = it.y; v < it.z; ++v) {
-      const int jp = __builtin_amdgcn_readfirstlane(a.kets[v]);  // cooperative classes: FP64 kets only
-      const PairRec kq = a.pr.rec[jp];
-      const int co = kq.fa, dof = kq.fb;
-      coop_quartet<G, false, NG>(bra, nbp, a.pr.pp64 + kq.poff, kq.np, ABx, ABy, ABz, kq.ABx, kq.ABy, kq.ABz,
-                                 tab, H);
-      double f = fA * ((kq.A == kq.B) ? 0.5 : 1.0);
-      if (jp == ip) f *= 0.5;
-      // density blocks of this quartet; partials cleared
-      for (int t = lane; t < NBLK; t += GX_WAVE) {
-        int r0, c0, s;
-        int u = t;
-        if (u < SAB) { r0 = ao; c0 = bo; s = NB; }
-        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; }
-        else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; s = NC; }
-        else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
-        else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
-        else { u -= SBC; r0 = bo; c0 = dof; s = ND; }
-        pblk[t] = a.P[mb + (long)(r0 + u / s) * N + c0 + u % s];
-        jblk[t] = 0.0;
-      }
-      __syncthreads();
-      const double* pab = pblk;
-      const double* pcd = pab + SAB;
-      const double* pac = pcd + SCD;
-      const double* pad = pac + SAC;
-      const double* pbc = pad + SAD;
-      const double* pbd = pbc + SBC;
-      double* jab = jblk;
-      double* jcd = jab + SAB;
-      double* kac = jcd + SCD;
-      double* kad = kac + SAC;
-      double* kbc = kad + SAD;
-      double* kbd = kbc + SBC;
-      const unsigned short* __restrict__ os = G::out();
-      for (int k = lane; k < G::NOUT; k += GX_WAVE) {
-        const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
-        const double x = f * H[os[k]];
-        atomicAdd(&jab[ia * NB + ib], x * pcd[ic * ND + id]);
-        atomicAdd(&jcd[ic * ND + id], x * pab[ia * NB + ib]);
-        atomicAdd(&kac[ia * NC + ic], x * pbd[ib * ND + id]);
-        atomicAdd(&kad[ia * ND + id], x * pbc[ib * NC + ic]);
-        atomicAdd(&kbc[ib * NC + ic], x * pad[ia * ND + id]);
-        atomicAdd(&kbd[ib * ND + id], x * pac[ia * NC + ic]);
-      }
-      __syncthreads();
-      for (int t = lane; t < NBLK; t += GX_WAVE) {
-        const double x = jblk[t];
-        if (x == 0.0) continue;
-        int r0, c0, s;
-        int u = t;
-        double* dst = a.Kt;
-        double m = 1.0;
-        if (u < SAB) { r0 = ao; c0 = bo; s = NB; dst = a.Jt; m = 2.0; }
-        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; s = ND; dst = a.Jt; m = 2.0; }
-        else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; s = NC; }
-        else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; s = ND; }
-        else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; s = NC; }
-        else { u -= SBC; r0 = bo; c0 = dof; s = ND; }
-        atomicAdd(&dst[mb + (long)(r0 + u / s) * N + c0 + u % s], m * x);
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------

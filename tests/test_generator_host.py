@@ -28,13 +28,18 @@ def host_lib():
     gen = os.path.join(BUILD, 'eri_gen.h')
     subprocess.run([sys.executable, os.path.join(ROOT, 'gaussium_amd', 'csrc', 'gen_eri.py'), gen, '2'],
                    check=True)
+    sys.path.insert(0, ROOT)
+    from gaussium_amd.build import generate_rys
+    rys_dir = generate_rys(os.path.join(BUILD, 'rys'))
     so = os.path.join(BUILD, 'liberi_host.so')
-    subprocess.run(['g++', '-O1', '-std=c++17', '-shared', '-fPIC', '-I', BUILD,
+    subprocess.run(['g++', '-O1', '-std=c++17', '-shared', '-fPIC', '-I', BUILD, '-I', os.path.dirname(rys_dir),
+                    '-I', os.path.join(ROOT, 'gaussium_amd', 'csrc'),
                     os.path.join(HARNESS, 'eri_host.cpp'), '-o', so], check=True)
     lib = ctypes.CDLL(so)
     lib.host_quartet.restype = ctypes.c_int
     lib.host_quartet_prog.restype = ctypes.c_int
     lib.host_quartet_grad.restype = ctypes.c_int
+    lib.host_quartet_rys.restype = ctypes.c_int
     return lib
 
 
@@ -83,13 +88,15 @@ CLASSES = [(la, lb, lc, ld) for la in range(3) for lb in range(la + 1)
            if (la * (la + 1) // 2 + lb) >= (lc * (lc + 1) // 2 + ld)]
 
 
-@pytest.mark.parametrize('entry', ['host_quartet', 'host_quartet_prog'])
+@pytest.mark.parametrize('entry', ['host_quartet', 'host_quartet_prog', 'host_quartet_rys'])
 @pytest.mark.parametrize('cls', CLASSES, ids=lambda c: '%d%d%d%d' % c)
 def test_generated_class_matches_oracle(host_lib, cls, entry):
     """``host_quartet``: the code path the device takes for the class (per-lane
     class code, or the cooperative program for the large classes);
     ``host_quartet_prog``: the cooperative program of every class, run level by
-    level with a check that each source slot is written before it is read."""
+    level with a check that each source slot is written before it is read;
+    ``host_quartet_rys``: Rys quadrature (tabulated roots and weights, 2D
+    integrals per root and axis)."""
     rng = np.random.default_rng(sum(v * 10 ** i for i, v in enumerate(cls)) + 7)
     for trial in range(2):
         centers = rng.uniform(-1.5, 1.5, (4, 3))
