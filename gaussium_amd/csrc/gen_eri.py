@@ -248,11 +248,15 @@ def emit_class(la, lb, lc, ld):
 #     slots 0..NACC-1 hold the contracted accumulators.
 #   out[k] = H[outslot[k]] in the reference component order.
 # ---------------------------------------------------------------------------
-COOP_NACC = 90  # classes with more contracted intermediates than this are cooperative ...
+COOP_NACC = 80  # classes with more contracted intermediates than this run one quartet per wave
+               # (J/K: the Rys kernel; (pp|pp), 81 intermediates, 32 -> 18 us there at the
+               # 255-function cluster) ...
 COOP_NACC_KET_S = 200  # ... except those without ket HRR (ld = 0) up to this many (measured on
-                       # gfx950: per-lane code, even spilling, beats the wave-per-quartet kernel
+                       # gfx950: per-lane code, even spilling, beats the wave-per-quartet kernels
                        # there: (dp|ds) 0.38 -> 0.11 ms, (dd|ps) 0.29 -> 0.07, (dd|ds) 0.39 -> 0.24
-                       # at the 255-function cluster; (dp|pp), with a ket HRR, got slower)
+                       # at the 255-function cluster against the Obara-Saika programs, and
+                       # 0.23-0.26 ms for the Rys kernel, whose one quartet per wave does not
+                       # suit their many contracted quartets)
 
 
 def compact_hrr(hentries, nacc, fixed, keep):

@@ -210,9 +210,31 @@ __global__ __launch_bounds__(GX_WAVE) void k_dense_coop(ShellView sh, PairView p
   const long ao = sh.bf[pr.A[ip]], bo = sh.bf[pr.B[ip]];
   const long co = sh.bf[pr.A[jp]], dof = sh.bf[pr.B[jp]];
   const unsigned short* __restrict__ os = G::out();
+  // Components that the quartet's own symmetries map onto each other (a == b
+  // shells, c == d shells, bra pair == ket pair) share their eight images;
+  // the lanes write them concurrently, so only the smallest component index
+  // of each such orbit writes (its images cover the others' addresses), and
+  // every address gets one value.
+  const bool sab = pr.A[ip] == pr.B[ip], scd = pr.A[jp] == pr.B[jp], sbk = same && ip == jp;
   for (int k = threadIdx.x; k < G::NOUT; k += GX_WAVE) {
     const int id = k % G::ND, ic = (k / G::ND) % G::NC, ib = (k / (G::ND * G::NC)) % G::NB;
     const int ia = k / (G::ND * G::NC * G::NB);
+    bool owner = true;
+#pragma unroll
+    for (int g = 1; g < 8; ++g) {
+      const bool swab = g & 1, swcd = g & 2, swbk = g & 4;
+      if ((swab && !sab) || (swcd && !scd) || (swbk && !sbk)) continue;
+      int a2 = swab ? ib : ia, b2 = swab ? ia : ib, c2 = swcd ? id : ic, d2 = swcd ? ic : id;
+      if (swbk) {
+        const int ta = a2, tb = b2;
+        a2 = c2;
+        b2 = d2;
+        c2 = ta;
+        d2 = tb;
+      }
+      if (((a2 * G::NB + b2) * G::NC + c2) * G::ND + d2 < k) owner = false;
+    }
+    if (!owner) continue;
     const long i = ao + ia, j = bo + ib, kk = co + ic, l = dof + id;
     const double x = H[os[k]] * scale[i] * scale[j] * scale[kk] * scale[l];
     out[((i * N + j) * N + kk) * N + l] = x;

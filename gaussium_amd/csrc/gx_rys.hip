@@ -18,10 +18,22 @@
 
 namespace gx {
 
+// Primitive quartets evaluated side by side in one round: as many as the
+// root (2n lanes each) and 2D-integral (3n lanes each) phases fit in a wave,
+// while their 2D tables stay within 2048 doubles of LDS.
 template <int LA, int LB, int LC, int LD>
-constexpr int rys_lds() {  // doubles: rule (2n), 2D tables (3 n NG)
+constexpr int rys_pq() {
   using R = RysClass<LA, LB, LC, LD>;
-  return 2 * R::NROOT + 3 * R::NROOT * R::NG;
+  constexpr int by_lanes = GX_WAVE / (3 * R::NROOT);
+  constexpr int by_lds = 2048 / (3 * R::NROOT * R::NG);
+  constexpr int p = by_lanes < by_lds ? by_lanes : by_lds;
+  return p < 1 ? 1 : p;
+}
+
+template <int LA, int LB, int LC, int LD>
+constexpr int rys_lds() {  // doubles: rules (2n per primitive quartet), 2D tables (3 n NG each)
+  using R = RysClass<LA, LB, LC, LD>;
+  return rys_pq<LA, LB, LC, LD>() * (2 * R::NROOT + 3 * R::NROOT * R::NG);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -32,9 +44,10 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
   constexpr int SAB = NA * NB, SCD = NC * ND, SAC = NA * NC, SAD = NA * ND, SBC = NB * NC, SBD = NB * ND;
   constexpr int NBLK = SAB + SCD + SAC + SAD + SBC + SBD;
   constexpr int NPL = (NOUT + GX_WAVE - 1) / GX_WAVE;  // output components per lane
+  constexpr int PQ = rys_pq<LA, LB, LC, LD>();          // primitive quartets per round
   extern __shared__ double smem[];
-  double* __restrict__ RW = smem;                      // roots u_i, then weights w_i (times the prefactor)
-  double* __restrict__ Gt = smem + 2 * NR;             // [root][axis][NG]
+  double* __restrict__ RW = smem;                      // [pq][roots u_i, then weights w_i times the prefactor]
+  double* __restrict__ Gt = smem + 2 * NR * PQ;        // [pq][root][axis][NG]
   double* __restrict__ pblk = smem + rys_lds<LA, LB, LC, LD>();  // P_ab P_cd P_ac P_ad P_bc P_bd
   double* __restrict__ jblk = pblk + NBLK;             // J_ab J_cd K_ac K_ad K_bc K_bd partials
   const int lane = threadIdx.x;
@@ -97,43 +110,57 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
 #pragma unroll
       for (int j = 0; j < NPL; ++j) acc[j] = 0.0;
       const int nq = kq.np * nbp;
-      for (int q = 0; q < nq; ++q) {
-        const int ik = q / nbp, ib = q - ik * nbp;
-        const PrimPair64 b = bra[ib];
-        const PrimPair64 k = ket[ik];
-        const double rs = gx_rsqrt(b.p + k.p);
-        const double is = rs * rs;
-        const double rho = b.p * k.p * is;
-        const double PQ[3] = {b.Px - k.Px, b.Py - k.Py, b.Pz - k.Pz};
-        const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-        const double pref = b.Kip * k.Kip * rs;
-        const double rp = rho * b.ip, rq = rho * k.ip;
-        __syncthreads();  // the previous primitive quartet's readers of RW and Gt are done
-        if (lane < 2 * NR) RW[lane] = rys_quantity(NR, T, lane) * (lane < NR ? 1.0 : pref);
+      // lane roles: roots phase (pq, quantity) for lane < 2 NR PQ; 2D phase
+      // (pq, root, axis) for lane < 3 NR PQ
+      const int pr = lane / (2 * NR), qr = lane - pr * (2 * NR);
+      const int pg = lane / (3 * NR), rg = (lane - pg * (3 * NR)) / 3, xg = lane - pg * (3 * NR) - 3 * rg;
+      for (int q0 = 0; q0 < nq; q0 += PQ) {
+        const int npq = nq - q0 < PQ ? nq - q0 : PQ;
+        __syncthreads();  // the previous round's readers of RW and Gt are done
+        if (pr < npq) {
+          const int q = q0 + pr, ik = q / nbp, ib = q - ik * nbp;
+          const PrimPair64 b = bra[ib];
+          const PrimPair64 k = ket[ik];
+          const double rs = gx_rsqrt(b.p + k.p);
+          const double rho = b.p * k.p * rs * rs;
+          const double PQx = b.Px - k.Px, PQy = b.Py - k.Py, PQz = b.Pz - k.Pz;
+          const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+          RW[lane] = rys_quantity(NR, T, qr) * (qr < NR ? 1.0 : b.Kip * k.Kip * rs);
+        }
         __syncthreads();
-        if (lane < 3 * NR) {
-          const int r = lane / 3, x = lane - 3 * r;
-          const double u = RW[r];
-          const double PA = x == 0 ? b.PAx : (x == 1 ? b.PAy : b.PAz);
-          const double QC = x == 0 ? k.PAx : (x == 1 ? k.PAy : k.PAz);
-          const double pq = x == 0 ? PQ[0] : (x == 1 ? PQ[1] : PQ[2]);
-          const double ab = x == 0 ? AB[0] : (x == 1 ? AB[1] : AB[2]);
-          const double cd = x == 0 ? CD[0] : (x == 1 ? CD[1] : CD[2]);
+        if (pg < npq) {
+          const int q = q0 + pg, ik = q / nbp, ib = q - ik * nbp;
+          const PrimPair64 b = bra[ib];
+          const PrimPair64 k = ket[ik];
+          const double rs = gx_rsqrt(b.p + k.p);
+          const double is = rs * rs;
+          const double rho = b.p * k.p * is;
+          const double rp = rho * b.ip, rq = rho * k.ip;
+          const double u = RW[pg * 2 * NR + rg];
+          const double PA = xg == 0 ? b.PAx : (xg == 1 ? b.PAy : b.PAz);
+          const double QC = xg == 0 ? k.PAx : (xg == 1 ? k.PAy : k.PAz);
+          const double pq = xg == 0 ? b.Px - k.Px : (xg == 1 ? b.Py - k.Py : b.Pz - k.Pz);
+          const double ab = xg == 0 ? AB[0] : (xg == 1 ? AB[1] : AB[2]);
+          const double cd = xg == 0 ? CD[0] : (xg == 1 ? CD[1] : CD[2]);
           R::g2d(fma(-rp * pq, u, PA), fma(rq * pq, u, QC), 0.5 * b.ip * fma(-rp, u, 1.0),
                  0.5 * k.ip * fma(-rq, u, 1.0), 0.5 * is * u, ab, cd, Gt + lane * NG);
         }
         __syncthreads();
+        for (int pp = 0; pp < npq; ++pp) {
+          const double* __restrict__ W = RW + pp * 2 * NR + NR;
+          const double* __restrict__ Gq = Gt + pp * 3 * NR * NG;
 #pragma unroll
-        for (int j = 0; j < NPL; ++j) {
-          if (lane + j * GX_WAVE < NOUT) {
-            const int ox = off[j] & 1023, oy = (off[j] >> 10) & 1023, oz = off[j] >> 20;
-            double s = 0.0;
+          for (int j = 0; j < NPL; ++j) {
+            if (lane + j * GX_WAVE < NOUT) {
+              const int ox = off[j] & 1023, oy = (off[j] >> 10) & 1023, oz = off[j] >> 20;
+              double s = 0.0;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-              const double* g = Gt + 3 * r * NG;
-              s = fma(RW[NR + r] * g[ox], g[NG + oy] * g[2 * NG + oz], s);
+              for (int r = 0; r < NR; ++r) {
+                const double* g = Gq + 3 * r * NG;
+                s = fma(W[r] * g[ox], g[NG + oy] * g[2 * NG + oz], s);
+              }
+              acc[j] += s;
             }
-            acc[j] += s;
           }
         }
       }
