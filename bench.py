@@ -62,7 +62,7 @@ def kernel_name(combo, gradient=False, mixed=False):
     coop = _lib.load().gx_kernel_kind(la, lb, lc, ld, 1 if gradient else 0) == 1
     if gradient:
         if coop:
-            return 'k_grad_coop<%d, %d, %d, %d>' % combo
+            return 'k_grad_rys<%d, %d, %d, %d>' % combo
         return 'k_grad_lane<%d, %d, %d, %d, %s>' % (la, lb, lc, ld, 'true' if mixed else 'false')
     if coop:
         return 'k_jk_rys<%d, %d, %d, %d>' % combo

@@ -176,6 +176,74 @@ def emit_2d(la, lb, lc, ld):
     return L + out
 
 
+def emit_2d_grad(la, lb, lc, ld):
+    """2D integrals of one root and axis and their derivatives with respect to
+    the centres A, B, C of this axis: G, GA, GB, GC[((a (lb+1) + b)(lc+1) + c)(ld+1) + d]
+    with  GA = 2 alpha E(a+1) - a E(a-1),  GB = 2 beta E(b+1) - b E(b-1),
+          GC = 2 gamma E(c+1) - c E(c-1)
+    (E the 2D integrals of the raised index; the exponents alpha, beta, gamma
+    of the primitives on A, B, C)."""
+    nb, nk = la + lb + 1, lc + ld + 1
+    L = []
+    n_tmp = [0]
+
+    def tmp(expr):
+        name = 'i%d' % n_tmp[0]
+        n_tmp[0] += 1
+        L.append('    const double %s = %s;' % (name, expr))
+        return name
+    I = {(0, 0): '1.0'}
+    for n in range(1, nb + 1):
+        e = 'C00 * %s' % I[(n - 1, 0)]
+        if n >= 2:
+            e += ' + %d.0 * B10 * %s' % (n - 1, I[(n - 2, 0)])
+        I[(n, 0)] = tmp(e)
+    for m in range(1, nk + 1):
+        for n in range(0, nb + 1):
+            if n + m > la + lb + lc + ld + 1:
+                continue
+            e = 'C0p * %s' % I[(n, m - 1)]
+            if m >= 2:
+                e += ' + %d.0 * B01 * %s' % (m - 1, I[(n, m - 2)])
+            if n >= 1:
+                e += ' + %d.0 * B00 * %s' % (n, I[(n - 1, m - 1)])
+            I[(n, m)] = tmp(e)
+    H = {}
+
+    def hb(a, b, m):  # bra HRR, memoised
+        key = (a, b, m)
+        if key not in H:
+            H[key] = I[(a, m)] if b == 0 else tmp('%s + AB * %s' % (hb(a + 1, b - 1, m), hb(a, b - 1, m)))
+        return H[key]
+    K = {}
+
+    def E(a, b, c, d):  # ket HRR, memoised
+        key = (a, b, c, d)
+        if key not in K:
+            K[key] = hb(a, b, c) if d == 0 else tmp('%s + CD * %s' % (E(a, b, c + 1, d - 1), E(a, b, c, d - 1)))
+        return K[key]
+    out = []
+    for a in range(la + 1):
+        for b in range(lb + 1):
+            for c in range(lc + 1):
+                for d in range(ld + 1):
+                    idx = ((a * (lb + 1) + b) * (lc + 1) + c) * (ld + 1) + d
+                    out.append('    G[%d] = %s;' % (idx, E(a, b, c, d)))
+                    ga = '2.0 * alpha * %s' % E(a + 1, b, c, d)
+                    if a:
+                        ga += ' - %d.0 * %s' % (a, E(a - 1, b, c, d))
+                    gb = '2.0 * beta * %s' % E(a, b + 1, c, d)
+                    if b:
+                        gb += ' - %d.0 * %s' % (b, E(a, b - 1, c, d))
+                    gc = '2.0 * gamma * %s' % E(a, b, c + 1, d)
+                    if c:
+                        gc += ' - %d.0 * %s' % (c, E(a, b, c - 1, d))
+                    out.append('    GA[%d] = %s;' % (idx, ga))
+                    out.append('    GB[%d] = %s;' % (idx, gb))
+                    out.append('    GC[%d] = %s;' % (idx, gc))
+    return L + out
+
+
 def emit_class(la, lb, lc, ld):
     L = la + lb + lc + ld
     n = L // 2 + 1
@@ -202,6 +270,15 @@ def emit_class(la, lb, lc, ld):
     s.append('    (void)C00; (void)C0p; (void)B10; (void)B01; (void)B00; (void)AB; (void)CD;')
     s += emit_2d(la, lb, lc, ld)
     s.append('  }')
+    if (L + 1) // 2 + 1 <= NMAX_DEFAULT:
+        s.append('  // the gradient: one more root at L + 1; G, GA, GB, GC[NG] (see emit_2d_grad)')
+        s.append('  static constexpr int NROOT_GRAD = %d;' % ((L + 1) // 2 + 1))
+        s.append('  static __device__ __forceinline__ void g2d_grad(double C00, double C0p, double B10, double B01,')
+        s.append('      double B00, double AB, double CD, double alpha, double beta, double gamma, double* G,')
+        s.append('      double* GA, double* GB, double* GC) {')
+        s.append('    (void)C00; (void)C0p; (void)B10; (void)B01; (void)B00; (void)AB; (void)CD;')
+        s += emit_2d_grad(la, lb, lc, ld)
+        s.append('  }')
     s.append('};')
     return '\n'.join(s)
 

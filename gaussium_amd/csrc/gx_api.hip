@@ -17,8 +17,8 @@
 // Single translation unit: the kernels are templates instantiated below.
 #include "gx_eri.hip"
 #include "gx_coop.hip"
-#include "gx_rys.hip"
 #include "gx_grad.hip"
+#include "gx_rys.hip"
 #include "gx_one.hip"
 
 // Device scratch of gx_jk_noalloc: the scaled density, Kt, Jt and the
@@ -471,14 +471,15 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
   return hipErrorInvalidValue;
 }
 
-// Two-electron gradient of one class combination (gx_coop.hip, k_grad_coop).
+// Two-electron gradient of one class combination (gx_grad.hip, k_grad_lane;
+// gx_rys.hip, k_grad_rys).
 size_t grad_lds_bytes(int la, int lb, int lc, int ld, int N) {
   if (gx_grad_is_lane(la, lb, lc, ld)) return (size_t)(ncart(la) + ncart(lb)) * N * sizeof(double);
 #define GX_CASE(A, B, C, D)                                                                     \
   if (la == A && lb == B && lc == C && ld == D) {                                               \
     using G = EriGradProg<A, B, C, D>;                                                          \
     const int nblk = (G::NA + G::NB) * (G::NC + G::ND) + G::NA * G::NB + G::NC * G::ND;          \
-    return ((size_t)gx::grad_lds<G>() + nblk) * sizeof(double);                                 \
+    return ((size_t)gx::rys_grad_lds<A, B, C, D>() + nblk) * sizeof(double);                    \
   }
   GX_FOR_EACH_CLASS(GX_CASE)
 #undef GX_CASE
@@ -513,7 +514,7 @@ hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s
 #undef GX_LANE
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
-    const void* fn = (const void*)gx::k_grad_coop<A, B, C, D>;                               \
+    const void* fn = (const void*)gx::k_grad_rys<A, B, C, D>;                                \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
     if (*per_cu <= 0) {                                                                       \
@@ -522,7 +523,7 @@ hipError_t launch_grad(int la, int lb, int lc, int ld, int num_cu, hipStream_t s
       if (*per_cu < 1) *per_cu = 1;                                                           \
     }                                                                                         \
     const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
-    hipLaunchKernelGGL((gx::k_grad_coop<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a); \
+    hipLaunchKernelGGL((gx::k_grad_rys<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a);  \
     return hipGetLastError();                                                                 \
   }
   GX_FOR_EACH_GRAD_COOP_CLASS(GX_CASE)

@@ -29,10 +29,8 @@ constexpr int coop_lds() {
 }
 
 // Contracted shell quartet into H: on return H[G::out()[k]] is component k
-// (reference order), unscaled.  Every lane of the wave calls it.  With
-// GRAD, G is a gradient program (EriGradProg): accumulator entries carry a
-// weight set (1, 2 alpha_a, 2 alpha_b, 2 alpha_c) and slot NACC stays zero.
-template <class G, bool GRAD = false>
+// (reference order), unscaled.  Every lane of the wave calls it.
+template <class G>
 __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra, int nb,
                                              const PrimPair64* __restrict__ ket, int nk, double ABx,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
@@ -106,32 +104,13 @@ __device__ __forceinline__ void coop_quartet(const PrimPair64* __restrict__ bra,
       }
       __syncthreads();
     }
-    if constexpr (GRAD) {
-      const double w1 = 2.0 * b.ea, w2 = 2.0 * (b.p - b.ea), w3 = 2.0 * k.ea;
-      for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
-        unsigned e[U];
+    for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
+      double v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) e[u] = ap[t0 + u * W < G::NACC ? t0 + u * W : 0];
-        double v[U], wt[U];
+      for (int u = 0; u < U; ++u) v[u] = T[ap[t0 + u * W < G::NACC ? t0 + u * W : 0]];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const unsigned st = e[u] >> 16;
-          wt[u] = st == 0 ? 1.0 : (st == 1 ? w1 : (st == 2 ? w2 : w3));
-          v[u] = T[e[u] & 0xffffu];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (t0 + u * W < G::NACC) Hg[t0 + u * W] = fma(wt[u], v[u], Hg[t0 + u * W]);
-      }
-    } else {
-      for (int t0 = sub; t0 < G::NACC; t0 += U * W) {
-        double v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = T[ap[t0 + u * W < G::NACC ? t0 + u * W : 0]];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (t0 + u * W < G::NACC) Hg[t0 + u * W] += v[u];
-      }
+      for (int u = 0; u < U; ++u)
+        if (t0 + u * W < G::NACC) Hg[t0 + u * W] += v[u];
     }
     __syncthreads();  // T is rewritten by the next primitive quartet
   }
@@ -263,160 +242,6 @@ __global__ __launch_bounds__(GX_WAVE) void k_quartets_coop(ShellView sh, PairVie
   const int t = blockIdx.x;
   coop_quartet_pairs<G>(sh, pr, bra[t], ket[t], tab + G::L * BOYS_NT * BOYS_ROW64, H);
   if (threadIdx.x == 0) out[pos[t]] = H[G::out()[comp[t]]] * sc[t];
-}
-
-// ---------------------------------------------------------------------------
-// Two-electron part of the RHF nuclear gradient over the canonical shell
-// quartets (the same work items and screening as the J/K build):
-//   g_X += sum_k D_k d(ab|cd)_k / dX,  D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc)
-// with f the 8-fold degeneracy weight (the energy of one canonical quartet is
-// f v D_k / f = f v (4 P_ab P_cd - P_ac P_bd - P_ad P_bc), from
-// E_2e = 1/2 tr(P J) - 1/4 tr(P K)).  Derivatives on A, B, C come from the
-// gradient program; D from translational invariance.
-struct GradArgs {
-  ShellView sh;
-  PairView pr;
-  SysView sy;
-  int bra0;
-  const int4* items;   // as JKArgs::items
-  int nitem;
-  const int* kets;     // as JKArgs::kets
-  const double* tab64;
-  const float* tab32;  // FP32 Taylor tables (mixed-precision per-lane kernels)
-  const double* P;     // packed densities, pre-scaled by s_k s_l
-  const int* atom;     // shell -> atom (over all systems)
-  const int* sys_atom; // per system: first nucleus (nsys + 1)
-  double* grad;        // 3 natom
-  int* item_ctr;       // [next item, workgroups done]: zero before a launch, reset by its last workgroup
-};
-
-// Gradient contributions collect in LDS per nucleus of the item's system (at
-// most GX_GRAD_MAXATOM) over all the items a workgroup takes from that
-// system, and go to HBM when the system changes and at the end: the few
-// nuclei of a system would otherwise take every quartet's (or every item's)
-// atomics, all on the same addresses.
-#define GX_GRAD_MAXATOM 256
-
-template <class G>
-constexpr int grad_lds() {
-  return G::NACC + G::NT > G::NH ? G::NACC + G::NT : G::NH;
-}
-
-template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(GX_WAVE) void k_grad_coop(GradArgs a) {
-  using G = EriGradProg<LA, LB, LC, LD>;
-  constexpr int NA = G::NA, NB = G::NB, NC = G::NC, ND = G::ND;
-  constexpr int SAB = NA * NB, SCD = NC * ND, SAC = NA * NC, SAD = NA * ND, SBC = NB * NC, SBD = NB * ND;
-  constexpr int NBLK = SAB + SCD + SAC + SAD + SBC + SBD;
-  extern __shared__ double smem[];
-  double* __restrict__ H = smem;
-  double* __restrict__ pblk = smem + grad_lds<G>();
-  __shared__ double gsh[3 * GX_GRAD_MAXATOM];
-  const int lane = threadIdx.x;
-  const double* __restrict__ tab = a.tab64 + G::L * BOYS_NT * BOYS_ROW64;
-  const int2* __restrict__ gt = G::gterm();
-  // nuclear sums of the current system, kept in LDS across the items of
-  // the workgroup and flushed when the system changes and at the end
-  int cur = -1, cur3 = 0;
-  // items in decreasing cost: the first is blockIdx.x, the rest are claimed
-  // from a counter as workgroups finish
-  for (int w = blockIdx.x; w < a.nitem;) {
-    const int4 it = a.items[w];
-    const int ip = a.bra0 + it.x;
-    const PairRec br = a.pr.rec[ip];
-    const int A = br.A, B = br.B;
-    const int ao = br.fa, bo = br.fb;
-    const int sys = br.sys;
-    const int N = a.sy.n[sys];
-    const long mb = a.sy.mb[sys];
-    const int at0 = a.sys_atom[sys], nat3 = 3 * (a.sys_atom[sys + 1] - at0);
-    const int p0 = it.w & 0xffff, nbp = (it.w >> 16) - p0;
-    const PrimPair64* __restrict__ bra = a.pr.pp64 + br.poff + p0;
-    const double ABx = br.ABx, ABy = br.ABy, ABz = br.ABz;
-    const double fA = (A == B) ? 0.5 : 1.0;
-    if (at0 != cur) {  // another system: flush the sums held for the previous one
-      __syncthreads();
-      for (int t = lane; t < cur3; t += GX_WAVE)
-        if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
-      for (int t = lane; t < nat3; t += GX_WAVE) gsh[t] = 0.0;
-      cur = at0;
-      cur3 = nat3;
-      __syncthreads();
-    }
-    for (int v = it.y; v < it.z; ++v) {
-      const int jp = __builtin_amdgcn_readfirstlane(a.kets[v]);  // gradients of these classes: FP64 only
-      const PairRec kq = a.pr.rec[jp];
-      const int Cs = kq.A, Ds = kq.B;
-      const int co = kq.fa, dof = kq.fb;
-      coop_quartet<G, true>(bra, nbp, a.pr.pp64 + kq.poff, kq.np, ABx, ABy, ABz, kq.ABx, kq.ABy, kq.ABz, tab, H);
-      double f = fA * ((Cs == Ds) ? 0.5 : 1.0);
-      if (jp == ip) f *= 0.5;
-      for (int t = lane; t < NBLK; t += GX_WAVE) {
-        int r0, c0, st;
-        int u = t;
-        if (u < SAB) { r0 = ao; c0 = bo; st = NB; }
-        else if ((u -= SAB) < SCD) { r0 = co; c0 = dof; st = ND; }
-        else if ((u -= SCD) < SAC) { r0 = ao; c0 = co; st = NC; }
-        else if ((u -= SAC) < SAD) { r0 = ao; c0 = dof; st = ND; }
-        else if ((u -= SAD) < SBC) { r0 = bo; c0 = co; st = NC; }
-        else { u -= SBC; r0 = bo; c0 = dof; st = ND; }
-        pblk[t] = a.P[mb + (long)(r0 + u / st) * N + c0 + u % st];
-      }
-      __syncthreads();
-      const double* pab = pblk;
-      const double* pcd = pab + SAB;
-      const double* pac = pcd + SCD;
-      const double* pad = pac + SAC;
-      const double* pbc = pad + SAD;
-      const double* pbd = pbc + SBC;
-      double g[9];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) g[j] = 0.0;
-      for (int k = lane; k < G::NOUT; k += GX_WAVE) {
-        const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
-        const double D = f * (4.0 * pab[ia * NB + ib] * pcd[ic * ND + id] - pac[ia * NC + ic] * pbd[ib * ND + id] -
-                              pad[ia * ND + id] * pbc[ib * NC + ic]);
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-          const int2 e = gt[9 * k + j];
-          const unsigned y = (unsigned)e.y;
-          g[j] = fma(D, H[e.x] - (double)(y >> 16) * H[y & 0xffffu], g[j]);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        double x = g[j];
-#pragma unroll
-        for (int off = GX_WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, GX_WAVE);
-        g[j] = x;
-      }
-      if (lane == 0) {
-        const int at[3] = {a.atom[A] - at0, a.atom[B] - at0, a.atom[Cs] - at0};
-        const int atD = a.atom[Ds] - at0;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          gsh[3 * at[0] + x] += g[x];
-          gsh[3 * at[1] + x] += g[3 + x];
-          gsh[3 * at[2] + x] += g[6 + x];
-          gsh[3 * atD + x] -= g[x] + g[3 + x] + g[6 + x];
-        }
-      }
-    }
-    int nx = 0;
-    if (lane == 0) nx = atomicAdd(&a.item_ctr[0], 1) + (int)gridDim.x;
-    w = __shfl(nx, 0, GX_WAVE);
-  }
-  __syncthreads();
-  for (int t = lane; t < cur3; t += GX_WAVE)
-    if (gsh[t] != 0.0) atomicAdd(&a.grad[3 * cur + t], gsh[t]);
-  // the last workgroup to finish zeroes the counters for the next launch
-  if (lane == 0) {
-    __threadfence();
-    if (atomicAdd(&a.item_ctr[1], 1) == (int)gridDim.x - 1) {
-      atomicExch(&a.item_ctr[0], 0);
-      atomicExch(&a.item_ctr[1], 0);
-    }
-  }
 }
 
 }  // namespace gx

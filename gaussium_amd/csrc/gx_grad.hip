@@ -6,7 +6,7 @@
 // give the nine derivatives on A, B, C (D by translational invariance).
 // Classes whose sets fit the registers at once run one pass; those whose
 // per-centre sets fit run three passes over the primitives (EriGradLane::NPASS).
-// The larger classes use k_grad_coop (gx_coop.hip).
+// The larger classes use k_grad_rys (gx_rys.hip).
 //
 // MIXED = true: the kets whose Schwarz product is below the FP32 bound (the
 // second range of every ket segment, as in k_jk) run their primitive loop in
@@ -14,6 +14,38 @@
 // converted and the HRR and contraction run in FP64.
 
 namespace gx {
+
+// Two-electron part of the RHF nuclear gradient over the canonical shell
+// quartets (the same work items and screening as the J/K build):
+//   g_X += sum_k D_k d(ab|cd)_k / dX,  D_k = f (4 P_ab P_cd - P_ac P_bd - P_ad P_bc)
+// with f the 8-fold degeneracy weight (the energy of one canonical quartet is
+// f v D_k / f = f v (4 P_ab P_cd - P_ac P_bd - P_ad P_bc), from
+// E_2e = 1/2 tr(P J) - 1/4 tr(P K)).  Derivatives on A, B, C come from the
+// recursions (per-lane class code, or Rys quadrature in gx_rys.hip); D from
+// translational invariance.
+struct GradArgs {
+  ShellView sh;
+  PairView pr;
+  SysView sy;
+  int bra0;
+  const int4* items;   // as JKArgs::items
+  int nitem;
+  const int* kets;     // as JKArgs::kets
+  const double* tab64;
+  const float* tab32;  // FP32 Taylor tables (mixed-precision per-lane kernels)
+  const double* P;     // packed densities, pre-scaled by s_k s_l
+  const int* atom;     // shell -> atom (over all systems)
+  const int* sys_atom; // per system: first nucleus (nsys + 1)
+  double* grad;        // 3 natom
+  int* item_ctr;       // [next item, workgroups done]: zero before a launch, reset by its last workgroup
+};
+
+// Gradient contributions collect in LDS per nucleus of the item's system (at
+// most GX_GRAD_MAXATOM) over all the items a workgroup takes from that
+// system, and go to HBM when the system changes and at the end: the few
+// nuclei of a system would otherwise take every quartet's (or every item's)
+// atomics, all on the same addresses.
+#define GX_GRAD_MAXATOM 256
 
 template <int L>
 __device__ __forceinline__ void boys_nb(double T, double scale, const double* __restrict__ tab, double (&F)[L + 1]) {

@@ -253,7 +253,7 @@ int gx_jk_launch_times(gx_ctx* ctx, double* out, int cap);
 
 /* Kernel family of class combination (la lb|lc ld) (canonical: la >= lb,
  * lc >= ld, (lc, ld) not after (la, lb)): 0 = one shell quartet per lane
- * (k_jk, k_grad_lane), 1 = one quartet per wave (k_jk_rys, k_grad_coop),
+ * (k_jk, k_grad_lane), 1 = one quartet per wave (k_jk_rys, k_grad_rys),
  * for the J/K (gradient = 0) or gradient (1) kernels; negative on bad
  * arguments.  No counterpart in the reference (measurement). */
 int gx_kernel_kind(int la, int lb, int lc, int ld, int gradient);

@@ -527,3 +527,78 @@ extern "C" int host_quartet_rys(const int* l, const double* ctr, const int* n, c
 #undef GX_CASE
   return -1;
 }
+
+// The Rys gradient path: per primitive quartet and root (n at L + 1), the
+// 2D integrals and their A, B, C derivatives (RysClass::g2d_grad), in the
+// layout of quartet_grad (component-major, then centre A, B, C, then axis).
+template <int LA, int LB, int LC, int LD>
+static int quartet_grad_rys(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
+  using R = RysClass<LA, LB, LC, LD>;
+  constexpr int N = R::NROOT_GRAD, NG = R::NG;
+  static double acc[9 * R::NOUT];
+  for (int t = 0; t < 9 * R::NOUT; ++t) acc[t] = 0.0;
+  const double two_pi_52 = 34.98683665524972;
+  double AB[3], CD[3];
+  for (int x = 0; x < 3; ++x) {
+    AB[x] = A.c[x] - B.c[x];
+    CD[x] = C.c[x] - D.c[x];
+  }
+  for (int i = 0; i < A.n; ++i)
+    for (int j = 0; j < B.n; ++j) {
+      const double a = A.a[i], b = B.a[j], p = a + b;
+      double P[3], ab2 = 0;
+      for (int x = 0; x < 3; ++x) {
+        P[x] = (a * A.c[x] + b * B.c[x]) / p;
+        ab2 += AB[x] * AB[x];
+      }
+      const double Kab = A.w[i] * B.w[j] * std::exp(-a * b / p * ab2);
+      for (int k = 0; k < C.n; ++k)
+        for (int l = 0; l < D.n; ++l) {
+          const double c = C.a[k], d = D.a[l], q = c + d;
+          double Q[3], cd2 = 0, pq2 = 0;
+          for (int x = 0; x < 3; ++x) {
+            Q[x] = (c * C.c[x] + d * D.c[x]) / q;
+            cd2 += CD[x] * CD[x];
+            pq2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+          }
+          const double Kcd = C.w[k] * D.w[l] * std::exp(-c * d / q * cd2);
+          const double rho = p * q / (p + q), rp = rho / p, rq = rho / q;
+          const double pref = two_pi_52 / (p * q * std::sqrt(p + q)) * Kab * Kcd;
+          const double T = rho * pq2;
+          double G[3][NG], GA[3][NG], GB[3][NG], GC[3][NG];
+          for (int r = 0; r < N; ++r) {
+            const double u = rys_quantity(N, T, r), wr = rys_quantity(N, T, N + r) * pref;
+            for (int x = 0; x < 3; ++x) {
+              const double PQ = P[x] - Q[x];
+              R::g2d_grad(P[x] - A.c[x] - rp * PQ * u, Q[x] - C.c[x] + rq * PQ * u, 0.5 / p * (1.0 - rp * u),
+                          0.5 / q * (1.0 - rq * u), 0.5 * u / (p + q), AB[x], CD[x], a, b, c, G[x], GA[x], GB[x],
+                          GC[x]);
+            }
+            for (int t = 0; t < R::NOUT; ++t) {
+              const int o = R::off()[t];
+              const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
+              const double* Dv[3][3] = {{GA[0], GA[1], GA[2]}, {GB[0], GB[1], GB[2]}, {GC[0], GC[1], GC[2]}};
+              for (int X = 0; X < 3; ++X) {
+                acc[9 * t + 3 * X + 0] += wr * Dv[X][0][ox] * G[1][oy] * G[2][oz];
+                acc[9 * t + 3 * X + 1] += wr * G[0][ox] * Dv[X][1][oy] * G[2][oz];
+                acc[9 * t + 3 * X + 2] += wr * G[0][ox] * G[1][oy] * Dv[X][2][oz];
+              }
+            }
+          }
+        }
+    }
+  for (int t = 0; t < 9 * R::NOUT; ++t) out[t] = acc[t];
+  return 0;
+}
+
+extern "C" int host_quartet_grad_rys(const int* l, const double* ctr, const int* n, const double* a,
+                                     const double* w, double* out) {
+  Shell s[4];
+  unpack(n, ctr, a, w, s);
+#define GX_CASE(A, B, C, D)                                                                              \
+  if constexpr ((A + B + C + D + 1) / 2 + 1 <= RYS_NMAX)                                                 \
+    if (l[0] == A && l[1] == B && l[2] == C && l[3] == D) return quartet_grad_rys<A, B, C, D>(s[0], s[1], s[2], s[3], out);
+  GX_FOR_EACH_RYS_CLASS(GX_CASE)
+#undef GX_CASE
+  return -1;
+}

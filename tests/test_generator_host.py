@@ -123,12 +123,14 @@ GRAD_CLASSES = [c for c in CLASSES if sum(c) <= 5] + [(2, 1, 2, 1), (2, 2, 2, 2)
 
 
 @pytest.mark.parametrize('cls', GRAD_CLASSES, ids=lambda c: '%d%d%d%d' % c)
-def test_gradient_program_matches_finite_differences(host_lib, cls):
+@pytest.mark.parametrize('entry', ['host_quartet_grad', 'host_quartet_grad_rys'])
+def test_gradient_program_matches_finite_differences(host_lib, cls, entry):
     """d/dX_i (ab|cd) for X = A, B, C from the gradient program (raised and
-    lowered classes with 2 alpha weights) against a 4-point central difference
-    of the oracle's integrals with the centre displaced (h = 1e-3, error
-    O(h^4))."""
-    host_lib.host_quartet_grad.restype = ctypes.c_int
+    lowered classes with 2 alpha weights; ``host_quartet_grad_rys``: the Rys
+    2D integrals and their centre derivatives) against a 4-point central
+    difference of the oracle's integrals with the centre displaced (h = 1e-3,
+    error O(h^4))."""
+    getattr(host_lib, entry).restype = ctypes.c_int
     rng = np.random.default_rng(sum(v * 10 ** i for i, v in enumerate(cls)) + 31)
     centers = rng.uniform(-1.2, 1.2, (4, 3))
     shells = [_shell(rng, l, c) for l, c in zip(cls, centers)]
@@ -139,9 +141,9 @@ def test_gradient_program_matches_finite_differences(host_lib, cls):
     w = np.ascontiguousarray(np.concatenate([s[3] for s in shells]))
     nout = int(np.prod([len(COMPS[x]) for x in cls]))
     out = np.zeros(9 * nout)
-    rc = host_lib.host_quartet_grad(l.ctypes.data_as(ctypes.c_void_p), ctr.ctypes.data_as(ctypes.c_void_p),
-                                    n.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
-                                    w.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+    rc = getattr(host_lib, entry)(l.ctypes.data_as(ctypes.c_void_p), ctr.ctypes.data_as(ctypes.c_void_p),
+                                  n.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
+                                  w.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
     assert rc == 0
     got = out.reshape(nout, 3, 3)
     h = 1e-3
