@@ -123,14 +123,15 @@ int upload(T** p, const std::vector<T>& v, hipStream_t s) {
   return GX_OK;
 }
 
-// Boys master table: F_m(T_n) by the everywhere-convergent series
+// Boys master table on the grid T_n = n * dt, n < nt: F_m(T_n) by the
+// everywhere-convergent series
 // F_M(T) = e^-T sum_k (2T)^k / ((2M+1)(2M+3)...(2M+2k+1)) in long double,
 // then downward recursion; last column exp(-T_n).
-void boys_master_table(std::vector<double>& tab) {
+void boys_master_table(std::vector<double>& tab, int nt, long double dt) {
   const int M = BOYS_MCOL - 1;
-  tab.assign((size_t)BOYS_NT * (BOYS_MCOL + 1), 0.0);
-  for (int n = 0; n < BOYS_NT; ++n) {
-    const long double T = (long double)n * (long double)BOYS_DT;
+  tab.assign((size_t)nt * (BOYS_MCOL + 1), 0.0);
+  for (int n = 0; n < nt; ++n) {
+    const long double T = (long double)n * dt;
     long double term = 1.0L / (2 * M + 1), sum = term;
     for (int k = 1; k < 4000; ++k) {
       term *= 2.0L * T / (2 * M + 2 * k + 1);
@@ -156,9 +157,12 @@ struct gx_ctx {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   int max_lds = 65536;
-  double* d_boys = nullptr;
-  double* d_tab64 = nullptr;  // per-L Taylor tables (L <= BOYS_MCOL - 8)
+  double* d_boys = nullptr;   // master table, fine grid (also the one-electron kernels')
+  double* d_boyst = nullptr;  // master table, coarse grid
+  double* d_tab64 = nullptr;  // per-L Taylor tables, fine grid (L <= BOYS_MCOL - BOYS_TAYLOR64)
   float* d_tab32 = nullptr;
+  double* d_tab64c = nullptr;  // per-L Taylor tables, coarse grid (L <= BOYS_MCOL - BoysCoarse::K64)
+  float* d_tab32c = nullptr;
   // basis
   int nshell = 0, nbf = 0, nprim = 0;
   std::vector<int> l, np, poff, bf, atom;
@@ -307,8 +311,11 @@ struct gx_ctx {
     dfree(&d_nxyz);
     dfree(&d_nZ);
     dfree(&d_boys);
+    dfree(&d_boyst);
     dfree(&d_tab64);
     dfree(&d_tab32);
+    dfree(&d_tab64c);
+    dfree(&d_tab32c);
     for (auto& e : ev) (void)hipEventDestroy(e);
     ev.clear();
     ev_combo.clear();
@@ -548,8 +555,11 @@ size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed, int nden =
 #undef GX_CASE
   const int na = ncart(la), nb = ncart(lb);
   const bool t64 = !mixed, t32 = mixed && GX_MIXED_LDS32;
-  const size_t b = (t64 ? (size_t)BOYS_NT * BOYS_ROW64 * sizeof(double) : 0) +
-                   (t32 ? (size_t)BOYS_NT * BOYS_ROW32 * sizeof(float) : 0);
+  const bool coarse = gx::jk_coarse_grid(la, lb, lc, ld);
+  const size_t nt = coarse ? gx::BoysCoarse::NT : gx::BoysFine::NT;
+  const size_t r64 = coarse ? gx::BoysCoarse::ROW64 : gx::BoysFine::ROW64;
+  const size_t r32 = coarse ? gx::BoysCoarse::ROW32 : gx::BoysFine::ROW32;
+  const size_t b = (t64 ? nt * r64 * sizeof(double) : 0) + (t32 ? nt * r32 * sizeof(float) : 0);
   return b + 2 * (size_t)nden * (na + nb) * N * sizeof(double);
 }
 
@@ -999,9 +1009,10 @@ hipError_t launch_combo(gx_ctx* c, int X, int Y, int combo, int mixed, int nden 
   a.items = reinterpret_cast<const int4*>(c->jk.d_items[combo]);
   a.nitem = c->jk.n_items[combo];
   a.kets = c->jk.d_kets[combo];
-  a.master = c->d_boys;
   a.tab64 = c->d_tab64;
   a.tab32 = c->d_tab32;
+  a.tab64c = c->d_tab64c;
+  a.tab32c = c->d_tab32c;
   a.P = Ps;
   a.P2 = nden == 2 ? c->d_Ps2 : Ps;
   a.Pj = nden == 2 ? c->d_Psj : Ps;
@@ -1144,15 +1155,22 @@ int gx_create(gx_ctx** out, int device) {
     c->max_lds = lds;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
     c->num_cu = ncu;
-  std::vector<double> tab;
-  boys_master_table(tab);
+  std::vector<double> tab, tabc;
+  boys_master_table(tab, BOYS_NT, (long double)BOYS_DT);
+  boys_master_table(tabc, gx::BoysCoarse::NT, 1.0L / gx::BoysCoarse::TPU);
   int rc = upload(&c->d_boys, tab, c->stream);
-  const int ltab = BOYS_MCOL - BOYS_TAYLOR64;  // highest L whose Taylor columns exist
+  if (rc == GX_OK) rc = upload(&c->d_boyst, tabc, c->stream);
+  // highest L whose Taylor columns exist
+  const int ltab = BOYS_MCOL - gx::BoysFine::K64, ltabc = BOYS_MCOL - gx::BoysCoarse::K64;
   if (rc == GX_OK) rc = dalloc(&c->d_tab64, (size_t)(ltab + 1) * BOYS_NT * BOYS_ROW64);
   if (rc == GX_OK) rc = dalloc(&c->d_tab32, (size_t)(ltab + 1) * BOYS_NT * BOYS_ROW32);
+  if (rc == GX_OK) rc = dalloc(&c->d_tab64c, (size_t)(ltabc + 1) * gx::BoysCoarse::NT * gx::BoysCoarse::ROW64);
+  if (rc == GX_OK) rc = dalloc(&c->d_tab32c, (size_t)(ltabc + 1) * gx::BoysCoarse::NT * gx::BoysCoarse::ROW32);
   if (rc == GX_OK) {
-    hipLaunchKernelGGL(gx::k_boys_tables, dim3(ltab + 1), dim3(256), 0, c->stream, ltab, c->d_boys,
+    hipLaunchKernelGGL(gx::k_boys_tables<gx::BoysFine>, dim3(ltab + 1), dim3(256), 0, c->stream, ltab, c->d_boys,
                        c->d_tab64, c->d_tab32);
+    hipLaunchKernelGGL(gx::k_boys_tables<gx::BoysCoarse>, dim3(ltabc + 1), dim3(256), 0, c->stream, ltabc,
+                       c->d_boyst, c->d_tab64c, c->d_tab32c);
     if (hipGetLastError() != hipSuccess) rc = fail(GX_ERR_HIP, "k_boys_tables launch");
   }
   // item counters start at zero; each J/K launch leaves them at zero

@@ -164,7 +164,7 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 #define GX_JKGM 1  // FP64 group inside mixed builds (few FP64 quartets there)
 #endif
 
-template <class C, typename R, typename PP, int G>
+template <class C, typename R, typename PP, int G, class BG>
 __device__ __forceinline__ void prim_group(R (&acc)[C::NACC], const PP* __restrict__ b, const PP& k,
                                            const R* __restrict__ tab) {
   R rs[G], rho[G], PQx[G], PQy[G], PQz[G], T[G], pref[G];
@@ -183,9 +183,9 @@ __device__ __forceinline__ void prim_group(R (&acc)[C::NACC], const PP* __restri
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     if constexpr (sizeof(R) == 8)
-      boys64_nb<C::L>(T[g], pref[g], tab, F[g]);
+      boys64_nb<C::L, BG>(T[g], pref[g], tab, F[g]);
     else
-      boys32_nb<C::L>(T[g], pref[g], tab, F[g]);
+      boys32_nb<C::L, BG>(T[g], pref[g], tab, F[g]);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -197,7 +197,7 @@ __device__ __forceinline__ void prim_group(R (&acc)[C::NACC], const PP* __restri
   }
 }
 
-template <class C, typename R, typename PP, int G>
+template <class C, typename R, typename PP, int G, class BG>
 __device__ __forceinline__ void quartet_jk(const PP* __restrict__ bra, int nb, const PP* __restrict__ kp,
                                            int nk, R ABx, R ABy, R ABz, R CDx, R CDy, R CDz,
                                            const R* __restrict__ tab, R (&out)[C::NOUT]) {
@@ -211,33 +211,33 @@ __device__ __forceinline__ void quartet_jk(const PP* __restrict__ bra, int nb, c
       PP b[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) b[g] = bra[ib + g];
-      prim_group<C, R, PP, G>(acc, b, k, tab);
+      prim_group<C, R, PP, G, BG>(acc, b, k, tab);
     }
     for (; ib < nb; ++ib) {
       PP b[1] = {bra[ib]};
-      prim_group<C, R, PP, 1>(acc, b, k, tab);
+      prim_group<C, R, PP, 1, BG>(acc, b, k, tab);
     }
   }
   C::template hrr<R>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
 }
 
-template <int LA, int LB, int LC, int LD, int G>
+template <int LA, int LB, int LC, int LD, int G, class BG>
 __device__ __forceinline__ void quartet64_jk(const PrimPair64* __restrict__ bra, int nb,
                                              const PrimPair64* __restrict__ kp, int nk, double ABx,
                                              double ABy, double ABz, double CDx, double CDy, double CDz,
                                              const double* __restrict__ tab,
                                              double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
-  quartet_jk<EriClass<LA, LB, LC, LD>, double, PrimPair64, G>(bra, nb, kp, nk, ABx, ABy, ABz, CDx, CDy,
-                                                               CDz, tab, out);
+  quartet_jk<EriClass<LA, LB, LC, LD>, double, PrimPair64, G, BG>(bra, nb, kp, nk, ABx, ABy, ABz, CDx, CDy,
+                                                                   CDz, tab, out);
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, class BG>
 __device__ __forceinline__ void quartet32_jk(const PrimPair32* __restrict__ bra, int nb,
                                              const PrimPair32* __restrict__ kp, int nk, float ABx,
                                              float ABy, float ABz, float CDx, float CDy, float CDz,
                                              const float* __restrict__ tab,
                                              float (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
-  quartet_jk<EriClass<LA, LB, LC, LD>, float, PrimPair32, GX_JKG32>(bra, nb, kp, nk, ABx, ABy, ABz, CDx,
+  quartet_jk<EriClass<LA, LB, LC, LD>, float, PrimPair32, GX_JKG32, BG>(bra, nb, kp, nk, ABx, ABy, ABz, CDx,
                                                                     CDy, CDz, tab, out);
 }
 
@@ -362,6 +362,17 @@ __global__ __launch_bounds__(64) void k_quartets(ShellView sh, PairView pr, int 
 #define JK_TEAM 256  // threads working on one work item
 #endif
 
+// Boys grid of the per-lane J/K kernel of class (la lb|lc ld): the coarse
+// grid (half the LDS of the fine one, two more terms per Taylor series) for
+// the classes with at least six density rows in LDS and at most 54
+// components, whose occupancy LDS rather than registers bounds (measured on
+// MI355X: (ds|ds) 107 -> 82 us at 255 functions; the classes with fewer rows
+// or more components run slower on it).
+__host__ __device__ constexpr int jk_ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr bool jk_coarse_grid(int la, int lb, int lc, int ld) {
+  return jk_ncart(la) + jk_ncart(lb) >= 6 && jk_ncart(la) * jk_ncart(lb) * jk_ncart(lc) * jk_ncart(ld) <= 54;
+}
+
 struct JKArgs {
   ShellView sh;
   PairView pr;
@@ -370,9 +381,10 @@ struct JKArgs {
   const int4* items;     // work items: bra position, ket-list range [v0, v1), p0 | p1 << 16; largest first
   int nitem;
   const int* kets;       // ket lists of the bras (GX_KET_* encoding), indexed by the items' [v0, v1)
-  const double* master;  // Boys master table
-  const double* tab64;   // per-L FP64 Taylor tables (BOYS_NT x BOYS_ROW64 each)
-  const float* tab32;    // per-L FP32 Taylor tables
+  const double* tab64;   // per-L FP64 Taylor tables, fine grid (BoysFine::NT x ROW64 each)
+  const float* tab32;    // per-L FP32 Taylor tables, fine grid
+  const double* tab64c;  // the same, coarse grid (BoysCoarse)
+  const float* tab32c;
   const double* P;       // packed densities, pre-scaled by s_k s_l (the K density)
   const double* P2;      // second K density (two-density builds), pre-scaled
   const double* Pj;      // the J density (P, or P + P2 in two-density builds), pre-scaled
@@ -419,14 +431,18 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
   // item's system)
   constexpr bool T64 = !MIXED;                    // FP64 table staged in LDS
   constexpr bool T32 = MIXED && GX_MIXED_LDS32;   // FP32 table staged in LDS
-  constexpr int off32 = T64 ? BOYS_NT * BOYS_ROW64 : 0;              // in doubles
-  constexpr int offrow = off32 + (T32 ? BOYS_NT * BOYS_ROW32 / 2 : 0);
-  const double* __restrict__ tab64 = T64 ? smem : a.tab64 + C::L * BOYS_NT * BOYS_ROW64;
-  const float* __restrict__ tab32 =
-      T32 ? reinterpret_cast<const float*>(smem + off32) : a.tab32 + C::L * BOYS_NT * BOYS_ROW32;
+  using BG = std::conditional_t<jk_coarse_grid(LA, LB, LC, LD), BoysCoarse, BoysFine>;
+  constexpr int off32 = T64 ? BG::NT * BG::ROW64 : 0;              // in doubles
+  constexpr int offrow = off32 + (T32 ? BG::NT * BG::ROW32 / 2 : 0);
+  const double* __restrict__ gtab64 =
+      (jk_coarse_grid(LA, LB, LC, LD) ? a.tab64c : a.tab64) + C::L * BG::NT * BG::ROW64;
+  const float* __restrict__ gtab32 =
+      (jk_coarse_grid(LA, LB, LC, LD) ? a.tab32c : a.tab32) + C::L * BG::NT * BG::ROW32;
+  const double* __restrict__ tab64 = T64 ? smem : gtab64;
+  const float* __restrict__ tab32 = T32 ? reinterpret_cast<const float*>(smem + off32) : gtab32;
   double* prow = smem + offrow;
-  if (T64) boys_stage64(C::L, a.master, smem);
-  if (T32) boys_stage32(C::L, a.master, reinterpret_cast<float*>(smem + off32));
+  if (T64) boys_copy64<BG>(gtab64, smem);
+  if (T32) boys_copy32<BG>(gtab32, reinterpret_cast<float*>(smem + off32));
 
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = tid / GX_WAVE;
@@ -498,12 +514,12 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
       double val[C::NOUT];
       const double CDx = kq.ABx, CDy = kq.ABy, CDz = kq.ABz;
       if (hi)
-        quartet64_jk<LA, LB, LC, LD, MIXED ? GX_JKGM : GX_JKG>(bra64, nbp, pp64 + kq.poff, kq.np, ABx, ABy,
+        quartet64_jk<LA, LB, LC, LD, MIXED ? GX_JKGM : GX_JKG, BG>(bra64, nbp, pp64 + kq.poff, kq.np, ABx, ABy,
                                                               ABz, CDx, CDy, CDz, tab64, val);
       if constexpr (MIXED) {
         if (lo) {
           float vf[C::NOUT];
-          quartet32_jk<LA, LB, LC, LD>(bra32, nbp, pp32 + kq.poff, kq.np, (float)ABx, (float)ABy, (float)ABz,
+          quartet32_jk<LA, LB, LC, LD, BG>(bra32, nbp, pp32 + kq.poff, kq.np, (float)ABx, (float)ABy, (float)ABz,
                                        (float)CDx, (float)CDy, (float)CDz, tab32, vf);
 #pragma unroll
           for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
@@ -713,13 +729,15 @@ __global__ void k_add(long n, const double* __restrict__ x, const double* __rest
   if (t < n) z[t] = x[t] + y[t];
 }
 
-// Per-L Taylor tables in HBM, the same layout the kernels stage into LDS.
+// Per-L Taylor tables of grid BG in HBM (from the master table of that
+// grid), the same layout the kernels stage into LDS.
+template <class BG>
 __global__ void k_boys_tables(int lmax, const double* __restrict__ master, double* __restrict__ t64,
                               float* __restrict__ t32) {
   const int L = blockIdx.x;
   if (L > lmax) return;
-  boys_stage64(L, master, t64 + (long)L * BOYS_NT * BOYS_ROW64);
-  boys_stage32(L, master, t32 + (long)L * BOYS_NT * BOYS_ROW32);
+  boys_stage64<BG>(L, master, t64 + (long)L * BG::NT * BG::ROW64);
+  boys_stage32<BG>(L, master, t32 + (long)L * BG::NT * BG::ROW32);
 }
 
 __global__ void k_fill(long n, double* __restrict__ x, double v) {
