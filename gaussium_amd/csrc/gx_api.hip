@@ -462,15 +462,16 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
 #define GX_CASE(A, B, C, D)                                                                   \
   if (la == A && lb == B && lc == C && ld == D) {                                             \
     const void* fn = (const void*)gx::k_jk_rys<A, B, C, D>;                                   \
+    constexpr int nt = gx::rys_threads<A, B, C, D>();                                         \
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                            \
     if (*per_cu <= 0) {                                                                       \
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, GX_WAVE, lds);             \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, nt, lds);                  \
       if (e != hipSuccess) return e;                                                          \
       if (*per_cu < 1) *per_cu = 1;                                                           \
     }                                                                                         \
     const int grid = std::min(a.nitem, *per_cu * num_cu);                                    \
-    hipLaunchKernelGGL((gx::k_jk_rys<A, B, C, D>), dim3(grid), dim3(GX_WAVE), lds, s, a);    \
+    hipLaunchKernelGGL((gx::k_jk_rys<A, B, C, D>), dim3(grid), dim3(nt), lds, s, a);         \
     return hipGetLastError();                                                                 \
   }
   GX_FOR_EACH_COOP_CLASS(GX_CASE)

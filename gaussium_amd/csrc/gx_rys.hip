@@ -36,14 +36,38 @@ constexpr int rys_lds() {  // doubles: rules (2n per primitive quartet), 2D tabl
   return rys_pq<LA, LB, LC, LD>() * (2 * R::NROOT + 3 * R::NROOT * R::NG);
 }
 
+// Component of output slot sl: (a, b) = sl % (NA NB), and for
+// t = sl / (NA NB), c = (a + t / ND) % NC, d = (b + t % ND) % ND.
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void rys_slot(int sl, int& ia, int& ib, int& ic, int& id) {
+  const int l = sl % (NA * NB), t = sl / (NA * NB);
+  ia = l / NB;
+  ib = l - ia * NB;
+  ic = (ia + t / ND) % NC;
+  id = (ib + t % ND) % ND;
+}
+
+#ifndef RYS_RECOMPUTE_NPL
+#define RYS_RECOMPUTE_NPL 16
+#endif
+
+// Threads per workgroup (one quartet at a time): two waves for the classes
+// with more than 16 output components per lane of one wave ((dd|dd):
+// 21 -> 11 per lane, 470 -> 256 VGPRs).
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
+constexpr int rys_threads() {
+  return RysClass<LA, LB, LC, LD>::NOUT > 16 * GX_WAVE ? 2 * GX_WAVE : GX_WAVE;
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__((rys_threads<LA, LB, LC, LD>())) void k_jk_rys(JKArgs a) {
   using R = RysClass<LA, LB, LC, LD>;
   constexpr int NR = R::NROOT, NG = R::NG, NOUT = R::NOUT;
   constexpr int NA = R::NA, NB = R::NB, NC = R::NC, ND = R::ND;
   constexpr int SAB = NA * NB, SCD = NC * ND, SAC = NA * NC, SAD = NA * ND, SBC = NB * NC, SBD = NB * ND;
   constexpr int NBLK = SAB + SCD + SAC + SAD + SBC + SBD;
-  constexpr int NPL = (NOUT + GX_WAVE - 1) / GX_WAVE;  // output components per lane
+  constexpr int NT = rys_threads<LA, LB, LC, LD>();
+  constexpr int NPL = (NOUT + NT - 1) / NT;              // output components per thread
   constexpr int PQ = rys_pq<LA, LB, LC, LD>();          // primitive quartets per round
   extern __shared__ double smem[];
   double* __restrict__ RW = smem;                      // [pq][roots u_i, then weights w_i times the prefactor]
@@ -51,21 +75,25 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
   double* __restrict__ pblk = smem + rys_lds<LA, LB, LC, LD>();  // P_ab P_cd P_ac P_ad P_bc P_bd
   double* __restrict__ jblk = pblk + NBLK;             // J_ab J_cd K_ac K_ad K_bc K_bd partials
   const int lane = threadIdx.x;
-  // Output slot s = lane + 64 j holds component (a, b, c, d) with (a, b) =
+  // Output slot s = lane + NT j holds component (a, b, c, d) with (a, b) =
   // s % (NA NB) and, for t = s / (NA NB), c = (a + t / ND) % NC and
   // d = (b + t % ND) % ND: every component once, and the 64 lanes of a slot
   // row meet each J_ab, J_cd, K_ad and K_bc entry at most twice in the
   // digestion's LDS atomics (consecutive components would put 36 lanes on one
-  // J_ab entry).  kidx: the component index; off: its x, y, z 2D offsets.
-  int off[NPL], kidx[NPL];
+  // J_ab entry).  off: the slot's x, y, z 2D offsets, packed.
+  //   With NPL <= RYS_RECOMPUTE_NPL the slot's indices and the decoded
+  // offsets are recomputed where they are used, from copies the compiler
+  // cannot see through, instead of being kept in registers across the
+  // primitive loop: measured to lift (dd|dp) from one to two waves per SIMD
+  // (281 -> 208 VGPRs, 115 -> 98 us at the 255-function cluster); (dd|dd),
+  // one wave per SIMD either way, runs faster with them kept.
+  constexpr bool RECOMPUTE = NPL <= RYS_RECOMPUTE_NPL;
+  int off[NPL];
 #pragma unroll
   for (int j = 0; j < NPL; ++j) {
-    const int sl = lane + j * GX_WAVE;
-    const int l = sl % (NA * NB), t = sl / (NA * NB);
-    const int ia = l / NB, ib = l - ia * NB;
-    const int ic = (ia + t / ND) % NC, id = (ib + t % ND) % ND;
-    kidx[j] = ((ia * NB + ib) * NC + ic) * ND + id;
-    off[j] = sl < NOUT ? R::off()[kidx[j]] : 0;
+    int ia, ib, ic, id;
+    rys_slot<NA, NB, NC, ND>(lane + j * NT, ia, ib, ic, id);
+    off[j] = lane + j * NT < NOUT ? R::off()[((ia * NB + ib) * NC + ic) * ND + id] : 0;
   }
   // static striding over the items (their claiming from a counter measured
   // slower for the one-quartet-per-wave kernels)
@@ -91,11 +119,11 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
       if (jp == ip) f *= 0.5;
       // density blocks of this quartet, loaded into registers now (in flight
       // during the primitive loop) and stored to LDS after it
-      constexpr int NPB = (NBLK + GX_WAVE - 1) / GX_WAVE;
+      constexpr int NPB = (NBLK + NT - 1) / NT;
       double pv[NPB];
 #pragma unroll
       for (int i = 0; i < NPB; ++i) {
-        const int t = lane + i * GX_WAVE;
+        const int t = lane + i * NT;
         int r0, c0, s;
         int u = t < NBLK ? t : 0;
         if (u < SAB) { r0 = ao; c0 = bo; s = NB; }
@@ -151,8 +179,10 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
           const double* __restrict__ Gq = Gt + pp * 3 * NR * NG;
 #pragma unroll
           for (int j = 0; j < NPL; ++j) {
-            if (lane + j * GX_WAVE < NOUT) {
-              const int ox = off[j] & 1023, oy = (off[j] >> 10) & 1023, oz = off[j] >> 20;
+            if (lane + j * NT < NOUT) {
+              int o = off[j];
+              if constexpr (RECOMPUTE) asm volatile("" : "+v"(o));
+              const int ox = o & 1023, oy = (o >> 10) & 1023, oz = o >> 20;
               double s = 0.0;
 #pragma unroll
               for (int r = 0; r < NR; ++r) {
@@ -166,7 +196,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
       }
 #pragma unroll
       for (int i = 0; i < NPB; ++i) {
-        const int t = lane + i * GX_WAVE;
+        const int t = lane + i * NT;
         if (t < NBLK) {
           pblk[t] = pv[i];
           jblk[t] = 0.0;
@@ -185,11 +215,13 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
       double* kad = kac + SAC;
       double* kbc = kad + SAD;
       double* kbd = kbc + SBC;
+      int ln = lane;
+      if constexpr (RECOMPUTE) asm volatile("" : "+v"(ln));
 #pragma unroll
       for (int j = 0; j < NPL; ++j) {
-        if (lane + j * GX_WAVE < NOUT) {
-          const int k = kidx[j];
-          const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
+        if (lane + j * NT < NOUT) {
+          int ia, ib, ic, id;
+          rys_slot<NA, NB, NC, ND>(ln + j * NT, ia, ib, ic, id);
           const double x = f * acc[j];
           atomicAdd(&jab[ia * NB + ib], x * pcd[ic * ND + id]);
           atomicAdd(&jcd[ic * ND + id], x * pab[ia * NB + ib]);
@@ -200,7 +232,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_jk_rys(JKArgs a) {
         }
       }
       __syncthreads();
-      for (int t = lane; t < NBLK; t += GX_WAVE) {
+      for (int t = lane; t < NBLK; t += NT) {
         const double x = jblk[t];
         if (x == 0.0) continue;
         int r0, c0, s;
