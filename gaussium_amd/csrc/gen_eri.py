@@ -86,8 +86,9 @@ class Emitter:
         return name
 
 
-def gen_vrr(la, lb, lc, ld):
-    """Body of prim<R>: returns (lines, accumulator layout)."""
+def gen_vrr(la, lb, lc, ld, fsub=None):
+    """Body of prim<R>: returns (lines, accumulator layout).  fsub (ld = 0
+    only): the ket components to accumulate (all when None)."""
     L = la + lb + lc + ld
     em = Emitter('R')
     memo = {}
@@ -138,7 +139,8 @@ def gen_vrr(la, lb, lc, ld):
         for e in cart(le):
             for lf in range(lc, lc + ld + 1):
                 for f in cart(lf):
-                    layout.append((e, f))
+                    if fsub is None or f in fsub:
+                        layout.append((e, f))
     # each accumulation right after the value it adds is first available
     for idx, (e, f) in enumerate(layout):
         v = node(e, f, 0)
@@ -147,8 +149,9 @@ def gen_vrr(la, lb, lc, ld):
     return em.lines, layout
 
 
-def gen_hrr(la, lb, lc, ld, layout):
-    """Body of hrr<R>: contracted [e0|f0] -> (ab|cd) in reference component order."""
+def gen_hrr(la, lb, lc, ld, layout, fsub=None):
+    """Body of hrr<R>: contracted [e0|f0] -> (ab|cd) in reference component order
+    (fsub: only the ket components c in it, in their cart(lc) order)."""
     em = Emitter('R')
     index = {ef: k for k, ef in enumerate(layout)}
     zero = (0, 0, 0)
@@ -185,6 +188,8 @@ def gen_hrr(la, lb, lc, ld, layout):
     for a in cart(la):
         for b in cart(lb):
             for c in cart(lc):
+                if fsub is not None and c not in fsub:
+                    continue
                 for d in cart(ld):
                     out.append('    out[%d] = %s;' % (k, ket(a, b, c, d)))
                     k += 1
@@ -209,6 +214,65 @@ def emit_class(la, lb, lc, ld):
     s.append('template <> struct EriClass<%d, %d, %d, %d> {' % (la, lb, lc, ld))
     s.append('  static constexpr int L = %d, NACC = %d, NOUT = %d;' % (L, len(layout), na * nb * nc * nd))
     s.append('  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;' % (na, nb, nc, nd))
+    s.append('  template <typename R>')
+    s.append('  static __device__ __forceinline__ void prim(R (&acc)[NACC], const R (&F)[L + 1],')
+    s.append('      R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,')
+    s.append('      R WQx, R WQy, R WQz, R o2p, R o2q, R o2pq, R rho_p, R rho_q) {')
+    s.append('    (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz;')
+    s.append('    (void)QCx; (void)QCy; (void)QCz; (void)WQx; (void)WQy; (void)WQz;')
+    s.append('    (void)o2p; (void)o2q; (void)o2pq; (void)rho_p; (void)rho_q;')
+    s += vrr
+    s.append('  }')
+    s.append('  template <typename R>')
+    s.append('  static __device__ __forceinline__ void hrr(const R (&acc)[NACC], R ABx, R ABy, R ABz,')
+    s.append('      R CDx, R CDy, R CDz, R (&out)[NOUT]) {')
+    s.append('    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;')
+    s += hrr
+    s.append('  }')
+    s.append('};')
+    return '\n'.join(s)
+
+
+# ---------------------------------------------------------------------------
+# Per-lane J/K classes without ket HRR (ld = 0) whose contracted
+# intermediates exceed JK_PASS_NACC run in passes over the ket components:
+# (ab|c0) for the components c of one pass needs only the [e0|c]
+# accumulators of those c, and the J/K digestion of the passes adds up.  A
+# pass repeats the primitive loop (Boys function and bra recursion) but keeps
+# a fraction of the registers: (dd|ds) takes 186 accumulators and 216 outputs
+# per lane in one pass (512 registers, 1.4 KB of scratch per lane).
+JK_PASS_NACC = 64
+
+
+def jk_lane_passes(c):
+    """Ket-component groups (indices into cart(lc)) of the J/K passes of a
+    per-lane class, grouped by the axis their recursion lowers so that a pass
+    shares its [e0|p] intermediates; one group when the class runs in one pass."""
+    la, lb, lc, ld = c
+    comps = cart(lc)
+    ne = sum(ncart(e) for e in range(la, la + lb + 1))
+    if ld != 0 or is_coop(c) or ne * len(comps) <= JK_PASS_NACC:
+        return [list(range(len(comps)))]
+    per = max(1, JK_PASS_NACC // ne)
+    npass = -(-len(comps) // per)
+    order = sorted(range(len(comps)), key=lambda k: (pick_axis(comps[k]), k))
+    return [sorted(order[len(order) * i // npass:len(order) * (i + 1) // npass]) for i in range(npass)]
+
+
+def emit_jk_pass(la, lb, lc, ld, P, group):
+    comps = cart(lc)
+    fsub = [comps[k] for k in group]
+    vrr, layout = gen_vrr(la, lb, lc, ld, fsub)
+    hrr = gen_hrr(la, lb, lc, ld, layout, fsub)
+    L = la + lb + lc + ld
+    na, nb, nd = ncart(la), ncart(lb), ncart(ld)
+    sel = ''.join('i == %d ? %d : ' % (i, k) for i, k in enumerate(group[:-1])) + '%d' % group[-1]
+    s = []
+    s.append('template <> struct EriJKPass<%d, %d, %d, %d, %d> {' % (la, lb, lc, ld, P))
+    s.append('  static constexpr int L = %d, NACC = %d, NOUT = %d;' % (L, len(layout), na * nb * len(group) * nd))
+    s.append('  static constexpr int NA = %d, NB = %d, NC = %d, ND = %d;' % (na, nb, len(group), nd))
+    s.append('  // ket component (index into cart(lc)) of the pass\'s component i')
+    s.append('  static __host__ __device__ constexpr int cidx(int i) { return %s; }' % sel)
     s.append('  template <typename R>')
     s.append('  static __device__ __forceinline__ void prim(R (&acc)[NACC], const R (&F)[L + 1],')
     s.append('      R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz,')
@@ -794,11 +858,21 @@ def main():
              'template <int LA, int LB, int LC, int LD> struct EriProg;',
              'template <int LA, int LB, int LC, int LD> struct EriGradProg;',
              'template <int LA, int LB, int LC, int LD> struct EriGradLane;',
-             'template <int LA, int LB, int LC, int LD, int X> struct EriGradLanePass;']
+             'template <int LA, int LB, int LC, int LD, int X> struct EriGradLanePass;',
+             'template <int LA, int LB, int LC, int LD, int P> struct EriJKPass;',
+             '// J/K passes of a per-lane class (EriJKPass<..., 0 .. NPASS - 1> when NPASS > 1)\n'
+             'template <int LA, int LB, int LC, int LD> struct EriJKLane {\n'
+             '  static constexpr int NPASS = 1;\n};']
     lane = [c for c in classes(lmax) if not is_coop(c)]
     coop = [c for c in classes(lmax) if is_coop(c)]
     for c in lane:
         parts.append(emit_class(*c))
+        groups = jk_lane_passes(c)
+        if len(groups) > 1:
+            parts.append('template <> struct EriJKLane<%d, %d, %d, %d> {\n  static constexpr int NPASS = %d;\n};'
+                         % (c + (len(groups),)))
+            for P, g in enumerate(groups):
+                parts.append(emit_jk_pass(*c, P, g))
     for c in classes(lmax):
         parts.append(emit_prog(*c))
     for c in classes(lmax):

@@ -221,24 +221,28 @@ __device__ __forceinline__ void quartet_jk(const PP* __restrict__ bra, int nb, c
   C::template hrr<R>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
 }
 
-template <int LA, int LB, int LC, int LD, int G, class BG>
-__device__ __forceinline__ void quartet64_jk(const PrimPair64* __restrict__ bra, int nb,
-                                             const PrimPair64* __restrict__ kp, int nk, double ABx,
-                                             double ABy, double ABz, double CDx, double CDy, double CDz,
-                                             const double* __restrict__ tab,
-                                             double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
-  quartet_jk<EriClass<LA, LB, LC, LD>, double, PrimPair64, G, BG>(bra, nb, kp, nk, ABx, ABy, ABz, CDx, CDy,
-                                                                   CDz, tab, out);
-}
+// The class code of one J/K pass: a class evaluated in one pass (every ket
+// component, cidx the identity), or pass P of a class evaluated in passes
+// over its ket components (EriJKPass, gen_eri.py).
+template <class C>
+struct JKOnePass : C {
+  static __host__ __device__ constexpr int cidx(int i) { return i; }
+};
+template <class T>
+struct JKPassTag {
+  using type = T;
+};
+template <int LA, int LB, int LC, int LD, int P>
+using JKPassClass = std::conditional_t<(EriJKLane<LA, LB, LC, LD>::NPASS > 1), EriJKPass<LA, LB, LC, LD, P>,
+                                       JKOnePass<EriClass<LA, LB, LC, LD>>>;
 
-template <int LA, int LB, int LC, int LD, class BG>
-__device__ __forceinline__ void quartet32_jk(const PrimPair32* __restrict__ bra, int nb,
-                                             const PrimPair32* __restrict__ kp, int nk, float ABx,
-                                             float ABy, float ABz, float CDx, float CDy, float CDz,
-                                             const float* __restrict__ tab,
-                                             float (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
-  quartet_jk<EriClass<LA, LB, LC, LD>, float, PrimPair32, GX_JKG32, BG>(bra, nb, kp, nk, ABx, ABy, ABz, CDx,
-                                                                    CDy, CDz, tab, out);
+// fn(JKPassTag<pass class>) for the passes P, P + 1, .. of the class.
+template <int LA, int LB, int LC, int LD, int P = 0, class F>
+__device__ __forceinline__ void jk_passes(F& fn) {
+  if constexpr (P < EriJKLane<LA, LB, LC, LD>::NPASS) {
+    fn(JKPassTag<JKPassClass<LA, LB, LC, LD, P>>{});
+    jk_passes<LA, LB, LC, LD, P + 1>(fn);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -418,8 +422,17 @@ __device__ __forceinline__ double* jkp_copy(const JKArgs& a) {
 // small and the occupancy up (staging both measured no faster).
 // NDEN = 2 (FP64 only): one evaluation of the integrals serves J of
 // P + P2 and K of P and of P2 (unrestricted Fock builds).
+// Classes whose per-lane J/K kernel is held to 256 VGPRs (two waves per
+// SIMD): they need a few more without the bound and run one wave per SIMD
+// then (measured on MI355X at the 255-function cluster: (ds|ds) 105 -> 81 us,
+// (ds|pp) 65 -> 57 us; (dp|ps) and (dd|ss) were slower bounded).
+template <int LA, int LB, int LC, int LD>
+constexpr int jk_min_waves() {
+  return ((LA == 2 && LB == 0 && LC == 2 && LD == 0) || (LA == 2 && LB == 0 && LC == 1 && LD == 1)) ? 2 : 1;
+}
+
 template <int LA, int LB, int LC, int LD, bool MIXED, int NDEN>
-__global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
+__global__ __launch_bounds__(JK_TEAM) __attribute__((amdgpu_waves_per_eu((jk_min_waves<LA, LB, LC, LD>())))) void k_jk(JKArgs a, const PrimPair64* __restrict__ pp64,
                                                 const PrimPair32* __restrict__ pp32) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
@@ -511,34 +524,40 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
       const bool lo = act && (ke & GX_KET_FP32);
       const bool hi = act && !(ke & GX_KET_FP32);
       const PairRec kq = a.pr.rec[jp];
-      double val[C::NOUT];
       const double CDx = kq.ABx, CDy = kq.ABy, CDz = kq.ABz;
-      if (hi)
-        quartet64_jk<LA, LB, LC, LD, MIXED ? GX_JKGM : GX_JKG, BG>(bra64, nbp, pp64 + kq.poff, kq.np, ABx, ABy,
-                                                              ABz, CDx, CDy, CDz, tab64, val);
-      if constexpr (MIXED) {
-        if (lo) {
-          float vf[C::NOUT];
-          quartet32_jk<LA, LB, LC, LD, BG>(bra32, nbp, pp32 + kq.poff, kq.np, (float)ABx, (float)ABy, (float)ABz,
-                                       (float)CDx, (float)CDy, (float)CDz, tab32, vf);
+      // Evaluation and digestion of the ket components PC::cidx(0 .. PC::NC)
+      // of the quartet: all of them, or those of one pass.
+      auto pass = [&](auto tag) {
+        using PC = typename decltype(tag)::type;
+        constexpr int NCP = PC::NC;
+        double val[PC::NOUT];
+        if (hi)
+          quartet_jk<PC, double, PrimPair64, MIXED ? GX_JKGM : GX_JKG, BG>(bra64, nbp, pp64 + kq.poff, kq.np, ABx,
+                                                                          ABy, ABz, CDx, CDy, CDz, tab64, val);
+        if constexpr (MIXED) {
+          if (lo) {
+            float vf[PC::NOUT];
+            quartet_jk<PC, float, PrimPair32, GX_JKG32, BG>(bra32, nbp, pp32 + kq.poff, kq.np, (float)ABx,
+                                                             (float)ABy, (float)ABz, (float)CDx, (float)CDy,
+                                                             (float)CDz, tab32, vf);
 #pragma unroll
-          for (int k = 0; k < C::NOUT; ++k) val[k] = (double)vf[k];
+            for (int k = 0; k < PC::NOUT; ++k) val[k] = (double)vf[k];
+          }
         }
-      }
-      if (act) {
+        if (!act) return;
         const int co = kq.fa, dof = kq.fb;
         double f = fA * ((kq.A == kq.B) ? 0.5 : 1.0);
         if (jp == ip) f *= 0.5;
 #pragma unroll
-        for (int k = 0; k < C::NOUT; ++k) val[k] *= f;
-        double Pcd[NC * ND];
+        for (int k = 0; k < PC::NOUT; ++k) val[k] *= f;
+        double Pcd[NCP * ND];
 #pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
+        for (int ic = 0; ic < NCP; ++ic)
 #pragma unroll
-          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pjm[(long)(co + ic) * N + dof + id];
-        double jcd[NC * ND];
+          for (int id = 0; id < ND; ++id) Pcd[ic * ND + id] = Pjm[(long)(co + PC::cidx(ic)) * N + dof + id];
+        double jcd[NCP * ND];
 #pragma unroll
-        for (int i = 0; i < NC * ND; ++i) jcd[i] = 0.0;
+        for (int i = 0; i < NCP * ND; ++i) jcd[i] = 0.0;
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -546,30 +565,30 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
             double pab = prow[ia * N + bo + ib];
             if (NDEN == 2) pab += prow[nrk + ia * N + bo + ib];
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
+            for (int ic = 0; ic < NCP; ++ic)
 #pragma unroll
               for (int id = 0; id < ND; ++id) {
-                const double x = val[((ia * NB + ib) * NC + ic) * ND + id];
+                const double x = val[((ia * NB + ib) * NCP + ic) * ND + id];
                 jab[ia * NB + ib] = fma(x, Pcd[ic * ND + id], jab[ia * NB + ib]);
                 jcd[ic * ND + id] = fma(x, pab, jcd[ic * ND + id]);
               }
           }
 #pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
+        for (int ic = 0; ic < NCP; ++ic)
 #pragma unroll
           for (int id = 0; id < ND; ++id)
-            atomicAdd(&Jkp[kq.jko + (long)(ic * ND + id) * a.jkcs], 2.0 * jcd[ic * ND + id]);
+            atomicAdd(&Jkp[kq.jko + (long)(PC::cidx(ic) * ND + id) * a.jkcs], 2.0 * jcd[ic * ND + id]);
 #pragma unroll
         for (int d = 0; d < NDEN; ++d) {
           const double* __restrict__ pr = prow + d * nrk;
           double* kr = krow + d * nrk;
-          double kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
+          double kac[NA * NCP], kad[NA * ND], kbc[NB * NCP], kbd[NB * ND];
 #pragma unroll
-          for (int i = 0; i < NA * NC; ++i) kac[i] = 0.0;
+          for (int i = 0; i < NA * NCP; ++i) kac[i] = 0.0;
 #pragma unroll
           for (int i = 0; i < NA * ND; ++i) kad[i] = 0.0;
 #pragma unroll
-          for (int i = 0; i < NB * NC; ++i) kbc[i] = 0.0;
+          for (int i = 0; i < NB * NCP; ++i) kbc[i] = 0.0;
 #pragma unroll
           for (int i = 0; i < NB * ND; ++i) kbd[i] = 0.0;
 #pragma unroll
@@ -577,36 +596,38 @@ __global__ __launch_bounds__(JK_TEAM) void k_jk(JKArgs a, const PrimPair64* __re
 #pragma unroll
             for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-              for (int ic = 0; ic < NC; ++ic) {
-                const double pac = pr[ia * N + co + ic];
-                const double pbc = pr[(rowB + ib) * N + co + ic];
+              for (int ic = 0; ic < NCP; ++ic) {
+                const double pac = pr[ia * N + co + PC::cidx(ic)];
+                const double pbc = pr[(rowB + ib) * N + co + PC::cidx(ic)];
 #pragma unroll
                 for (int id = 0; id < ND; ++id) {
-                  const double x = val[((ia * NB + ib) * NC + ic) * ND + id];
+                  const double x = val[((ia * NB + ib) * NCP + ic) * ND + id];
                   const double pad = pr[ia * N + dof + id];
                   const double pbd = pr[(rowB + ib) * N + dof + id];
-                  kac[ia * NC + ic] = fma(x, pbd, kac[ia * NC + ic]);
+                  kac[ia * NCP + ic] = fma(x, pbd, kac[ia * NCP + ic]);
                   kad[ia * ND + id] = fma(x, pbc, kad[ia * ND + id]);
-                  kbc[ib * NC + ic] = fma(x, pad, kbc[ib * NC + ic]);
+                  kbc[ib * NCP + ic] = fma(x, pad, kbc[ib * NCP + ic]);
                   kbd[ib * ND + id] = fma(x, pac, kbd[ib * ND + id]);
                 }
               }
 #pragma unroll
           for (int ia = 0; ia < NA; ++ia) {
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic) atomicAdd(&kr[ia * N + co + ic], kac[ia * NC + ic]);
+            for (int ic = 0; ic < NCP; ++ic) atomicAdd(&kr[ia * N + co + PC::cidx(ic)], kac[ia * NCP + ic]);
 #pragma unroll
             for (int id = 0; id < ND; ++id) atomicAdd(&kr[ia * N + dof + id], kad[ia * ND + id]);
           }
 #pragma unroll
           for (int ib = 0; ib < NB; ++ib) {
 #pragma unroll
-            for (int ic = 0; ic < NC; ++ic) atomicAdd(&kr[(rowB + ib) * N + co + ic], kbc[ib * NC + ic]);
+            for (int ic = 0; ic < NCP; ++ic)
+              atomicAdd(&kr[(rowB + ib) * N + co + PC::cidx(ic)], kbc[ib * NCP + ic]);
 #pragma unroll
             for (int id = 0; id < ND; ++id) atomicAdd(&kr[(rowB + ib) * N + dof + id], kbd[ib * ND + id]);
           }
         }
-      }
+      };
+      jk_passes<LA, LB, LC, LD>(pass);
 #if GX_DYN_CHUNK
       int ck = 0;
       if (lane == 0) ck = atomicAdd(&next_chunk, 1);

@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #define __device__
+#define __host__
 #define __forceinline__ inline
 struct int2 {
   int x, y;
@@ -43,9 +44,10 @@ struct Shell {
   const double* w;
 };
 
-template <int LA, int LB, int LC, int LD>
-static void quartet(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
-  using K = EriClass<LA, LB, LC, LD>;
+// One contracted quartet through class code K (EriClass, or one J/K pass
+// EriJKPass): out[0 .. K::NOUT) in K's component order.
+template <class K>
+static void quartet_cls(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
   double acc[K::NACC] = {0};
   const double two_pi_52 = 34.98683665524972;
   for (int i = 0; i < A.n; ++i)
@@ -83,6 +85,34 @@ static void quartet(const Shell& A, const Shell& B, const Shell& C, const Shell&
   K::template hrr<double>(acc, A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2], C.c[0] - D.c[0],
                           C.c[1] - D.c[1], C.c[2] - D.c[2], o);
   for (int t = 0; t < K::NOUT; ++t) out[t] = o[t];
+}
+
+// The J/K passes P.. of a class evaluated in passes over its ket components,
+// each pass's outputs put at their place in the class's component order.
+template <int LA, int LB, int LC, int LD, int P>
+static void quartet_passes(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
+  if constexpr (P < EriJKLane<LA, LB, LC, LD>::NPASS) {
+    using K = EriJKPass<LA, LB, LC, LD, P>;
+    double o[K::NOUT];
+    quartet_cls<K>(A, B, C, D, o);
+    constexpr int NCF = (LC + 1) * (LC + 2) / 2;
+    for (int ia = 0; ia < K::NA; ++ia)
+      for (int ib = 0; ib < K::NB; ++ib)
+        for (int ic = 0; ic < K::NC; ++ic)
+          for (int id = 0; id < K::ND; ++id)
+            out[((ia * K::NB + ib) * NCF + K::cidx(ic)) * K::ND + id] = o[((ia * K::NB + ib) * K::NC + ic) * K::ND + id];
+    quartet_passes<LA, LB, LC, LD, P + 1>(A, B, C, D, out);
+  }
+}
+
+// A per-lane class the way the J/K kernel evaluates it: in one pass, or in
+// passes over the ket components.
+template <int LA, int LB, int LC, int LD>
+static void quartet(const Shell& A, const Shell& B, const Shell& C, const Shell& D, double* out) {
+  if constexpr (EriJKLane<LA, LB, LC, LD>::NPASS > 1)
+    quartet_passes<LA, LB, LC, LD, 0>(A, B, C, D, out);
+  else
+    quartet_cls<EriClass<LA, LB, LC, LD>>(A, B, C, D, out);
 }
 
 template <class G>

@@ -92,7 +92,8 @@ CLASSES = [(la, lb, lc, ld) for la in range(3) for lb in range(la + 1)
 @pytest.mark.parametrize('cls', CLASSES, ids=lambda c: '%d%d%d%d' % c)
 def test_generated_class_matches_oracle(host_lib, cls, entry):
     """``host_quartet``: the code path the device takes for the class (per-lane
-    class code, or the cooperative program for the large classes);
+    class code, in passes over the ket components for the large classes
+    without ket HRR, or the cooperative program for the large classes);
     ``host_quartet_prog``: the cooperative program of every class, run level by
     level with a check that each source slot is written before it is read;
     ``host_quartet_rys``: Rys quadrature (tabulated roots and weights, 2D
