@@ -1626,11 +1626,9 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
                                     ", needs " + std::to_string(lds) + " bytes)");
     }
   const unsigned g = (unsigned)((nn + 255) / 256);
-  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
-                     c->d_sbfb, c->d_scale, P, c->d_Ps);
-  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((c->jt_doubles() + 255) / 256)), dim3(256), 0, c->stream,
-                     c->jt_doubles(), c->d_Jt, 0.0);
-  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, c->d_Kt, 0.0);
+  const long njt = c->jt_doubles();
+  hipLaunchKernelGGL(gx::k_jk_prepare, dim3((unsigned)((std::max(nn, njt) + 255) / 256)), dim3(256), 0, c->stream,
+                     nn, c->nsys, c->d_smb, c->d_sn, c->d_sbfb, c->d_scale, P, c->d_Ps, c->d_Kt, njt, c->d_Jt);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, fp32_bound, mixed));
   const bool timing = (flags & GX_JK_TIMING) != 0;
@@ -1708,11 +1706,9 @@ int gx_jk_noalloc(gx_ctx* c, const double* P, double* J, double* K, double cutof
   const unsigned g = (unsigned)((nn + 255) / 256);
   // the scratch keeps the previous call's contents: the accumulators are
   // cleared by kernels (no memset node, so a captured call stays replayable)
-  hipLaunchKernelGGL(gx::k_scale_density, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
-                     c->d_sbfb, c->d_scale, P, Ps);
-  hipLaunchKernelGGL(gx::k_fill, dim3((unsigned)((c->jt_doubles() + 255) / 256)), dim3(256), 0, c->stream,
-                     c->jt_doubles(), Jt, 0.0);
-  hipLaunchKernelGGL(gx::k_fill, dim3(g), dim3(256), 0, c->stream, nn, Kt, 0.0);
+  const long njt = c->jt_doubles();
+  hipLaunchKernelGGL(gx::k_jk_prepare, dim3((unsigned)((std::max(nn, njt) + 255) / 256)), dim3(256), 0, c->stream,
+                     nn, c->nsys, c->d_smb, c->d_sn, c->d_sbfb, c->d_scale, P, Ps, Kt, njt, Jt);
   GX_HIP(hipGetLastError());
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)

@@ -744,6 +744,21 @@ __global__ void k_scale_density(long total, int nsys, const long* __restrict__ m
   Ps[t] = P[t] * scale[bfb[s] + i] * scale[bfb[s] + j];
 }
 
+// Start of a J/K build in one launch: Ps = s P s (packed, `total`
+// elements), Kt = 0 over the same elements, Jt = 0 over its `njt` doubles.
+__global__ void k_jk_prepare(long total, int nsys, const long* __restrict__ mb, const int* __restrict__ n,
+                             const int* __restrict__ bfb, const double* __restrict__ scale,
+                             const double* __restrict__ P, double* __restrict__ Ps, double* __restrict__ Kt,
+                             long njt, double* __restrict__ Jt) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < njt) Jt[t] = 0.0;
+  if (t >= total) return;
+  int s, i, j;
+  packed_index(t, nsys, mb, n, s, i, j);
+  Ps[t] = P[t] * scale[bfb[s] + i] * scale[bfb[s] + j];
+  Kt[t] = 0.0;
+}
+
 // z = x + y, elementwise.
 __global__ void k_add(long n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ z) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
