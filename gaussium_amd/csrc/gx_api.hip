@@ -555,8 +555,8 @@ size_t jk_lds_bytes(int la, int lb, int lc, int ld, int N, int mixed, int nden =
   GX_FOR_EACH_COOP_CLASS(GX_CASE)
 #undef GX_CASE
   const int na = ncart(la), nb = ncart(lb);
-  const bool t64 = !mixed, t32 = mixed && GX_MIXED_LDS32;
-  const bool coarse = gx::jk_coarse_grid(la, lb, lc, ld);
+  const bool t64 = true, t32 = mixed && GX_MIXED_LDS32;
+  const bool coarse = gx::jk_coarse_grid(la, lb, lc, ld) || mixed;
   const size_t nt = coarse ? gx::BoysCoarse::NT : gx::BoysFine::NT;
   const size_t r64 = coarse ? gx::BoysCoarse::ROW64 : gx::BoysFine::ROW64;
   const size_t r32 = coarse ? gx::BoysCoarse::ROW32 : gx::BoysFine::ROW32;

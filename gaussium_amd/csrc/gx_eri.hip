@@ -416,10 +416,13 @@ __device__ __forceinline__ double* jkp_copy(const JKArgs& a) {
 }
 
 // MIXED = false: pure FP64 build, the FP32 path is compiled out and the FP64
-// Taylor table is staged in LDS.  MIXED = true: most primitive quartets are
-// FP32; the FP32 table is staged in LDS and the FP64 one (for the few FP64
-// quartets) is read through the vector-memory path (L1), which keeps the LDS
-// small and the occupancy up (staging both measured no faster).
+// Taylor table is staged in LDS.  MIXED = true: the quartets below the FP32
+// bound are FP32, the others FP64; both tables are staged in LDS, on the
+// coarse grid (13.9 + 4.6 KB for every class).  Measured on MI355X at the
+// 220-function cluster: with the FP64 table read through L1 instead, the
+// FP64 quartets of a mixed (ss|ss) launch made it slower than the all-FP64
+// kernel (211 vs 129 us) and the autotuner kept it in FP64; staged, (ss|ss)
+// runs in 125 us and the mixed build takes 0.379 ms (0.430 before).
 // NDEN = 2 (FP64 only): one evaluation of the integrals serves J of
 // P + P2 and K of P and of P2 (unrestricted Fock builds).
 // Classes whose per-lane J/K kernel is held to 256 VGPRs (two waves per
@@ -442,15 +445,16 @@ __global__ __launch_bounds__(JK_TEAM) __attribute__((amdgpu_waves_per_eu((jk_min
   const int tid = threadIdx.x;
   // dynamic LDS: Boys FP64 (pure FP64 build) | P rows | K rows (rows of the
   // item's system)
-  constexpr bool T64 = !MIXED;                    // FP64 table staged in LDS
+  constexpr bool T64 = true;                      // FP64 table staged in LDS
   constexpr bool T32 = MIXED && GX_MIXED_LDS32;   // FP32 table staged in LDS
-  using BG = std::conditional_t<jk_coarse_grid(LA, LB, LC, LD), BoysCoarse, BoysFine>;
+  constexpr bool COARSE = jk_coarse_grid(LA, LB, LC, LD) || MIXED;
+  using BG = std::conditional_t<COARSE, BoysCoarse, BoysFine>;
   constexpr int off32 = T64 ? BG::NT * BG::ROW64 : 0;              // in doubles
   constexpr int offrow = off32 + (T32 ? BG::NT * BG::ROW32 / 2 : 0);
   const double* __restrict__ gtab64 =
-      (jk_coarse_grid(LA, LB, LC, LD) ? a.tab64c : a.tab64) + C::L * BG::NT * BG::ROW64;
+      (COARSE ? a.tab64c : a.tab64) + C::L * BG::NT * BG::ROW64;
   const float* __restrict__ gtab32 =
-      (jk_coarse_grid(LA, LB, LC, LD) ? a.tab32c : a.tab32) + C::L * BG::NT * BG::ROW32;
+      (COARSE ? a.tab32c : a.tab32) + C::L * BG::NT * BG::ROW32;
   const double* __restrict__ tab64 = T64 ? smem : gtab64;
   const float* __restrict__ tab32 = T32 ? reinterpret_cast<const float*>(smem + off32) : gtab32;
   double* prow = smem + offrow;
