@@ -48,8 +48,13 @@ class TwoElectronRepulsionMatrixOS:
         return float(self.engine.eri_quartets([(i, j, k, l)])[0])
 
     def create_repulsion_matrix(self):
-        """Dense [N,N,N,N] numpy tensor (small N; the SCF path never forms it)."""
-        return self.engine.eri_dense().cpu().numpy()
+        """Dense [N,N,N,N] numpy tensor (small N; the SCF path never forms it).
+        With a ``Symmetry`` of a point group other than C_1, the integrals it
+        forces to zero are exactly 0.0 (electron_repulsion_matrix.py:18-26)."""
+        out = self.engine.eri_dense().cpu().numpy()
+        if self.symmetry is not None and hasattr(self.symmetry, 'zero_mask'):
+            out[self.symmetry.zero_mask()] = 0.0
+        return out
 
 
 class FockMatrixRestricted:

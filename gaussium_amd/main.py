@@ -4,8 +4,12 @@ main.py:11-34 and energy/energy.py:17-81 of the reference.  The 'RHF',
 'UHF', 'GUHF' and 'MP2' methods run on this engine (the other branches of
 ``Energy.calculate_energy`` -- DFT, CCSD, TDHF, CIS -- are outside the
 ERI + J/K path and raise ``NotImplementedError``).
-``symmetry=True`` is accepted; point-group quartet skipping is not
-performed (the result is the same energy, as in the reference).
+``symmetry=True`` centres and orients the molecule and finds its point
+group (``symmetry.MoleculeFactory``, main.py:25-27 of the reference); the
+``Symmetry`` object then zeroes the integrals the group forbids in the dense
+repulsion matrix (``TwoElectronRepulsionMatrixOS``).  The direct J/K build
+of the SCF evaluates whole shell quartets and is unaffected: the energy is
+the same as without symmetry.
 """
 import time
 
@@ -21,6 +25,7 @@ class Energy:
         self.method = method
         self.options = options
         self.symmetry_object = None
+        self.point_group = None
         self.last = None
 
     def calculate_energy(self, nuclei_array, basis_set):
@@ -51,6 +56,11 @@ def start(mol_file, basis_file, method, processors, symmetry=False, geometry_opt
     t0 = time.perf_counter()
     nuclei, electrons, multiplicity = read_mol_file(mol_file)
     energy_object = Energy(electrons, multiplicity, processors, method, **options)
+    if symmetry and geometry_optimization is None:
+        from .symmetry import MoleculeFactory, Symmetry
+        nuclei, point_group = MoleculeFactory(symmetry=True).create(nuclei)
+        energy_object.point_group = point_group
+        energy_object.symmetry_object = Symmetry(point_group, read_basis_set_file(basis_file, nuclei))
     if geometry_optimization is not None:
         from .geometry import optimize
         energy = optimize(basis_file, energy_object, nuclei, method=geometry_optimization)

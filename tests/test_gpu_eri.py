@@ -96,3 +96,36 @@ def test_dense_symmetry_and_schwarz(Engine):
     bound = np.sqrt(np.abs(diag)).reshape(n * n)
     m = d.reshape(n * n, n * n)
     assert np.all(np.abs(m) <= np.outer(bound, bound) * (1 + 1e-12) + 1e-14)
+
+
+def test_dense_repulsion_matrix_with_point_group(Engine):
+    """``TwoElectronRepulsionMatrixOS`` with the ``Symmetry`` of C2H4 (D2h,
+    oriented by ``MoleculeFactory``): the integrals the group forbids are
+    exactly 0.0 (as the reference's ``calculate_integral`` returns them), the
+    others match the oracle, and ``calculate_integral`` agrees; the RHF
+    energy of ``start(symmetry=True)`` equals that without symmetry."""
+    import os
+    from gaussium_amd.basis import read_basis_set_file, read_mol_file
+    from gaussium_amd.hartree_fock import TwoElectronRepulsionMatrixOS
+    from gaussium_amd.main import start
+    from gaussium_amd.symmetry import MoleculeFactory, Symmetry
+    from oracle import eri_c
+    from tests.conftest import ROOT
+    g = os.path.join(ROOT, 'tests', 'golden')
+    nuclei, _, _ = read_mol_file(os.path.join(g, 'C2H4.mol'))
+    nuclei, pg = MoleculeFactory(symmetry=True).create(nuclei)
+    assert pg.label == 'D_{2h}'
+    basis = read_basis_set_file(os.path.join(g, 'STO-3G.gbs'), nuclei)
+    sym = Symmetry(pg, basis)
+    op = TwoElectronRepulsionMatrixOS(basis, symmetry=sym, engine=Engine(basis, nuclei))
+    got = op.create_repulsion_matrix()
+    mask = sym.zero_mask()
+    assert mask.sum() > 0 and np.all(got[mask] == 0.0)
+    ref = eri_c.eri_dense(basis)
+    assert np.abs(got - ref).max() < 1e-10
+    for q in [(0, 1, 2, 3), (1, 1, 5, 7), (2, 4, 6, 8)]:
+        v = op.calculate_integral(*q)
+        assert (v == 0.0) if mask[q] else abs(v - ref[q]) < 1e-10
+    e_sym = start(os.path.join(g, 'C2H4.mol'), os.path.join(g, 'STO-3G.gbs'), 'RHF', 1, symmetry=True)
+    e_c1 = start(os.path.join(g, 'C2H4.mol'), os.path.join(g, 'STO-3G.gbs'), 'RHF', 1)
+    assert abs(e_sym - e_c1) < 1e-9
