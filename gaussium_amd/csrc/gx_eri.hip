@@ -160,9 +160,13 @@ __device__ __forceinline__ void quartet32(const ShellView& sh, const PairView& p
 #ifndef GX_DYN_CHUNK
 #define GX_DYN_CHUNK 1
 #endif
-#ifndef GX_JKGM
-#define GX_JKGM 1  // FP64 group inside mixed builds (few FP64 quartets there)
-#endif
+// FP64 group inside mixed builds: 1, 2 for (ss|ss) (measured on MI355X at
+// the 220-function cluster: (ss|ss) 125 -> 117 us with groups of 2, (ps|ss)
+// 92 -> 98 us).
+template <int LA, int LB, int LC, int LD>
+constexpr int jk_group_mixed64() {
+  return LA + LB + LC + LD == 0 ? 2 : 1;
+}
 
 template <class C, typename R, typename PP, int G, class BG>
 __device__ __forceinline__ void prim_group(R (&acc)[C::NACC], const PP* __restrict__ b, const PP& k,
@@ -536,7 +540,7 @@ __global__ __launch_bounds__(JK_TEAM) __attribute__((amdgpu_waves_per_eu((jk_min
         constexpr int NCP = PC::NC;
         double val[PC::NOUT];
         if (hi)
-          quartet_jk<PC, double, PrimPair64, MIXED ? GX_JKGM : GX_JKG, BG>(bra64, nbp, pp64 + kq.poff, kq.np, ABx,
+          quartet_jk<PC, double, PrimPair64, MIXED ? jk_group_mixed64<LA, LB, LC, LD>() : GX_JKG, BG>(bra64, nbp, pp64 + kq.poff, kq.np, ABx,
                                                                           ABy, ABz, CDx, CDy, CDz, tab64, val);
         if constexpr (MIXED) {
           if (lo) {
