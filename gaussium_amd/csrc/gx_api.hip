@@ -2099,6 +2099,37 @@ int gx_kernel_kind(int la, int lb, int lc, int ld, int gradient) {
   return gx_class_is_coop(la, lb, lc, ld) ? 1 : 0;
 }
 
+int gx_boys_values(gx_ctx* c, int grid, int fp32, int L, const double* T, int n, double* out) {
+  if (!c || (n > 0 && (!T || !out))) return fail(GX_ERR_ARG, "gx_boys_values: null argument");
+  if (grid < 0 || grid > 1 || fp32 < 0 || fp32 > 1 || L < 0 || L > 10 || n < 0)
+    return fail(GX_ERR_ARG, "gx_boys_values: grid 0 or 1, fp32 0 or 1, 0 <= L <= 10, n >= 0");
+  if (n == 0) return GX_OK;
+  GX_HIP(hipSetDevice(c->device));
+  const unsigned g = (unsigned)((n + 255) / 256);
+#define GX_BOYS_CASE(LL)                                                                                  \
+  case LL:                                                                                               \
+    if (grid == 0 && !fp32)                                                                              \
+      hipLaunchKernelGGL((gx::k_boys_values<LL, gx::BoysFine, false>), dim3(g), dim3(256), 0, c->stream, n, \
+                         T, c->d_tab64, c->d_tab32, out);                                               \
+    else if (grid == 0)                                                                                  \
+      hipLaunchKernelGGL((gx::k_boys_values<LL, gx::BoysFine, true>), dim3(g), dim3(256), 0, c->stream, n,  \
+                         T, c->d_tab64, c->d_tab32, out);                                               \
+    else if (!fp32)                                                                                      \
+      hipLaunchKernelGGL((gx::k_boys_values<LL, gx::BoysCoarse, false>), dim3(g), dim3(256), 0, c->stream, \
+                         n, T, c->d_tab64c, c->d_tab32c, out);                                           \
+    else                                                                                                 \
+      hipLaunchKernelGGL((gx::k_boys_values<LL, gx::BoysCoarse, true>), dim3(g), dim3(256), 0, c->stream,  \
+                         n, T, c->d_tab64c, c->d_tab32c, out);                                           \
+    break;
+  switch (L) {
+    GX_BOYS_CASE(0) GX_BOYS_CASE(1) GX_BOYS_CASE(2) GX_BOYS_CASE(3) GX_BOYS_CASE(4) GX_BOYS_CASE(5)
+    GX_BOYS_CASE(6) GX_BOYS_CASE(7) GX_BOYS_CASE(8) GX_BOYS_CASE(9) GX_BOYS_CASE(10)
+  }
+#undef GX_BOYS_CASE
+  GX_HIP(hipGetLastError());
+  return GX_OK;
+}
+
 int gx_one_electron(gx_ctx* c, double* S, double* T, double* V) {
   if (!c || !S || !T) return fail(GX_ERR_ARG, "gx_one_electron: null argument");
   if (c->npair == 0) return fail(GX_ERR_STATE, "gx_one_electron: no basis set");

@@ -773,6 +773,26 @@ __global__ void k_add(long n, const double* __restrict__ x, const double* __rest
   if (t < n) z[t] = x[t] + y[t];
 }
 
+// F_0..F_L at T[i] by the branch-free evaluation of grid BG, in FP64 or
+// FP32 (test entry point gx_boys_values).
+template <int L, class BG, bool F32>
+__global__ void k_boys_values(int n, const double* __restrict__ T, const double* __restrict__ t64,
+                              const float* __restrict__ t32, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if constexpr (F32) {
+    float F[L + 1];
+    boys32_nb<L, BG>((float)T[i], 1.0f, t32 + L * BG::NT * BG::ROW32, F);
+#pragma unroll
+    for (int m = 0; m <= L; ++m) out[(long)i * (L + 1) + m] = (double)F[m];
+  } else {
+    double F[L + 1];
+    boys64_nb<L, BG>(T[i], 1.0, t64 + L * BG::NT * BG::ROW64, F);
+#pragma unroll
+    for (int m = 0; m <= L; ++m) out[(long)i * (L + 1) + m] = F[m];
+  }
+}
+
 // Per-L Taylor tables of grid BG in HBM (from the master table of that
 // grid), the same layout the kernels stage into LDS.
 template <class BG>

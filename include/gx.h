@@ -258,6 +258,15 @@ int gx_jk_launch_times(gx_ctx* ctx, double* out, int cap);
  * arguments.  No counterpart in the reference (measurement). */
 int gx_kernel_kind(int la, int lb, int lc, int ld, int gradient);
 
+/* Boys function values of the device code, for tests: out[i (L + 1) + m] =
+ * F_m(T[i]), m <= L <= 10, from the per-L Taylor tables of the fine
+ * (grid = 0) or coarse (grid = 1) grid by the branch-free evaluation the
+ * J/K kernels use, in FP64 (fp32 = 0) or FP32 (fp32 = 1).  T and out are
+ * device buffers of n and n (L + 1) doubles; runs on the context's stream.
+ * Replaces no call of the reference: it exposes the device counterpart of
+ * integrals/boys_function.py:4-76 to the tests. */
+int gx_boys_values(gx_ctx* ctx, int grid, int fp32, int L, const double* T, int n, double* out);
+
 /* One-electron matrices S, T, V (nbf x nbf FP64 device buffers); V needs
  * gx_set_nuclei first. */
 int gx_one_electron(gx_ctx* ctx, double* S_dev, double* T_dev, double* V_dev);
