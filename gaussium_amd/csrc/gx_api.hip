@@ -482,7 +482,8 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int num_cu, size_t lds, hip
 // Two-electron gradient of one class combination (gx_grad.hip, k_grad_lane;
 // gx_rys.hip, k_grad_rys).
 size_t grad_lds_bytes(int la, int lb, int lc, int ld, int N) {
-  if (gx_grad_is_lane(la, lb, lc, ld)) return (size_t)(ncart(la) + ncart(lb)) * N * sizeof(double);
+  if (gx_grad_is_lane(la, lb, lc, ld))
+    return ((size_t)gx::GRAD_TAB_DOUBLES + (size_t)(ncart(la) + ncart(lb)) * N) * sizeof(double);
 #define GX_CASE(A, B, C, D)                                                                     \
   if (la == A && lb == B && lc == C && ld == D) {                                               \
     using G = EriGradProg<A, B, C, D>;                                                          \
@@ -1085,6 +1086,8 @@ hipError_t launch_grad_combo(gx_ctx* c, int X, int Y, int combo, double* grad, h
   a.kets = c->gr.d_kets[combo];
   a.tab64 = c->d_tab64;
   a.tab32 = c->d_tab32;
+  a.tab64c = c->d_tab64c;
+  a.tab32c = c->d_tab32c;
   a.P = c->d_Ps;
   a.atom = c->d_atom;
   a.sys_atom = c->d_sys_atom;
