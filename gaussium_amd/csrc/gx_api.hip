@@ -194,6 +194,7 @@ struct gx_ctx {
     std::vector<int*> d_items, d_kets;
     std::vector<int> n_items;
     std::vector<double> ipw;  // items per resident workgroup, per combo and mode (FP64, mixed; autotuned)
+    std::vector<double> tms;  // launch time (ms) the autotuner measured, per combo and mode; 0 = not timed
     double cutoff = -1.0, fp32 = -1.0;
     int mixed = -1;
     void release() {
@@ -276,6 +277,8 @@ struct gx_ctx {
     gr.release();
     jk.ipw.clear();
     gr.ipw.clear();
+    jk.tms.clear();
+    gr.tms.clear();
     jk_fp32.clear();
     wg_per_cu.clear();
     wg_per_cu_grad.clear();
@@ -341,6 +344,40 @@ struct gx_ctx {
 namespace {
 
 int combo_count() { return NCLS * (NCLS + 1) / 2; }
+
+// Launch plan of one build: (combo, X, Y, side stream) per class combination
+// with work.  On side streams, once the autotuner has timed every such
+// combination (`tms`, per combo of mode `mode`): longest first, each onto the
+// stream with the least time queued so far, so that the streams end together;
+// otherwise class order, round robin.  Stream -1: the context stream.
+struct PlanStep {
+  int combo, X, Y, side;
+};
+std::vector<PlanStep> launch_plan(const std::vector<int>& n_items, const std::vector<double>& tms, int mode,
+                                  int nside) {
+  std::vector<PlanStep> plan;
+  int combo = 0;
+  for (int X = 0; X < NCLS; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++combo)
+      if (n_items[combo] > 0) plan.push_back({combo, X, Y, -1});
+  auto ms = [&](const PlanStep& p) {
+    const size_t i = (size_t)(p.combo + mode * combo_count());
+    return i < tms.size() ? tms[i] : 0.0;
+  };
+  bool timed = nside > 1;
+  for (const PlanStep& p : plan) timed = timed && ms(p) > 0.0;
+  if (!timed) {
+    for (size_t i = 0; i < plan.size(); ++i) plan[i].side = nside > 0 ? (int)(i % nside) : -1;
+    return plan;
+  }
+  std::stable_sort(plan.begin(), plan.end(), [&](const PlanStep& a, const PlanStep& b) { return ms(a) > ms(b); });
+  std::vector<double> queued(nside, 0.0);
+  for (PlanStep& p : plan) {
+    p.side = (int)(std::min_element(queued.begin(), queued.end()) - queued.begin());
+    queued[p.side] += ms(p);
+  }
+  return plan;
+}
 
 // ---------------------------------------------------------------------------
 // Kernel dispatch by class.  Per-lane classes (EriClass) run one quartet per
@@ -929,6 +966,7 @@ int build_orders(gx_ctx* c, double cutoff, double fp32, int mixed, bool grad = f
   ws.d_kets.assign(combo_count(), nullptr);
   ws.n_items.assign(combo_count(), 0);
   if ((int)ws.ipw.size() != 2 * combo_count()) ws.ipw.assign(2 * combo_count(), GX_ITEMS_PER_WG);
+  if ((int)ws.tms.size() != 2 * combo_count()) ws.tms.assign(2 * combo_count(), 0.0);
   if ((int)c->jk_fp32.size() != combo_count()) c->jk_fp32.assign(combo_count(), 1);
   int combo = 0;
   for (int X = 0; X < NCLS; ++X)
@@ -1653,18 +1691,14 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   // events bracket it alone
   const int nside = timing ? 0 : fork_side(c);
   if (nside < 0) return fail(GX_ERR_HIP, "gx_jk: side streams");
-  int combo = 0, launched = 0;
-  for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo) {
-      if (c->jk.n_items[combo] == 0) continue;
-      if (timing) {
-        GX_HIP(next_events(combo));
-        GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
-      }
-      GX_HIP_FORKED(launch_combo(c, X, Y, combo, mixed, 1, nside ? c->side[launched++ % nside] : c->stream),
-                    nside);
-      if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
+  for (const PlanStep& p : launch_plan(c->jk.n_items, c->jk.tms, mixed, nside)) {
+    if (timing) {
+      GX_HIP(next_events(p.combo));
+      GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
     }
+    GX_HIP_FORKED(launch_combo(c, p.X, p.Y, p.combo, mixed, 1, p.side >= 0 ? c->side[p.side] : c->stream), nside);
+    if (timing) GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1) + 1], c->stream));
+  }
   GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, c->d_Jt, c->d_Jt + nn, c->jkp_n, c->d_jkp_of, c->d_Kt, J, K);
@@ -1762,13 +1796,8 @@ int gx_jk_uhf(gx_ctx* c, const double* Pa, const double* Pb, double* J, double* 
   GX_TRY(build_orders(c, cutoff, 0.0, 0));
   const int nside = fork_side(c);
   if (nside < 0) return fail(GX_ERR_HIP, "gx_jk_uhf: side streams");
-  int combo = 0, launched = 0;
-  for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo) {
-      if (c->jk.n_items[combo] == 0) continue;
-      GX_HIP_FORKED(launch_combo(c, X, Y, combo, 0, 2, nside ? c->side[launched++ % nside] : c->stream),
-                    nside);
-    }
+  for (const PlanStep& p : launch_plan(c->jk.n_items, c->jk.tms, 0, nside))
+    GX_HIP_FORKED(launch_combo(c, p.X, p.Y, p.combo, 0, 2, p.side >= 0 ? c->side[p.side] : c->stream), nside);
   GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
                      c->d_sbfb, c->d_scale, c->d_Jt, c->d_Jt + nn, c->jkp_n, c->d_jkp_of, c->d_Kt, J, Ka);
@@ -1826,6 +1855,7 @@ int gx_jk_autotune(gx_ctx* c, const double* P, double cutoff, double fp32_bound,
       }
       if (rc != GX_OK) break;
       ipw = best;
+      c->jk.tms[combo + mixed * combo_count()] = best_ms;
       if (mixed) c->jk_fp32[combo] = (char)best_fp32;
       rc = build_combo(c, c->jk, X, Y, combo, cutoff, fp32_bound, mixed, false, best);
       double* r = out + 6 * row++;
@@ -1898,24 +1928,20 @@ int gx_grad_ex(gx_ctx* c, const double* P, const double* W, double* grad, double
   // timed calls keep every kernel on the context stream, bracketed by its events
   const int nside = timing ? 0 : fork_side(c);
   if (nside < 0) return fail(GX_ERR_HIP, "gx_grad: side streams");
-  int combo = 0, launched = 0;
-  for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo) {
-      if (c->gr.n_items[combo] == 0) continue;
-      if (timing) {
-        const size_t i = c->gev_combo.size();
-        while (c->gev.size() < 2 * i + 2) {
-          hipEvent_t e;
-          GX_HIP(hipEventCreate(&e));
-          c->gev.push_back(e);
-        }
-        c->gev_combo.push_back(combo);
-        GX_HIP(hipEventRecord(c->gev[2 * i], c->stream));
+  for (const PlanStep& p : launch_plan(c->gr.n_items, c->gr.tms, 0, nside)) {
+    if (timing) {
+      const size_t i = c->gev_combo.size();
+      while (c->gev.size() < 2 * i + 2) {
+        hipEvent_t e;
+        GX_HIP(hipEventCreate(&e));
+        c->gev.push_back(e);
       }
-      GX_HIP_FORKED(launch_grad_combo(c, X, Y, combo, grad, nside ? c->side[launched++ % nside] : c->stream),
-                    nside);
-      if (timing) GX_HIP(hipEventRecord(c->gev[2 * (c->gev_combo.size() - 1) + 1], c->stream));
+      c->gev_combo.push_back(p.combo);
+      GX_HIP(hipEventRecord(c->gev[2 * i], c->stream));
     }
+    GX_HIP_FORKED(launch_grad_combo(c, p.X, p.Y, p.combo, grad, p.side >= 0 ? c->side[p.side] : c->stream), nside);
+    if (timing) GX_HIP(hipEventRecord(c->gev[2 * (c->gev_combo.size() - 1) + 1], c->stream));
+  }
   GX_HIP(join_side(c, nside));
   return GX_OK;
 }
@@ -1990,6 +2016,7 @@ int gx_grad_autotune(gx_ctx* c, const double* P, const double* W, double cutoff,
       }
       if (rc != GX_OK) break;
       ipw = best;
+      c->gr.tms[combo] = best_ms;
       rc = build_combo(c, c->gr, X, Y, combo, cutoff, 0.0, 0, true, best);
       double* r = out + 6 * row++;
       r[0] = class_la(X);

@@ -188,8 +188,11 @@ int gx_jk_uhf(gx_ctx* ctx, const double* Pa_dev, const double* Pb_dev, double* J
  * all its quartets in FP64, and runs so in later mixed builds where that is
  * faster (never less accurate).  Writes one row of 6 doubles per
  * combination: la, lb, lc, ld, chosen value (negated when the combination
- * runs in FP64 in mixed builds), its milliseconds.  Returns the number of
- * rows or an error code.  No counterpart in the reference. */
+ * runs in FP64 in mixed builds), its milliseconds.  The measured times
+ * also order later builds on the side streams: once every combination of a
+ * build has one, the longest is launched first, each onto the side stream
+ * with the least time queued.  Returns the number of rows or an error code.
+ * No counterpart in the reference. */
 int gx_jk_autotune(gx_ctx* ctx, const double* P_dev, double cutoff, double fp32_bound, int flags,
                    const double* candidates, int ncand, int reps, double* out, int cap);
 
@@ -226,7 +229,8 @@ int gx_grad_launch_times(gx_ctx* ctx, double* out, int cap);
 /* Autotuner of the gradient's work-item size per class combination, as
  * gx_jk_autotune (rows of la, lb, lc, ld, chosen value, milliseconds); the
  * choice is kept for later gx_grad calls with this cutoff (until the next
- * gx_set_basis).  Returns the number of rows or an error code.  No
+ * gx_set_basis), and its times order the gradient's side-stream launches
+ * as for gx_jk.  Returns the number of rows or an error code.  No
  * counterpart in the reference. */
 int gx_grad_autotune(gx_ctx* ctx, const double* P_dev, const double* W_dev, double cutoff,
                      const double* candidates, int ncand, int reps, double* out, int cap);

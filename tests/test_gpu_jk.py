@@ -191,6 +191,40 @@ def test_mixed_autotune_precision_choice(Engine):
     assert (J2 - J).abs().max().item() < 1e-11 and (K2 - K).abs().max().item() < 1e-11
 
 
+def test_tuned_launch_plan_keeps_results(Engine):
+    """Once the autotuner has timed every class combination, side-stream
+    builds launch them longest first, each onto the least loaded stream.
+    J/K in both modes, the two-density build and the gradient then equal
+    the one-stream builds of the same inputs (per-launch events keep every
+    kernel on the caller's stream, in class order) and the untuned build of
+    the two-density case."""
+    from gaussium_amd import workloads
+    nuc, basis, ne, info = workloads.cluster_66bf()
+    eng = Engine(basis, nuc)
+    rng = np.random.default_rng(12)
+    P = torch.from_numpy(_random_density(len(basis), rng)).cuda()
+    Pb = torch.from_numpy(_random_density(len(basis), rng)).cuda()
+    U0 = eng.jk_uhf(P, Pb, cutoff=1e-11)
+    for mixed in (False, True):
+        tuned = eng.autotune(P, cutoff=1e-11, fp32_bound=1e-4, mixed=mixed, candidates=(2.0, 8.0), reps=1)
+        assert tuned and all(v[1] > 0.0 for v in tuned.values())
+        J, K = eng.jk(P, cutoff=1e-11, fp32_bound=1e-4, mixed=mixed)
+        Jt, Kt = eng.jk(P, cutoff=1e-11, fp32_bound=1e-4, mixed=mixed, timing=True)
+        eng.jk_launch_times()
+        assert (J - Jt).abs().max().item() < 1e-11 and (K - Kt).abs().max().item() < 1e-11
+    U1 = eng.jk_uhf(P, Pb, cutoff=1e-11)
+    for x, y in zip(U0, U1):
+        assert (x - y).abs().max().item() < 1e-11
+    W = Pb
+    tuned = eng.grad_autotune(P, W, cutoff=1e-11, candidates=(8.0, 64.0), reps=1)
+    assert tuned and all(v[1] > 0.0 for v in tuned.values())
+    g = eng.grad(P, W, cutoff=1e-11)
+    gt = eng.grad(P, W, cutoff=1e-11, timing=True)
+    eng.grad_launch_times()
+    assert (g - gt).abs().max().item() < 1e-10
+    eng.close()
+
+
 def test_side_streams_join_back_to_caller_stream(Engine):
     """Builds forked onto the library's side streams are ordered on the
     caller's stream: back-to-back builds of two densities on a non-default
