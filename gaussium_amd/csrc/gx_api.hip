@@ -1917,17 +1917,17 @@ int gx_grad_ex(gx_ctx* c, const double* P, const double* W, double* grad, double
   if (max_atoms > GX_GRAD_MAXATOM)
     return fail(GX_ERR_ARG, "gx_grad: a system has " + std::to_string(max_atoms) + " nuclei (at most " +
                                 std::to_string(GX_GRAD_MAXATOM) + ")");
-  for (int k = 0; k < NCLS; ++k) {
-    const int p0 = c->cls_start[k], np = c->cls_start[k + 1] - p0;
-    if (np > 0) GX_HIP(launch_one_grad(c, class_la(k), class_lb(k), oa, p0, np, max_atoms, P, W, grad));
-  }
-  hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_atom_sys,
-                     c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
   GX_HIP(hipGetLastError());
   GX_TRY(build_orders(c, cutoff, fp32_bound, mixed, true));
   // timed calls keep every kernel on the context stream, bracketed by its events
   const int nside = timing ? 0 : fork_side(c);
   if (nside < 0) return fail(GX_ERR_HIP, "gx_grad: side streams");
+  // the one-electron terms (atomics into grad) on the context stream, beside
+  // the two-electron kernels on the side streams
+  for (int k = 0; k < NCLS; ++k) {
+    const int p0 = c->cls_start[k], np = c->cls_start[k + 1] - p0;
+    if (np > 0) GX_HIP_FORKED(launch_one_grad(c, class_la(k), class_lb(k), oa, p0, np, max_atoms, P, W, grad), nside);
+  }
   for (const PlanStep& p : launch_plan(c->gr.n_items, c->gr.tms, 0, nside)) {
     if (timing) {
       const size_t i = c->gev_combo.size();
@@ -1943,6 +1943,10 @@ int gx_grad_ex(gx_ctx* c, const double* P, const double* W, double* grad, double
     if (timing) GX_HIP(hipEventRecord(c->gev[2 * (c->gev_combo.size() - 1) + 1], c->stream));
   }
   GX_HIP(join_side(c, nside));
+  // the nuclear repulsion term adds without atomics: after every other term
+  hipLaunchKernelGGL(gx::k_nuc_grad, dim3((c->natom + 63) / 64), dim3(64), 0, c->stream, c->natom, c->d_atom_sys,
+                     c->d_sys_atom, c->d_nxyz, c->d_nZ, grad);
+  GX_HIP(hipGetLastError());
   return GX_OK;
 }
 
