@@ -346,10 +346,11 @@ namespace {
 int combo_count() { return NCLS * (NCLS + 1) / 2; }
 
 // Launch plan of one build: (combo, X, Y, side stream) per class combination
-// with work.  On side streams, once the autotuner has timed every such
+// with work.  With side streams, once the autotuner has timed every such
 // combination (`tms`, per combo of mode `mode`): longest first, each onto the
-// stream with the least time queued so far, so that the streams end together;
-// otherwise class order, round robin.  Stream -1: the context stream.
+// stream with the least time queued so far (the side streams and the context
+// stream), so that the streams end together; otherwise class order, round
+// robin over the side streams.  Stream -1: the context stream.
 struct PlanStep {
   int combo, X, Y, side;
 };
@@ -371,10 +372,11 @@ std::vector<PlanStep> launch_plan(const std::vector<int>& n_items, const std::ve
     return plan;
   }
   std::stable_sort(plan.begin(), plan.end(), [&](const PlanStep& a, const PlanStep& b) { return ms(a) > ms(b); });
-  std::vector<double> queued(nside, 0.0);
+  std::vector<double> queued(nside + 1, 0.0);  // the side streams, then the context stream
   for (PlanStep& p : plan) {
-    p.side = (int)(std::min_element(queued.begin(), queued.end()) - queued.begin());
-    queued[p.side] += ms(p);
+    const int k = (int)(std::min_element(queued.begin(), queued.end()) - queued.begin());
+    queued[k] += ms(p);
+    p.side = k < nside ? k : -1;
   }
   return plan;
 }
