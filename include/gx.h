@@ -191,8 +191,8 @@ int gx_jk_uhf(gx_ctx* ctx, const double* Pa_dev, const double* Pb_dev, double* J
  * runs in FP64 in mixed builds), its milliseconds.  The measured times
  * also order later builds on the side streams: once every combination of a
  * build has one, the longest is launched first, each onto the stream (the
- * side streams or the caller's) with the least time queued.  Returns the number of rows or an error code.
- * No counterpart in the reference. */
+ * side streams or the caller's) with the least time queued.  Returns the
+ * number of rows or an error code.  No counterpart in the reference. */
 int gx_jk_autotune(gx_ctx* ctx, const double* P_dev, double cutoff, double fp32_bound, int flags,
                    const double* candidates, int ncand, int reps, double* out, int cap);
 
