@@ -13,6 +13,7 @@
 
 #include "../../include/gx.h"
 #include "gx_common.h"
+#include "gx_plan.h"
 #include "_gen/eri_gen.h"
 // Single translation unit: the kernels are templates instantiated below.
 #include "gx_eri.hip"
@@ -344,42 +345,6 @@ struct gx_ctx {
 namespace {
 
 int combo_count() { return NCLS * (NCLS + 1) / 2; }
-
-// Launch plan of one build: (combo, X, Y, side stream) per class combination
-// with work.  With side streams, once the autotuner has timed every such
-// combination (`tms`, per combo of mode `mode`): longest first, each onto the
-// stream with the least time queued so far (the side streams and the context
-// stream), so that the streams end together; otherwise class order, round
-// robin over the side streams.  Stream -1: the context stream.
-struct PlanStep {
-  int combo, X, Y, side;
-};
-std::vector<PlanStep> launch_plan(const std::vector<int>& n_items, const std::vector<double>& tms, int mode,
-                                  int nside) {
-  std::vector<PlanStep> plan;
-  int combo = 0;
-  for (int X = 0; X < NCLS; ++X)
-    for (int Y = 0; Y <= X; ++Y, ++combo)
-      if (n_items[combo] > 0) plan.push_back({combo, X, Y, -1});
-  auto ms = [&](const PlanStep& p) {
-    const size_t i = (size_t)(p.combo + mode * combo_count());
-    return i < tms.size() ? tms[i] : 0.0;
-  };
-  bool timed = nside > 1;
-  for (const PlanStep& p : plan) timed = timed && ms(p) > 0.0;
-  if (!timed) {
-    for (size_t i = 0; i < plan.size(); ++i) plan[i].side = nside > 0 ? (int)(i % nside) : -1;
-    return plan;
-  }
-  std::stable_sort(plan.begin(), plan.end(), [&](const PlanStep& a, const PlanStep& b) { return ms(a) > ms(b); });
-  std::vector<double> queued(nside + 1, 0.0);  // the side streams, then the context stream
-  for (PlanStep& p : plan) {
-    const int k = (int)(std::min_element(queued.begin(), queued.end()) - queued.begin());
-    queued[k] += ms(p);
-    p.side = k < nside ? k : -1;
-  }
-  return plan;
-}
 
 // ---------------------------------------------------------------------------
 // Kernel dispatch by class.  Per-lane classes (EriClass) run one quartet per
@@ -1693,7 +1658,7 @@ int gx_jk(gx_ctx* c, const double* P, double* J, double* K, double cutoff, doubl
   // events bracket it alone
   const int nside = timing ? 0 : fork_side(c);
   if (nside < 0) return fail(GX_ERR_HIP, "gx_jk: side streams");
-  for (const PlanStep& p : launch_plan(c->jk.n_items, c->jk.tms, mixed, nside)) {
+  for (const gx::PlanStep& p : gx::launch_plan(NCLS, c->jk.n_items, c->jk.tms, mixed, nside)) {
     if (timing) {
       GX_HIP(next_events(p.combo));
       GX_HIP(hipEventRecord(c->ev[2 * (c->ev_combo.size() - 1)], c->stream));
@@ -1798,7 +1763,7 @@ int gx_jk_uhf(gx_ctx* c, const double* Pa, const double* Pb, double* J, double* 
   GX_TRY(build_orders(c, cutoff, 0.0, 0));
   const int nside = fork_side(c);
   if (nside < 0) return fail(GX_ERR_HIP, "gx_jk_uhf: side streams");
-  for (const PlanStep& p : launch_plan(c->jk.n_items, c->jk.tms, 0, nside))
+  for (const gx::PlanStep& p : gx::launch_plan(NCLS, c->jk.n_items, c->jk.tms, 0, nside))
     GX_HIP_FORKED(launch_combo(c, p.X, p.Y, p.combo, 0, 2, p.side >= 0 ? c->side[p.side] : c->stream), nside);
   GX_HIP(join_side(c, nside));
   hipLaunchKernelGGL(gx::k_symmetrize, dim3(g), dim3(256), 0, c->stream, nn, c->nsys, c->d_smb, c->d_sn,
@@ -1930,7 +1895,7 @@ int gx_grad_ex(gx_ctx* c, const double* P, const double* W, double* grad, double
     const int p0 = c->cls_start[k], np = c->cls_start[k + 1] - p0;
     if (np > 0) GX_HIP_FORKED(launch_one_grad(c, class_la(k), class_lb(k), oa, p0, np, max_atoms, P, W, grad), nside);
   }
-  for (const PlanStep& p : launch_plan(c->gr.n_items, c->gr.tms, 0, nside)) {
+  for (const gx::PlanStep& p : gx::launch_plan(NCLS, c->gr.n_items, c->gr.tms, 0, nside)) {
     if (timing) {
       const size_t i = c->gev_combo.size();
       while (c->gev.size() < 2 * i + 2) {
