@@ -7,6 +7,7 @@ largest integral of the quartet.
 """
 import ctypes
 import os
+import shutil
 import subprocess
 import sys
 
@@ -24,18 +25,20 @@ COMPS = {0: [(0, 0, 0)], 1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
 
 @pytest.fixture(scope='module')
 def host_lib():
-    os.makedirs(BUILD, exist_ok=True)
-    gen = os.path.join(BUILD, 'eri_gen.h')
+    build = os.path.join(BUILD, 'p%d' % os.getpid())  # one per process (parallel test workers)
+    os.makedirs(build, exist_ok=True)
+    gen = os.path.join(build, 'eri_gen.h')
     subprocess.run([sys.executable, os.path.join(ROOT, 'gaussium_amd', 'csrc', 'gen_eri.py'), gen, '2'],
                    check=True)
     sys.path.insert(0, ROOT)
     from gaussium_amd.build import generate_rys
-    rys_dir = generate_rys(os.path.join(BUILD, 'rys'))
-    so = os.path.join(BUILD, 'liberi_host.so')
-    subprocess.run(['g++', '-O1', '-std=c++17', '-shared', '-fPIC', '-I', BUILD, '-I', os.path.dirname(rys_dir),
+    rys_dir = generate_rys(os.path.join(BUILD, 'rys'))  # shared: about 30 s to make, kept while current
+    so = os.path.join(build, 'liberi_host.so')
+    subprocess.run(['g++', '-O1', '-std=c++17', '-shared', '-fPIC', '-I', build, '-I', os.path.dirname(rys_dir),
                     '-I', os.path.join(ROOT, 'gaussium_amd', 'csrc'),
                     os.path.join(HARNESS, 'eri_host.cpp'), '-o', so], check=True)
     lib = ctypes.CDLL(so)
+    shutil.rmtree(build)
     lib.host_quartet.restype = ctypes.c_int
     lib.host_quartet_prog.restype = ctypes.c_int
     lib.host_quartet_grad.restype = ctypes.c_int
